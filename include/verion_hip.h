@@ -1,0 +1,212 @@
+/*
+ * verion_hip.h - C ABI of the MI355X path-tracing backend.
+ *
+ * One-to-one with the Rust backend contract of Verion:
+ *   trait RayTracingBackend   crates/verion-gpu/src/lib.rs:12-58
+ *   trait Accumulation        crates/verion-gpu/src/lib.rs:74-93
+ * A `verion-gpu-hip` crate implements those traits with `extern "C"` calls to the functions below;
+ * the scene reaches the library as the flat tables `pack::pack` already produces
+ * (crates/verion-gpu/src/pack.rs:110-133, 726-756), so the Rust side keeps the one packer.
+ *
+ * Conventions
+ *   - Every call returns a vh_status (0 = success) unless its return type says otherwise; the
+ *     message of the last failure on the calling thread is vh_last_error().
+ *   - Handles nest: an accumulation belongs to a renderer, a renderer to a context. Destroy in
+ *     reverse order. All calls on one context come from one host thread (the reference session is
+ *     not Send: crates/verion-render/src/session.rs:157-158). Nothing calls back.
+ *   - Input tables are copied at vh_renderer_create; the caller may free them afterwards.
+ *   - Output buffers are allocated by the caller; each call documents its size.
+ *   - vh_accumulate blocks until the GPU has finished (lib.rs:50-52).
+ */
+#ifndef VERION_HIP_H
+#define VERION_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error kinds of the Vulkan backend's `Error` enum (crates/verion-gpu-vulkan/src/lib.rs:23-43). */
+typedef enum vh_status {
+    VH_OK = 0,
+    VH_ERR_NO_DEVICE = 1,        /* Error::NoDevice */
+    VH_ERR_UNSUPPORTED = 2,      /* Error::Unsupported{missing} */
+    VH_ERR_OUT_OF_MEMORY = 3,    /* Error::OutOfMemory{needed, available}, raised before allocating */
+    VH_ERR_KERNEL = 4,           /* Error::Shader: a kernel failed to launch or the device reported a fault */
+    VH_ERR_SCENE = 5,            /* Error::Scene: invalid tables, empty scene, mid-accumulation switches */
+    VH_ERR_INVALID_ARGUMENT = 6, /* null handle, zero image size, bad enum value */
+} vh_status;
+
+/* Integrator (lib.rs:97-103). Both give the same image bit for bit. */
+typedef enum vh_integrator {
+    VH_MEGAKERNEL = 0,
+    VH_WAVEFRONT = 1,
+} vh_integrator;
+
+/* pack::SceneInfo (pack.rs:55-69). */
+typedef struct vh_scene_info {
+    uint32_t light_count;  /* lights in the table, the virtual main light included */
+    uint32_t emitter_count;
+    uint32_t env_width;    /* 0 = no environment map */
+    uint32_t env_height;
+    float env_pdf_scale;
+    float sky[3];
+    uint32_t virtual_light;          /* bool */
+    uint32_t transparent_background; /* bool */
+} vh_scene_info;
+
+/* pack::View (pack.rs:72-82): camera[2] = tan(fov/2), camera[4..16] = camera-to-world 3x4 rows. */
+typedef struct vh_view {
+    float camera[16];
+    float aperture;
+    float focus;
+    float shift[2];
+} vh_view;
+
+/* pack::Packed (pack.rs:110-133). Lengths count elements (floats or words), not bytes.
+ * Layouts are those of the bindings in shaders/include/pt_core.slang:12-49. */
+typedef struct vh_scene_tables {
+    const float* positions;    uint64_t positions_len;   /* xyz per vertex */
+    const uint32_t* indices;   uint64_t indices_len;     /* mesh-local vertex indices */
+    const float* uvs;          uint64_t uvs_len;         /* uv per vertex */
+    const float* vnormals;     uint64_t vnormals_len;    /* xyz per vertex, zero when a mesh has none */
+    const uint32_t* instances; uint64_t instances_len;   /* 16 words per instance */
+    const float* materials;    uint64_t materials_len;   /* 24 words per material */
+    const uint32_t* lights;    uint64_t lights_len;      /* 16 per light, albedo tables, link masks, IES */
+    const float* emitters;     uint64_t emitters_len;    /* 16 per emissive triangle */
+    const float* emit_cdf;     uint64_t emit_cdf_len;
+    const float* env_pixels;   uint64_t env_pixels_len;
+    const float* env_marginal; uint64_t env_marginal_len;
+    const float* env_cond;     uint64_t env_cond_len;
+    const float* texels;       uint64_t texels_len;
+    const uint32_t* tex_info;  uint64_t tex_info_len;    /* 4 words per texture */
+    const uint32_t* meshes;    uint64_t mesh_count;      /* pack::MeshRange, 4 words per mesh */
+    const uint32_t* instance_mesh; uint64_t instance_count; /* the TLAS list: mesh index per instance */
+    vh_scene_info info;
+    vh_view view;
+} vh_scene_tables;
+
+/* Adaptive (lib.rs:110-161). */
+typedef struct vh_adaptive {
+    float threshold;
+    uint32_t min_spp;
+} vh_adaptive;
+
+/* Restir (lib.rs:131-149). */
+typedef struct vh_restir {
+    uint32_t di;         /* bool */
+    uint32_t gi;         /* bool; ReSTIR GI is not built: a renderer rejects gi = 1 with VH_ERR_UNSUPPORTED */
+    uint32_t candidates;
+    uint32_t history;
+} vh_restir;
+
+/* MemoryUsage (lib.rs:242-249). */
+typedef struct vh_memory_usage {
+    uint64_t budget;
+    uint64_t allocated;
+    uint64_t uploaded;
+} vh_memory_usage;
+
+/* UpdateReport (lib.rs:65-71). */
+typedef struct vh_update_report {
+    uint64_t blases_built;
+    uint32_t tlas_built;
+    uint64_t bytes_uploaded;
+} vh_update_report;
+
+typedef struct vh_context vh_context;
+typedef struct vh_renderer vh_renderer;
+typedef struct vh_accumulation vh_accumulation;
+
+/* --- errors --------------------------------------------------------------------------------- */
+
+/* Message of the last failure on this thread ("" when none). Valid until the next failing call. */
+const char* vh_last_error(void);
+/* Error::OutOfMemory{needed, available} of the last VH_ERR_OUT_OF_MEMORY on this thread. */
+uint64_t vh_last_error_needed(void);
+uint64_t vh_last_error_available(void);
+
+/* --- context (RayTracingBackend::context / memory, lib.rs:19-20) ------------------------------ */
+
+/* Opens device 0 (the current HIP device). VH_ERR_NO_DEVICE without one, VH_ERR_UNSUPPORTED when
+ * it is not gfx950. */
+int vh_context_create(vh_context** out);
+void vh_context_destroy(vh_context* ctx);
+int vh_memory(const vh_context* ctx, vh_memory_usage* out);
+
+/* --- renderer (RayTracingBackend::new / update / setters, lib.rs:21-45) ----------------------- */
+
+/* Uploads the tables, computes the world-space triangles and builds the LBVH on the GPU
+ * (this replaces the driver BLAS/TLAS of crates/verion-gpu-vulkan/src/accel.rs:153-233).
+ * VH_ERR_SCENE when the scene has no instances (pack.rs:727-729) or a table is inconsistent. */
+int vh_renderer_create(vh_context* ctx, const vh_scene_tables* tables, vh_renderer** out);
+void vh_renderer_destroy(vh_renderer* r);
+/* RayTracingBackend::update (lib.rs:27): replaces every table and rebuilds the LBVH. */
+int vh_renderer_update(vh_renderer* r, const vh_scene_tables* tables, vh_update_report* report);
+/* set_camera (lib.rs:30): the view of pack::View::new. */
+int vh_set_camera(vh_renderer* r, const vh_view* view);
+int vh_set_multiscatter(vh_renderer* r, int on);
+int vh_set_integrator(vh_renderer* r, int integrator);
+int vh_integrator(const vh_renderer* r);
+int vh_set_stage_timing(vh_renderer* r, int on);
+/* NULL switches adaptive sampling off. */
+int vh_set_adaptive(vh_renderer* r, const vh_adaptive* adaptive);
+/* NULL switches ReSTIR off. gi = 1 is VH_ERR_UNSUPPORTED. */
+int vh_set_restir(vh_renderer* r, const vh_restir* restir);
+/* Random-stream seed for accumulations started afterwards. 0 (the default) is the reference's own
+ * stream, pcg(pixel ^ pcg(sample * 0x9E3779B9 + 0x7F4A7C15)) (pt_core.slang:651); any other seed
+ * XORs pcg(seed ^ 0x5851F42D) into the pixel hash. No counterpart in the reference (it has no seed). */
+int vh_set_seed(vh_renderer* r, uint32_t seed);
+
+/* --- accumulation (new_accumulation / accumulate, lib.rs:47-52; trait Accumulation :74-93) ---- */
+
+/* VH_ERR_INVALID_ARGUMENT for a zero size (vulkan renderer.rs:510-512). */
+int vh_accumulation_create(vh_renderer* r, uint32_t width, uint32_t height, int lighting, vh_accumulation** out);
+void vh_accumulation_destroy(vh_accumulation* acc);
+/* Traces `samples` more paths per pixel (at most, under adaptive sampling) and waits for the GPU.
+ * VH_ERR_SCENE when adaptive sampling or ReSTIR was switched since the first sample. */
+int vh_accumulate(vh_renderer* r, vh_accumulation* acc, uint32_t samples);
+
+uint32_t vh_acc_samples(const vh_accumulation* acc);
+double vh_acc_gpu_seconds(const vh_accumulation* acc);
+/* Extend and shade GPU seconds of the wavefront when stage timing was on; VH_ERR_UNSUPPORTED otherwise. */
+int vh_acc_stage_seconds(const vh_accumulation* acc, double out[2]);
+uint64_t vh_acc_wavefront_bytes(const vh_accumulation* acc);
+uint64_t vh_acc_restir_bytes(const vh_accumulation* acc);
+/* Pixels the next sample would trace (readback::active_pixels, readback.rs:51-61). */
+uint64_t vh_acc_active_pixels(const vh_accumulation* acc);
+/* Raw running sums, FP64: 3 doubles per pixel (radiance sum), row-major from the top-left. */
+int vh_acc_read_film(const vh_accumulation* acc, double* out);
+/* Adaptive statistics, FP64: samples, luminance sum, square sum per pixel. VH_ERR_UNSUPPORTED without adaptive. */
+int vh_acc_read_stats(const vh_accumulation* acc, double* out);
+/* Lighting split sums, 65 floats per pixel (pt_core.slang:39-47). VH_ERR_UNSUPPORTED without lighting. */
+int vh_acc_read_lighting(const vh_accumulation* acc, float* out);
+/* Accumulation::mean (readback.rs:18-38): 3 floats per pixel. */
+int vh_acc_mean(const vh_accumulation* acc, float* out);
+
+/* --- primary-hit AOVs (render_normals / render_aovs, lib.rs:54-57) ---------------------------- */
+
+/* 3 floats per pixel. */
+int vh_render_normals(vh_renderer* r, uint32_t width, uint32_t height, float* normals);
+/* albedo_depth: 4 floats per pixel; object_id, material_id: 1 word; coverage: 1 float. */
+int vh_render_aovs(vh_renderer* r, uint32_t width, uint32_t height, float* albedo_depth,
+                   uint32_t* object_id, uint32_t* material_id, float* coverage);
+
+/* --- debug exports (no counterpart in the reference) --------------------------------------- */
+
+/* Closest hit of the camera ray of sample `sample` for every pixel (generatePathFor,
+ * pt_core.slang:645-682, then traceClosest :684-700): hit flag, instance, primitive, t bits per
+ * pixel (4 words). Used by the bit-exact primary-hit gate. */
+int vh_primary_hits(vh_renderer* r, uint32_t width, uint32_t height, uint32_t sample, uint32_t* hits);
+/* Closest hits of `n` arbitrary rays (org xyz, dir xyz per ray; 4 words out per ray). */
+int vh_trace_rays(vh_renderer* r, uint64_t n, const float* rays, float tmax, uint32_t* hits);
+/* LBVH shape: nodes, leaves, maximum depth, triangles. */
+int vh_bvh_stats(const vh_renderer* r, uint64_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VERION_HIP_H */

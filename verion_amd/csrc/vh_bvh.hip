@@ -1,0 +1,589 @@
+// vh_bvh.hip - LBVH build on the GPU, replacing the driver BLAS/TLAS builds of
+// crates/verion-gpu-vulkan/src/accel.rs:153-233 (and crates/verion-gpu-metal/src/accel.rs:72-167).
+//
+// Steps, all on the device:
+//   1. world-space triangles: every instance's mesh triangles through its 3x4 transform, with the
+//      same float expression as pt_core.slang `transformPoint` (:90-99), so traversal and shading see
+//      the same vertices; the global triangle id orders (instance, primitive);
+//   2. scene bounds (block reduction + ordered-integer atomics);
+//   3. 30-bit Morton codes of the triangle centroids;
+//   4. LSD radix sort of (code, id) pairs, 8 bits per pass: per-tile LDS histograms, one scan, and a
+//      stable scatter that ranks keys within each wave with ballots and across waves through LDS;
+//   5. the binary hierarchy of Karras ("Maximizing Parallelism in the Construction of BVHs, Octrees,
+//      and k-d Trees", HPG 2012), with the key index as the tie-break of equal codes;
+//   6. node boxes bottom-up, the second child to arrive at a parent computes it (atomic counter);
+//   7. collapse to 4-wide nodes of 128 bytes, level by level: each wide node opens its largest
+//      binary children until it holds four, and a binary subtree of at most kLeafMax triangles
+//      becomes a leaf (the triangles of an LBVH subtree are contiguous in key order).
+#include "vh_internal.h"
+
+namespace vh {
+
+namespace {
+
+constexpr uint32_t kLeafMax = 4;
+constexpr uint32_t kRsBlock = 256;
+constexpr uint32_t kRsItems = 8;
+constexpr uint32_t kRsTile = kRsBlock * kRsItems;
+
+__device__ __forceinline__ uint32_t ord(float f)
+{
+    uint32_t u = asu(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ __forceinline__ float unord(uint32_t u) { return asf((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u); }
+
+// 1. world triangles, in global id order: 3 float4 (p0, gid) (p1, instance) (p2, primitive)
+__global__ __launch_bounds__(256) void k_world(const float* __restrict__ positions, const uint32_t* __restrict__ indices,
+                                               const uint32_t* __restrict__ instances, const uint32_t* __restrict__ instStart,
+                                               uint32_t ninst, uint32_t ntri, float4* __restrict__ out)
+{
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ntri; g += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = ninst - 1; // last instance whose start <= g
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) / 2;
+            if (instStart[mid] <= g)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        uint32_t inst = lo;
+        uint32_t prim = g - instStart[inst];
+        uint32_t base = inst * 16;
+        uint32_t tri = instances[base + 1] + 3 * prim;
+        uint32_t vo = instances[base];
+        V3 p[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            p[k] = transformPoint(instances, base + 4, vtx(positions, indices[tri + k] + vo));
+        out[3 * (size_t)g] = make_float4(p[0].x, p[0].y, p[0].z, asf(g));
+        out[3 * (size_t)g + 1] = make_float4(p[1].x, p[1].y, p[1].z, asf(inst));
+        out[3 * (size_t)g + 2] = make_float4(p[2].x, p[2].y, p[2].z, asf(prim));
+    }
+}
+
+// 2. bounds of all vertices: bnd[0..2] min (ordered), bnd[3..5] max (ordered), bnd[6] max |coord| (ordered)
+__global__ __launch_bounds__(256) void k_bounds(const float4* __restrict__ tris, uint32_t ntri, uint32_t* __restrict__ bnd)
+{
+    float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+    float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ntri; g += gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            float4 v = tris[3 * (size_t)g + k];
+            lo[0] = fminf(lo[0], v.x);
+            lo[1] = fminf(lo[1], v.y);
+            lo[2] = fminf(lo[2], v.z);
+            hi[0] = fmaxf(hi[0], v.x);
+            hi[1] = fmaxf(hi[1], v.y);
+            hi[2] = fmaxf(hi[2], v.z);
+        }
+    }
+    __shared__ float s[6][256];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        s[k][threadIdx.x] = lo[k];
+        s[3 + k][threadIdx.x] = hi[k];
+    }
+    __syncthreads();
+    for (uint32_t st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                s[k][threadIdx.x] = fminf(s[k][threadIdx.x], s[k][threadIdx.x + st]);
+                s[3 + k][threadIdx.x] = fmaxf(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + st]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        float m = 0.0f;
+        for (int k = 0; k < 3; k++) {
+            atomicMin(&bnd[k], ord(s[k][0]));
+            atomicMax(&bnd[3 + k], ord(s[3 + k][0]));
+            m = fmaxf(m, fmaxf(fabsf(s[k][0]), fabsf(s[3 + k][0])));
+        }
+        atomicMax(&bnd[6], ord(m));
+    }
+}
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v)
+{
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+// 3. Morton codes of the centroids in the scene box
+__global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ tris, uint32_t ntri, const uint32_t* __restrict__ bnd,
+                                                uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    float lo[3] = {unord(bnd[0]), unord(bnd[1]), unord(bnd[2])};
+    float ext[3];
+    for (int k = 0; k < 3; k++)
+        ext[k] = fmaxf(unord(bnd[3 + k]) - lo[k], 1.0e-30f);
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ntri; g += gridDim.x * blockDim.x) {
+        float4 a = tris[3 * (size_t)g], b = tris[3 * (size_t)g + 1], c = tris[3 * (size_t)g + 2];
+        float cx = (a.x + b.x + c.x) * (1.0f / 3.0f), cy = (a.y + b.y + c.y) * (1.0f / 3.0f), cz = (a.z + b.z + c.z) * (1.0f / 3.0f);
+        uint32_t qx = (uint32_t)fminf(fmaxf((cx - lo[0]) / ext[0] * 1024.0f, 0.0f), 1023.0f);
+        uint32_t qy = (uint32_t)fminf(fmaxf((cy - lo[1]) / ext[1] * 1024.0f, 0.0f), 1023.0f);
+        uint32_t qz = (uint32_t)fminf(fmaxf((cz - lo[2]) / ext[2] * 1024.0f, 0.0f), 1023.0f);
+        keys[g] = (expand10(qx) << 2) | (expand10(qy) << 1) | expand10(qz);
+        vals[g] = g;
+    }
+}
+
+// 4a. per-tile digit histograms, stored digit-major: hist[d * tiles + tile]
+__global__ __launch_bounds__(kRsBlock) void k_rs_hist(const uint32_t* __restrict__ keys, uint32_t n, uint32_t shift,
+                                                      uint32_t* __restrict__ hist, uint32_t tiles)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * kRsTile;
+#pragma unroll
+    for (uint32_t r = 0; r < kRsItems; r++) {
+        uint32_t i = base + r * kRsBlock + threadIdx.x;
+        if (i < n)
+            atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// 4b. exclusive scan of the histogram array in one block
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ a, uint32_t n)
+{
+    __shared__ uint32_t s[1024];
+    uint32_t per = (n + 1023) / 1024;
+    uint32_t b = threadIdx.x * per, e = umin(b + per, n);
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < e; i++)
+        sum += a[i];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - sum;
+    for (uint32_t i = b; i < e; i++) {
+        uint32_t v = a[i];
+        a[i] = run;
+        run += v;
+    }
+}
+
+// 4c. stable scatter: ranks within a wave from eight ballots, across the block's waves through LDS
+__global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                         uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
+                                                         uint32_t shift, const uint32_t* __restrict__ hist, uint32_t tiles)
+{
+    constexpr uint32_t kWaves = kRsBlock / 64;
+    __shared__ uint32_t off[256];
+    __shared__ uint32_t wcnt[kWaves][256];
+    off[threadIdx.x] = hist[threadIdx.x * tiles + blockIdx.x];
+    uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64u - lane));
+    uint32_t base = blockIdx.x * kRsTile;
+    for (uint32_t r = 0; r < kRsItems; r++) {
+        for (uint32_t w = 0; w < kWaves; w++)
+            wcnt[w][threadIdx.x] = 0;
+        __syncthreads();
+        uint32_t i = base + r * kRsBlock + threadIdx.x;
+        bool valid = i < n;
+        uint32_t key = valid ? kin[i] : 0u;
+        uint32_t digit = (key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (uint32_t b = 0; b < 8; b++) {
+            uint64_t bb = __ballot((digit >> b) & 1u);
+            peers &= ((digit >> b) & 1u) ? bb : ~bb;
+        }
+        uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0)
+            wcnt[wave][digit] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = off[digit] + rank;
+            for (uint32_t w = 0; w < wave; w++)
+                pos += wcnt[w][digit];
+            kout[pos] = key;
+            vout[pos] = vin[i];
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (uint32_t w = 0; w < kWaves; w++)
+            add += wcnt[w][threadIdx.x];
+        off[threadIdx.x] += add;
+        __syncthreads();
+    }
+}
+
+// 5. Karras hierarchy over the sorted keys. Inner node i has children encoded as index | 0x80000000
+// for a leaf; parents of inner nodes in parent[0..n-2], of leaves in parent[n-1 .. 2n-2].
+__device__ __forceinline__ int delta(const uint32_t* keys, int n, int i, int j)
+{
+    if (j < 0 || j >= n)
+        return -1;
+    uint32_t a = keys[i], b = keys[j];
+    if (a == b)
+        return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+    return __clz(a ^ b);
+}
+
+__global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ keys, int n, uint2* __restrict__ child,
+                                                uint2* __restrict__ range, uint32_t* __restrict__ parent)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n - 1; i += gridDim.x * blockDim.x) {
+        int d = delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1) > 0 ? 1 : -1;
+        int dmin = delta(keys, n, i, i - d);
+        int lmax = 2;
+        while (delta(keys, n, i, i + lmax * d) > dmin)
+            lmax <<= 1;
+        int l = 0;
+        for (int t = lmax >> 1; t > 0; t >>= 1)
+            if (delta(keys, n, i, i + (l + t) * d) > dmin)
+                l += t;
+        int j = i + l * d;
+        int dnode = delta(keys, n, i, j);
+        int s = 0;
+        int t = l;
+        do {
+            t = (t + 1) >> 1;
+            if (delta(keys, n, i, i + (s + t) * d) > dnode)
+                s += t;
+        } while (t > 1);
+        int split = i + s * d + (d < 0 ? -1 : 0);
+        int first = i < j ? i : j, last = i < j ? j : i;
+        uint32_t left = (first == split) ? (0x80000000u | (uint32_t)split) : (uint32_t)split;
+        uint32_t right = (last == split + 1) ? (0x80000000u | (uint32_t)(split + 1)) : (uint32_t)(split + 1);
+        child[i] = make_uint2(left, right);
+        range[i] = make_uint2((uint32_t)first, (uint32_t)last);
+        parent[(left & 0x80000000u) ? (n - 1 + (left & 0x7fffffffu)) : left] = (uint32_t)i;
+        parent[(right & 0x80000000u) ? (n - 1 + (right & 0x7fffffffu)) : right] = (uint32_t)i;
+    }
+}
+
+// leaf-ordered triangles: sorted[k] = world[vals[k]]
+__global__ __launch_bounds__(256) void k_gather(const float4* __restrict__ world, const uint32_t* __restrict__ vals, uint32_t n,
+                                                float4* __restrict__ sorted)
+{
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        uint32_t g = vals[k];
+        sorted[3 * (size_t)k] = world[3 * (size_t)g];
+        sorted[3 * (size_t)k + 1] = world[3 * (size_t)g + 1];
+        sorted[3 * (size_t)k + 2] = world[3 * (size_t)g + 2];
+    }
+}
+
+__device__ __forceinline__ void triBox(const float4* tris, uint32_t k, float lo[3], float hi[3])
+{
+    float4 a = tris[3 * (size_t)k], b = tris[3 * (size_t)k + 1], c = tris[3 * (size_t)k + 2];
+    lo[0] = fminf(a.x, fminf(b.x, c.x));
+    lo[1] = fminf(a.y, fminf(b.y, c.y));
+    lo[2] = fminf(a.z, fminf(b.z, c.z));
+    hi[0] = fmaxf(a.x, fmaxf(b.x, c.x));
+    hi[1] = fmaxf(a.y, fmaxf(b.y, c.y));
+    hi[2] = fmaxf(a.z, fmaxf(b.z, c.z));
+}
+
+// 6. boxes of inner nodes, bottom-up; box[i] = (lo.xyz, -), (hi.xyz, -)
+__global__ __launch_bounds__(256) void k_boxes(const float4* __restrict__ tris, int n, const uint2* __restrict__ child,
+                                               const uint32_t* __restrict__ parent, uint32_t* __restrict__ flags,
+                                               float4* __restrict__ box)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        uint32_t node = parent[n - 1 + k];
+        while (true) {
+            __threadfence();
+            if (atomicAdd(&flags[node], 1u) == 0)
+                break; // the sibling is not done yet; it will carry on
+            __threadfence();
+            float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+            float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+            uint2 c = child[node];
+            uint32_t cs[2] = {c.x, c.y};
+            for (int s = 0; s < 2; s++) {
+                float l[3], h[3];
+                if (cs[s] & 0x80000000u) {
+                    triBox(tris, cs[s] & 0x7fffffffu, l, h);
+                } else {
+                    float4 bl = __builtin_nontemporal_load(&box[2 * cs[s]]);
+                    float4 bh = __builtin_nontemporal_load(&box[2 * cs[s] + 1]);
+                    l[0] = bl.x; l[1] = bl.y; l[2] = bl.z;
+                    h[0] = bh.x; h[1] = bh.y; h[2] = bh.z;
+                }
+                for (int a = 0; a < 3; a++) {
+                    lo[a] = fminf(lo[a], l[a]);
+                    hi[a] = fmaxf(hi[a], h[a]);
+                }
+            }
+            box[2 * node] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+            box[2 * node + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+            if (node == 0)
+                break;
+            node = parent[node];
+        }
+    }
+}
+
+__device__ __forceinline__ float area(const float lo[3], const float hi[3])
+{
+    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// binary node reference (leaf flag in the top bit) -> first, count, box
+__device__ __forceinline__ void refInfo(uint32_t ref, const float4* tris, const uint2* range, const float4* box, uint32_t& first,
+                                        uint32_t& count, float lo[3], float hi[3])
+{
+    if (ref & 0x80000000u) {
+        first = ref & 0x7fffffffu;
+        count = 1;
+        triBox(tris, first, lo, hi);
+    } else {
+        uint2 r = range[ref];
+        first = r.x;
+        count = r.y - r.x + 1;
+        float4 bl = box[2 * ref], bh = box[2 * ref + 1];
+        lo[0] = bl.x; lo[1] = bl.y; lo[2] = bl.z;
+        hi[0] = bh.x; hi[1] = bh.y; hi[2] = bh.z;
+    }
+}
+
+// 7. one level of the collapse: items (binary inner node, wide node id) -> wide nodes + next level
+__global__ __launch_bounds__(256) void k_collapse(const float4* __restrict__ tris, const uint2* __restrict__ child,
+                                                  const uint2* __restrict__ range, const float4* __restrict__ box,
+                                                  const uint2* __restrict__ qin, const uint32_t* __restrict__ qinCount,
+                                                  uint2* __restrict__ qout, uint32_t* __restrict__ qoutCount,
+                                                  uint32_t* __restrict__ wideCount, float4* __restrict__ nodes, float pad)
+{
+    uint32_t total = *qinCount;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+        uint2 item = qin[q];
+        uint32_t refs[4];
+        uint32_t nref = 2;
+        uint2 c = child[item.x];
+        refs[0] = c.x;
+        refs[1] = c.y;
+        // open the largest inner child until four slots are used
+        while (nref < 4) {
+            int best = -1;
+            float bestA = -1.0f;
+            for (uint32_t k = 0; k < nref; k++) {
+                if (refs[k] & 0x80000000u)
+                    continue;
+                float4 bl = box[2 * refs[k]], bh = box[2 * refs[k] + 1];
+                float lo[3] = {bl.x, bl.y, bl.z}, hi[3] = {bh.x, bh.y, bh.z};
+                float a = area(lo, hi);
+                if (a > bestA) {
+                    bestA = a;
+                    best = (int)k;
+                }
+            }
+            if (best < 0)
+                break;
+            uint2 cc = child[refs[best]];
+            refs[best] = cc.x;
+            refs[nref++] = cc.y;
+        }
+        float lo[3][4], hi[3][4];
+        uint32_t cref[4], ccount[4];
+        for (uint32_t k = 0; k < 4; k++) {
+            if (k >= nref) {
+                for (int a = 0; a < 3; a++) {
+                    lo[a][k] = __builtin_inff();
+                    hi[a][k] = -__builtin_inff();
+                }
+                cref[k] = 0;
+                ccount[k] = 0;
+                continue;
+            }
+            uint32_t first, count;
+            float l[3], h[3];
+            refInfo(refs[k], tris, range, box, first, count, l, h);
+            for (int a = 0; a < 3; a++) {
+                lo[a][k] = l[a] - pad;
+                hi[a][k] = h[a] + pad;
+            }
+            if (count <= kLeafMax) {
+                cref[k] = first;
+                ccount[k] = count;
+            } else {
+                uint32_t id = atomicAdd(wideCount, 1u);
+                uint32_t slot = atomicAdd(qoutCount, 1u);
+                qout[slot] = make_uint2(refs[k], id);
+                cref[k] = id;
+                ccount[k] = 0;
+            }
+        }
+        float4* nd = nodes + 8 * (size_t)item.y;
+        nd[0] = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+        nd[1] = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+        nd[2] = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+        nd[3] = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+        nd[4] = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+        nd[5] = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+        nd[6] = make_float4(asf(cref[0]), asf(cref[1]), asf(cref[2]), asf(cref[3]));
+        nd[7] = make_float4(asf(ccount[0]), asf(ccount[1]), asf(ccount[2]), asf(ccount[3]));
+    }
+}
+
+// single-leaf root for scenes of at most kLeafMax triangles
+__global__ void k_root_leaf(const float4* __restrict__ tris, uint32_t n, float4* __restrict__ nodes, float pad)
+{
+    float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+    float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    for (uint32_t k = 0; k < n; k++) {
+        float l[3], h[3];
+        triBox(tris, k, l, h);
+        for (int a = 0; a < 3; a++) {
+            lo[a] = fminf(lo[a], l[a]);
+            hi[a] = fmaxf(hi[a], h[a]);
+        }
+    }
+    float inf = __builtin_inff();
+    nodes[0] = make_float4(lo[0] - pad, inf, inf, inf);
+    nodes[1] = make_float4(hi[0] + pad, -inf, -inf, -inf);
+    nodes[2] = make_float4(lo[1] - pad, inf, inf, inf);
+    nodes[3] = make_float4(hi[1] + pad, -inf, -inf, -inf);
+    nodes[4] = make_float4(lo[2] - pad, inf, inf, inf);
+    nodes[5] = make_float4(hi[2] + pad, -inf, -inf, -inf);
+    nodes[6] = make_float4(asf(0u), asf(0u), asf(0u), asf(0u));
+    nodes[7] = make_float4(asf(n), asf(0u), asf(0u), asf(0u));
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 8192)
+{
+    uint64_t g = (n + block - 1) / block;
+    return (uint32_t)(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+} // namespace
+
+#define VH_CHECK(x)                                                                                                         \
+    do {                                                                                                                    \
+        hipError_t e_ = (x);                                                                                                \
+        if (e_ != hipSuccess)                                                                                               \
+            return fail(VH_ERR_KERNEL, std::string(#x) + ": " + hipGetErrorString(e_));                                     \
+    } while (0)
+
+int build_bvh(BvhBuild& out, const DeviceTables& T, hipStream_t s)
+{
+    uint32_t ninst = T.instanceCount;
+    uint32_t n = T.triangleCount;
+    out.ntri = n;
+    out.nodes = 0;
+    out.depth = 0;
+    out.pad = 0.0f;
+    if (n == 0)
+        return VH_OK;
+    DevBuf world, keysA, valsA, keysB, valsB, hist, bnd, child, range, parent, flags, box, qa, qb, qc, wide;
+    VH_CHECK(world.alloc(sizeof(float4) * 3 * (size_t)n));
+    VH_CHECK(out.tris.alloc(sizeof(float4) * 3 * (size_t)n));
+    k_world<<<grid_for(n, 256), 256, 0, s>>>(T.positions, T.indices, T.instances, T.instStart, ninst, n, world.as<float4>());
+    VH_CHECK(hipGetLastError());
+    VH_CHECK(bnd.alloc(8 * sizeof(uint32_t)));
+    uint32_t init[8] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u, 0u, 0u};
+    VH_CHECK(hipMemcpyAsync(bnd.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+    k_bounds<<<grid_for(n, 256, 1024), 256, 0, s>>>(world.as<float4>(), n, bnd.as<uint32_t>());
+    VH_CHECK(hipGetLastError());
+    VH_CHECK(keysA.alloc(4 * (size_t)n));
+    VH_CHECK(valsA.alloc(4 * (size_t)n));
+    VH_CHECK(keysB.alloc(4 * (size_t)n));
+    VH_CHECK(valsB.alloc(4 * (size_t)n));
+    k_morton<<<grid_for(n, 256), 256, 0, s>>>(world.as<float4>(), n, bnd.as<uint32_t>(), keysA.as<uint32_t>(), valsA.as<uint32_t>());
+    VH_CHECK(hipGetLastError());
+    uint32_t host_bnd[8];
+    VH_CHECK(hipMemcpyAsync(host_bnd, bnd.p, sizeof(host_bnd), hipMemcpyDeviceToHost, s));
+
+    uint32_t tiles = (n + kRsTile - 1) / kRsTile;
+    VH_CHECK(hist.alloc(4 * 256 * (size_t)tiles));
+    uint32_t *kin = keysA.as<uint32_t>(), *vin = valsA.as<uint32_t>(), *kout = keysB.as<uint32_t>(), *vout = valsB.as<uint32_t>();
+    for (uint32_t shift = 0; shift < 32; shift += 8) {
+        k_rs_hist<<<tiles, kRsBlock, 0, s>>>(kin, n, shift, hist.as<uint32_t>(), tiles);
+        k_rs_scan<<<1, 1024, 0, s>>>(hist.as<uint32_t>(), 256 * tiles);
+        k_rs_scatter<<<tiles, kRsBlock, 0, s>>>(kin, vin, kout, vout, n, shift, hist.as<uint32_t>(), tiles);
+        VH_CHECK(hipGetLastError());
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    // after four passes the sorted pairs are back in keysA/valsA
+    k_gather<<<grid_for(n, 256), 256, 0, s>>>(world.as<float4>(), vin, n, out.tris.as<float4>());
+    VH_CHECK(hipGetLastError());
+    VH_CHECK(hipStreamSynchronize(s));
+    float scale = unord(host_bnd[6]);
+    out.pad = scale * 1.0e-5f + 1.0e-6f;
+
+    if (n <= kLeafMax) {
+        VH_CHECK(out.nodes_buf.alloc(sizeof(float4) * 8));
+        k_root_leaf<<<1, 1, 0, s>>>(out.tris.as<float4>(), n, out.nodes_buf.as<float4>(), out.pad);
+        VH_CHECK(hipGetLastError());
+        VH_CHECK(hipStreamSynchronize(s));
+        out.nodes = 1;
+        out.depth = 1;
+        out.leaves = 1;
+        return VH_OK;
+    }
+    VH_CHECK(child.alloc(sizeof(uint2) * (n - 1)));
+    VH_CHECK(range.alloc(sizeof(uint2) * (n - 1)));
+    VH_CHECK(parent.alloc(4 * (size_t)(2 * n - 1)));
+    VH_CHECK(flags.alloc(4 * (size_t)(n - 1)));
+    VH_CHECK(box.alloc(sizeof(float4) * 2 * (size_t)(n - 1)));
+    k_fill_u32<<<grid_for(n - 1, 256), 256, 0, s>>>(flags.as<uint32_t>(), n - 1, 0u);
+    k_karras<<<grid_for(n - 1, 256), 256, 0, s>>>(kin, (int)n, child.as<uint2>(), range.as<uint2>(), parent.as<uint32_t>());
+    k_boxes<<<grid_for(n, 256), 256, 0, s>>>(out.tris.as<float4>(), (int)n, child.as<uint2>(), parent.as<uint32_t>(),
+                                             flags.as<uint32_t>(), box.as<float4>());
+    VH_CHECK(hipGetLastError());
+
+    // level-by-level collapse; wide nodes are at most n - 1 (each holds at least two children)
+    uint32_t maxWide = n;
+    VH_CHECK(wide.alloc(sizeof(float4) * 8 * (size_t)maxWide));
+    VH_CHECK(qa.alloc(sizeof(uint2) * (size_t)maxWide));
+    VH_CHECK(qb.alloc(sizeof(uint2) * (size_t)maxWide));
+    VH_CHECK(qc.alloc(4 * 4)); // [0] count in, [1] count out, [2] wide count
+    uint32_t c0[4] = {1u, 0u, 1u, 0u};
+    uint2 root = make_uint2(0u, 0u);
+    VH_CHECK(hipMemcpyAsync(qc.p, c0, sizeof(c0), hipMemcpyHostToDevice, s));
+    VH_CHECK(hipMemcpyAsync(qa.p, &root, sizeof(root), hipMemcpyHostToDevice, s));
+    uint2 *qin = qa.as<uint2>(), *qout = qb.as<uint2>();
+    uint32_t* cnt = qc.as<uint32_t>();
+    uint32_t level = 0;
+    uint32_t items = 1;
+    while (items > 0) {
+        level++;
+        k_fill_u32<<<1, 1, 0, s>>>(cnt + 1, 1, 0u);
+        k_collapse<<<grid_for(items, 256), 256, 0, s>>>(out.tris.as<float4>(), child.as<uint2>(), range.as<uint2>(), box.as<float4>(),
+                                                       qin, cnt, qout, cnt + 1, cnt + 2, wide.as<float4>(), out.pad);
+        VH_CHECK(hipGetLastError());
+        uint32_t hc[4];
+        VH_CHECK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
+        VH_CHECK(hipStreamSynchronize(s));
+        items = hc[1];
+        // the next level's input count
+        VH_CHECK(hipMemcpyAsync(cnt, cnt + 1, sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+        std::swap(qin, qout);
+        out.nodes = hc[2];
+        if (level > 200)
+            return fail(VH_ERR_SCENE, "BVH collapse did not terminate");
+    }
+    out.depth = level;
+    // keep only the used nodes
+    VH_CHECK(out.nodes_buf.alloc(sizeof(float4) * 8 * (size_t)out.nodes));
+    VH_CHECK(hipMemcpyAsync(out.nodes_buf.p, wide.p, sizeof(float4) * 8 * (size_t)out.nodes, hipMemcpyDeviceToDevice, s));
+    VH_CHECK(hipStreamSynchronize(s));
+    return VH_OK;
+}
+
+} // namespace vh

@@ -1,0 +1,320 @@
+// vh_render.hip - the path-tracing kernels: megakernel, wavefront stages, debug ray queries.
+//
+// Megakernel: shaders/pathtrace.slang:9-25, one thread per pixel, params[35] samples in a row.
+// Wavefront: shaders/wf_select.slang, wf_args.slang, wf_extend.slang, wf_shade.slang with
+// include/wf_state.slang, re-shaped for CDNA:
+//   - path state is indexed by queue slot, not by pixel: every stage reads and writes whole
+//     contiguous runs of each state word (the reference indexes state by pixel and scatters);
+//     two copies ping-pong between depths and the pixel id travels as state word 16;
+//   - compaction of the survivors is one ballot per wave, an LDS prefix over the block's waves and
+//     one global atomic per block (the reference appends with one atomic per path, wf_shade.slang:24);
+//   - instead of an indirect dispatch sized by a one-thread kernel (wf_args.slang), the extend and
+//     shade kernels are grid-stride loops over the live count of their depth, read on the device,
+//     so the host never waits between bounces and no launch depends on a count.
+// Both integrators call the same device functions (vh_shade.h), so their images are identical.
+#include "vh_internal.h"
+
+namespace vh {
+
+namespace {
+
+constexpr uint32_t kWfBlock = 256;
+constexpr uint32_t kBaseWords = 17;  // 16 words of wf_state.slang:75 + the pixel id
+constexpr uint32_t kSplitWords = 17; // wf_state.slang:12 (WF_SPLIT)
+
+template <bool RESTIR, bool SPLIT>
+__global__ __launch_bounds__(64) void k_pathtrace(Scene S, Frame F, Params P, uint32_t samples)
+{
+    uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7u);
+    uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
+    if (x >= P.w[0] || y >= P.w[1])
+        return;
+    Ctx c{S, F, P};
+    uint32_t pixel = y * P.w[0] + x;
+    for (uint32_t k = 0; k < samples; k++) {
+        if (P.w[32] != 0 && !pixelActive(c, pixel))
+            return;
+        PathState st;
+        generatePath(P, x, y, P.w[6] + k, st);
+        for (uint32_t depth = 0; depth < P.w[7]; depth++) {
+            Hit h = trace<false>(S.bvh, st.org, st.dir, 1.0e30f);
+            if (!shadePath<RESTIR, SPLIT>(c, st, h, depth, pixel))
+                break;
+        }
+        finishPath<SPLIT>(c, st, pixel);
+    }
+}
+
+__device__ __forceinline__ void put(uint32_t* s, uint32_t n, uint32_t f, uint32_t i, float v) { s[(size_t)f * n + i] = asu(v); }
+__device__ __forceinline__ float getf(const uint32_t* s, uint32_t n, uint32_t f, uint32_t i) { return asf(s[(size_t)f * n + i]); }
+__device__ __forceinline__ void put3(uint32_t* s, uint32_t n, uint32_t f, uint32_t i, V3 v)
+{
+    put(s, n, f, i, v.x);
+    put(s, n, f + 1, i, v.y);
+    put(s, n, f + 2, i, v.z);
+}
+__device__ __forceinline__ V3 get3(const uint32_t* s, uint32_t n, uint32_t f, uint32_t i)
+{
+    return v3(getf(s, n, f, i), getf(s, n, f + 1, i), getf(s, n, f + 2, i));
+}
+
+// storePath / loadPath (wf_state.slang:35-92), slot-indexed
+template <bool SPLIT>
+__device__ __forceinline__ void storePath(uint32_t* s, uint32_t n, uint32_t i, const PathState& p, uint32_t pixel)
+{
+    put3(s, n, 0, i, p.org);
+    put3(s, n, 3, i, p.dir);
+    put3(s, n, 6, i, p.thr);
+    s[(size_t)9 * n + i] = p.rng;
+    s[(size_t)10 * n + i] = p.cls | ((uint32_t)(p.chan + 1) << 2) | (p.misPrev ? 16u : 0u) | (p.restir ? 32u : 0u) |
+                            (p.gi ? 64u : 0u) | (p.prevInst << 8);
+    put(s, n, 11, i, p.pathLen);
+    put(s, n, 12, i, p.qPrev);
+    put3(s, n, 13, i, p.parts.total);
+    s[(size_t)16 * n + i] = pixel;
+    if (SPLIT) {
+        put3(s, n, kBaseWords, i, p.parts.directDiffuse);
+        put3(s, n, kBaseWords + 3, i, p.parts.indirectDiffuse);
+        put3(s, n, kBaseWords + 6, i, p.parts.specular);
+        put3(s, n, kBaseWords + 9, i, p.parts.transmission);
+        put3(s, n, kBaseWords + 12, i, p.parts.emission);
+        put(s, n, kBaseWords + 15, i, p.visLit);
+        put(s, n, kBaseWords + 16, i, p.visTotal);
+    }
+}
+
+template <bool SPLIT>
+__device__ __forceinline__ void loadPath(const uint32_t* s, uint32_t n, uint32_t i, PathState& p, uint32_t& pixel)
+{
+    p.org = get3(s, n, 0, i);
+    p.dir = get3(s, n, 3, i);
+    p.thr = get3(s, n, 6, i);
+    p.rng = s[(size_t)9 * n + i];
+    uint32_t flags = s[(size_t)10 * n + i];
+    p.cls = flags & 3u;
+    p.chan = (int)((flags >> 2) & 3u) - 1;
+    p.misPrev = (flags & 16u) != 0;
+    p.restir = (flags & 32u) != 0;
+    p.gi = (flags & 64u) != 0;
+    p.prevInst = flags >> 8;
+    p.pathLen = getf(s, n, 11, i);
+    p.qPrev = getf(s, n, 12, i);
+    p.parts = Parts{get3(s, n, 13, i), v3(0.0f), v3(0.0f), v3(0.0f), v3(0.0f), v3(0.0f)};
+    pixel = s[(size_t)16 * n + i];
+    p.visLit = 0.0f;
+    p.visTotal = 0.0f;
+    if (SPLIT) {
+        p.parts.directDiffuse = get3(s, n, kBaseWords, i);
+        p.parts.indirectDiffuse = get3(s, n, kBaseWords + 3, i);
+        p.parts.specular = get3(s, n, kBaseWords + 6, i);
+        p.parts.transmission = get3(s, n, kBaseWords + 9, i);
+        p.parts.emission = get3(s, n, kBaseWords + 12, i);
+        p.visLit = getf(s, n, kBaseWords + 15, i);
+        p.visTotal = getf(s, n, kBaseWords + 16, i);
+    }
+}
+
+// wf_args at depth 0 (wf_args.slang:12-17): the depth-0 count and empty later queues
+__global__ void k_wf_reset(uint32_t* counts, uint32_t n, uint32_t adaptive, uint32_t maxDepth)
+{
+    if (threadIdx.x == 0) {
+        if (!adaptive)
+            counts[0] = n;
+        for (uint32_t k = 1; k <= maxDepth; k++)
+            counts[k] = 0;
+    }
+}
+
+// Block-wide compaction: returns this thread's slot in the queue `counter` counts, for `keep` threads.
+__device__ __forceinline__ uint32_t compact(bool keep, uint32_t* counter)
+{
+    __shared__ uint32_t s_wave[kWfBlock / 64];
+    __shared__ uint32_t s_base;
+    uint64_t m = __ballot(keep);
+    uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint64_t lt = lane == 0 ? 0ull : ((~0ull) >> (64u - lane));
+    uint32_t rank = (uint32_t)__popcll(m & lt);
+    if (lane == 0)
+        s_wave[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t w = 0; w < kWfBlock / 64; w++) {
+            uint32_t c = s_wave[w];
+            s_wave[w] = run;
+            run += c;
+        }
+        s_base = run ? atomicAdd(counter, run) : 0u;
+    }
+    __syncthreads();
+    uint32_t slot = s_base + s_wave[wave] + rank;
+    __syncthreads(); // s_wave and s_base are reused by the next call
+    return slot;
+}
+
+// wf_select (wf_select.slang:7-17): pixels that still take samples -> depth-0 queue
+__global__ __launch_bounds__(kWfBlock) void k_wf_select(Frame F, Params P, Wavefront W)
+{
+    Scene S{};
+    Ctx c{S, F, P};
+    uint32_t n = W.n;
+    for (uint32_t base = blockIdx.x * kWfBlock; base < n; base += gridDim.x * kWfBlock) {
+        uint32_t p = base + threadIdx.x;
+        bool active = p < n && pixelActive(c, p);
+        uint32_t slot = compact(active, &W.counts[0]);
+        if (active)
+            W.queue0[slot] = p;
+    }
+}
+
+// wf_extend (wf_extend.slang:6-29): closest hit of every live path at depth d
+__global__ __launch_bounds__(kWfBlock) void k_wf_extend(Scene S, Params P, Wavefront W, uint32_t d)
+{
+    uint32_t count = W.counts[d];
+    const uint32_t* st = W.state[d & 1u];
+    for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < count; i += gridDim.x * kWfBlock) {
+        V3 org, dir;
+        if (d == 0) {
+            uint32_t p = P.w[32] != 0 ? W.queue0[i] : i;
+            PathState s;
+            generatePath(P, p % P.w[0], p / P.w[0], P.w[6], s);
+            org = s.org;
+            dir = s.dir;
+        } else {
+            org = get3(st, W.n, 0, i);
+            dir = get3(st, W.n, 3, i);
+        }
+        Hit h = trace<false>(S.bvh, org, dir, 1.0e30f);
+        W.hits[i] = make_uint4(h.hit ? 1u : 0u, h.inst, h.prim, asu(h.t));
+    }
+}
+
+// wf_shade (wf_shade.slang:7-28): shade, then append the survivor to the next depth or finish it
+template <bool RESTIR, bool SPLIT>
+__global__ __launch_bounds__(kWfBlock) void k_wf_shade(Scene S, Frame F, Params P, Wavefront W, uint32_t d)
+{
+    Ctx c{S, F, P};
+    uint32_t count = W.counts[d];
+    const uint32_t* in = W.state[d & 1u];
+    uint32_t* out = W.state[(d + 1) & 1u];
+    for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += gridDim.x * kWfBlock) {
+        uint32_t i = base + threadIdx.x;
+        bool keep = false;
+        PathState st;
+        uint32_t pixel = 0;
+        if (i < count) {
+            if (d == 0) {
+                pixel = P.w[32] != 0 ? W.queue0[i] : i;
+                generatePath(P, pixel % P.w[0], pixel / P.w[0], P.w[6], st);
+            } else {
+                loadPath<SPLIT>(in, W.n, i, st, pixel);
+            }
+            uint4 hw = W.hits[i];
+            Hit h;
+            h.hit = hw.x != 0;
+            h.inst = hw.y;
+            h.prim = hw.z;
+            h.t = asf(hw.w);
+            h.b1 = h.b2 = 0.0f;
+            h.gid = 0;
+            keep = shadePath<RESTIR, SPLIT>(c, st, h, d, pixel) && d + 1 < P.w[7];
+            if (!keep)
+                finishPath<SPLIT>(c, st, pixel);
+        }
+        uint32_t slot = compact(keep, &W.counts[d + 1]);
+        if (keep)
+            storePath<SPLIT>(out, W.n, slot, st, pixel);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_primary(Scene S, Params P, uint4* out)
+{
+    uint32_t n = P.w[0] * P.w[1];
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        PathState s;
+        generatePath(P, p % P.w[0], p / P.w[0], P.w[6], s);
+        Hit h = trace<false>(S.bvh, s.org, s.dir, 1.0e30f);
+        out[p] = h.hit ? make_uint4(1u, h.inst, h.prim, asu(h.t)) : make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rays(Scene S, const float* rays, uint64_t n, float tmax, uint4* out)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const float* r = rays + 6 * i;
+        Hit h = trace<false>(S.bvh, v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), tmax);
+        out[i] = h.hit ? make_uint4(1u, h.inst, h.prim, asu(h.t)) : make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
+uint32_t blocks_for(uint64_t n, uint32_t block, uint32_t cap)
+{
+    uint64_t g = (n + block - 1) / block;
+    return (uint32_t)(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+} // namespace
+
+int launch_megakernel(const Scene& S, const Frame& F, const Params& P, uint32_t samples, bool restir, bool split, hipStream_t s)
+{
+    dim3 grid((P.w[0] + 7) / 8, (P.w[1] + 7) / 8);
+    if (restir && split)
+        k_pathtrace<true, true><<<grid, 64, 0, s>>>(S, F, P, samples);
+    else if (restir)
+        k_pathtrace<true, false><<<grid, 64, 0, s>>>(S, F, P, samples);
+    else if (split)
+        k_pathtrace<false, true><<<grid, 64, 0, s>>>(S, F, P, samples);
+    else
+        k_pathtrace<false, false><<<grid, 64, 0, s>>>(S, F, P, samples);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("pathtrace launch: ") + hipGetErrorString(e));
+}
+
+int launch_wf_sample(const Scene& S, const Frame& F, const Params& P0, const Wavefront& W, bool restir, bool split, const LaunchCfg& cfg,
+                     hipEvent_t* ev)
+{
+    hipStream_t s = cfg.stream;
+    uint32_t grid = cfg.wfGrid ? cfg.wfGrid : 1024;
+    Params P = P0;
+    uint32_t maxDepth = P.w[7];
+    k_wf_reset<<<1, 64, 0, s>>>(W.counts, W.n, P.w[32], maxDepth);
+    if (P.w[32] != 0)
+        k_wf_select<<<blocks_for(W.n, kWfBlock, grid), kWfBlock, 0, s>>>(F, P, W);
+    for (uint32_t d = 0; d < maxDepth; d++) {
+        P.w[30] = d;
+        if (ev)
+            (void)hipEventRecord(ev[4 * d], s);
+        k_wf_extend<<<grid, kWfBlock, 0, s>>>(S, P, W, d);
+        if (ev)
+            (void)hipEventRecord(ev[4 * d + 1], s);
+        if (restir && split)
+            k_wf_shade<true, true><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+        else if (restir)
+            k_wf_shade<true, false><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+        else if (split)
+            k_wf_shade<false, true><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+        else
+            k_wf_shade<false, false><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+        if (ev)
+            (void)hipEventRecord(ev[4 * d + 2], s);
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("wavefront launch: ") + hipGetErrorString(e));
+}
+
+int launch_primary_hits(const Scene& S, const Params& P, uint4* out, hipStream_t s)
+{
+    uint64_t n = (uint64_t)P.w[0] * P.w[1];
+    k_primary<<<blocks_for(n, 256, 8192), 256, 0, s>>>(S, P, out);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("primary launch: ") + hipGetErrorString(e));
+}
+
+int launch_trace_rays(const Scene& S, const float* rays, uint64_t n, float tmax, uint4* out, hipStream_t s)
+{
+    k_rays<<<blocks_for(n, 256, 8192), 256, 0, s>>>(S, rays, n, tmax, out);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("ray launch: ") + hipGetErrorString(e));
+}
+
+} // namespace vh

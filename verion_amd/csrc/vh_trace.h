@@ -1,0 +1,215 @@
+// vh_trace.h - ray queries over the LBVH: closest hit and shadow (any) hit.
+//
+// Replaces the driver ray queries of pt_core.slang `traceClosest` (:684-700) and `occluded`
+// (:580-613). The BVH is 4-wide: a node is 128 bytes (one L2 line of MI355X), eight 16-byte words:
+//   0..5  child boxes as SoA: lo.x[4], hi.x[4], lo.y[4], hi.y[4], lo.z[4], hi.z[4]
+//   6     child references: a node index, or for a leaf the first triangle in leaf order
+//   7     triangle counts (0 = inner node; empty slots have an inverted box and never hit)
+// Triangles are stored in leaf order, 48 bytes each: (p0, global id) (p1, instance) (p2, primitive).
+// Boxes are padded at build time, so the slab test is conservative; the triangle test is the
+// watertight one of Woop, Benthin and Wald (JCGT 2013) without culling, and the closest hit is the
+// smallest t with ties to the smallest global triangle id (= smallest (instance, primitive)),
+// which makes the answer independent of the tree.
+#pragma once
+
+#include "vh_math.h"
+
+namespace vh {
+
+struct Hit {
+    bool hit;
+    uint32_t inst, prim;
+    float t;
+    float b1, b2; // barycentric weights of vertices 1 and 2 (for opacity textures)
+    uint32_t gid;
+};
+
+struct Bvh {
+    const float4* nodes; // 8 float4 per node; node 0 is the root
+    const float4* tris;  // 3 float4 per triangle, in leaf order
+    uint32_t ntri;
+};
+
+// Watertight ray/triangle test. Inputs are the world-space vertices; on a hit with 0 < t < tmax
+// it returns t and the weights of vertices 1 and 2.
+struct RayPre {
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+
+VH_HD RayPre ray_pre(V3 d)
+{
+    float dv[3] = {d.x, d.y, d.z};
+    float ad[3] = {fabsf(d.x), fabsf(d.y), fabsf(d.z)};
+    RayPre r;
+    r.kz = ad[0] > ad[1] ? (ad[0] > ad[2] ? 0 : 2) : (ad[1] > ad[2] ? 1 : 2);
+    r.kx = r.kz + 1 == 3 ? 0 : r.kz + 1;
+    r.ky = r.kx + 1 == 3 ? 0 : r.kx + 1;
+    if (dv[r.kz] < 0.0f) {
+        int t = r.kx;
+        r.kx = r.ky;
+        r.ky = t;
+    }
+    r.sx = dv[r.kx] / dv[r.kz];
+    r.sy = dv[r.ky] / dv[r.kz];
+    r.sz = 1.0f / dv[r.kz];
+    return r;
+}
+
+VH_HD float pick(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+VH_HD bool intersect(const RayPre& r, V3 o, V3 p0, V3 p1, V3 p2, float tmax, float& tout, float& b1, float& b2)
+{
+    V3 A = p0 - o, B = p1 - o, C = p2 - o;
+    float Akz = pick(A, r.kz), Bkz = pick(B, r.kz), Ckz = pick(C, r.kz);
+    float ax = pick(A, r.kx) - r.sx * Akz, ay = pick(A, r.ky) - r.sy * Akz;
+    float bx = pick(B, r.kx) - r.sx * Bkz, by = pick(B, r.ky) - r.sy * Bkz;
+    float cx = pick(C, r.kx) - r.sx * Ckz, cy = pick(C, r.ky) - r.sy * Ckz;
+    float U = cx * by - cy * bx;
+    float V = ax * cy - ay * cx;
+    float W = bx * ay - by * ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)cx * (double)by - (double)cy * (double)bx);
+        V = (float)((double)ax * (double)cy - (double)ay * (double)cx);
+        W = (float)((double)bx * (double)ay - (double)by * (double)ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f))
+        return false;
+    float det = U + V + W;
+    if (det == 0.0f)
+        return false;
+    float T = U * (r.sz * Akz) + V * (r.sz * Bkz) + W * (r.sz * Ckz);
+    float t = T / det;
+    if (!(t > 0.0f && t < tmax))
+        return false;
+    tout = t;
+    b1 = V / det;
+    b2 = W / det;
+    return true;
+}
+
+__device__ __forceinline__ V3 xyz(float4 a) { return v3(a.x, a.y, a.z); }
+
+constexpr int kStack = 64;
+
+// Box test of the four children: returns a 4-bit mask of hits and their entry distances.
+__device__ __forceinline__ uint32_t slab4(const float4* n, float ix, float iy, float iz, float ox, float oy, float oz,
+                                          float tmax, float tn[4])
+{
+    float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
+    float a[6][4] = {{lx.x, lx.y, lx.z, lx.w}, {hx.x, hx.y, hx.z, hx.w}, {ly.x, ly.y, ly.z, ly.w},
+                     {hy.x, hy.y, hy.z, hy.w}, {lz.x, lz.y, lz.z, lz.w}, {hz.x, hz.y, hz.z, hz.w}};
+    uint32_t mask = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        float x0 = fmaf(a[0][c], ix, ox), x1 = fmaf(a[1][c], ix, ox);
+        float y0 = fmaf(a[2][c], iy, oy), y1 = fmaf(a[3][c], iy, oy);
+        float z0 = fmaf(a[4][c], iz, oz), z1 = fmaf(a[5][c], iz, oz);
+        float t0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+        float t1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax)) * 1.0000004f;
+        tn[c] = t0;
+        mask |= (t0 <= t1 ? 1u : 0u) << c;
+    }
+    return mask;
+}
+
+__device__ __forceinline__ float safe_inv(float d)
+{
+    return 1.0f / (fabsf(d) > 1.0e-30f ? d : (d < 0.0f ? -1.0e-30f : 1.0e-30f));
+}
+
+// Closest hit (any = false) or the first hit found (any = true) over (0, tmax).
+template <bool ANY>
+__device__ Hit trace(const Bvh& bvh, V3 o, V3 d, float tmax)
+{
+    Hit h;
+    h.hit = false;
+    h.inst = h.prim = h.gid = 0;
+    h.t = 0.0f;
+    h.b1 = h.b2 = 0.0f;
+    if (bvh.ntri == 0)
+        return h;
+    RayPre pre = ray_pre(d);
+    float ix = safe_inv(d.x), iy = safe_inv(d.y), iz = safe_inv(d.z);
+    float ox = -o.x * ix, oy = -o.y * iy, oz = -o.z * iz;
+    float best = tmax; // closest accepted t (or tmax)
+    uint32_t stackN[kStack];
+    float stackT[kStack];
+    int sp = 0;
+    uint32_t node = 0;
+    while (true) {
+        const float4* n = bvh.nodes + 8 * (size_t)node;
+        float tn[4];
+        uint32_t mask = slab4(n, ix, iy, iz, ox, oy, oz, best, tn);
+        uint4 child = *reinterpret_cast<const uint4*>(n + 6);
+        uint4 count = *reinterpret_cast<const uint4*>(n + 7);
+        uint32_t ch[4] = {child.x, child.y, child.z, child.w};
+        uint32_t cn[4] = {count.x, count.y, count.z, count.w};
+        // leaves first, nearest first; inner nodes pushed farthest first
+        uint32_t inner[4];
+        float innerT[4];
+        int ni = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            if (!(mask >> c & 1u))
+                continue;
+            if (cn[c] == 0) {
+                inner[ni] = ch[c];
+                innerT[ni] = tn[c];
+                ni++;
+            } else if (!(h.hit && tn[c] > h.t)) {
+                for (uint32_t k = 0; k < cn[c]; k++) {
+                    const float4* tr = bvh.tris + 3 * (size_t)(ch[c] + k);
+                    float4 a = tr[0], b = tr[1], cc = tr[2];
+                    uint32_t gid = asu(a.w);
+                    float t, b1, b2;
+                    float lim = h.hit ? __builtin_nextafterf(h.t, __builtin_inff()) : tmax;
+                    if (intersect(pre, o, xyz(a), xyz(b), xyz(cc), lim, t, b1, b2) && t < tmax &&
+                        (!h.hit || t < h.t || (t == h.t && gid < h.gid))) {
+                        h.hit = true;
+                        h.t = t;
+                        h.b1 = b1;
+                        h.b2 = b2;
+                        h.gid = gid;
+                        h.inst = asu(b.w);
+                        h.prim = asu(cc.w);
+                        best = t;
+                        if (ANY)
+                            return h;
+                    }
+                }
+            }
+        }
+        // sort the inner children by entry distance (descending) and push; the nearest is visited next
+        for (int i = 1; i < ni; i++) {
+            for (int j = i; j > 0 && innerT[j - 1] < innerT[j]; j--) {
+                float tt = innerT[j];
+                innerT[j] = innerT[j - 1];
+                innerT[j - 1] = tt;
+                uint32_t nn = inner[j];
+                inner[j] = inner[j - 1];
+                inner[j - 1] = nn;
+            }
+        }
+        for (int i = 0; i < ni; i++) {
+            stackN[sp] = inner[i];
+            stackT[sp] = innerT[i];
+            sp++;
+        }
+        // pop the next node that can still hold a closer hit
+        bool found = false;
+        while (sp > 0) {
+            sp--;
+            if (h.hit && stackT[sp] > h.t)
+                continue;
+            node = stackN[sp];
+            found = true;
+            break;
+        }
+        if (!found)
+            break;
+    }
+    return h;
+}
+
+} // namespace vh

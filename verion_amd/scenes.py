@@ -1,0 +1,198 @@
+"""Seeded generators of the benchmark scenes, plus the small closed-form scenes of the reference tests.
+
+None of these scenes exists in the reference; they are specified by the workload list this backend
+is measured on:
+  cornell(): a Cornell box in a 1 m cube, 5 walls + 2 boxes + a 0.25 m^2 quad light of radiance
+             (17, 12, 4), Lambert red (.63, .065, .05), green (.14, .45, .09), white .73 (~36 triangles);
+  procedural(): 1,024 level-3 icospheres (1,280 triangles each, 1,310,720 in all) at uniform
+             positions in a 20 m cube, lognormal radii (mu 0, sigma 0.5), GGX roughness U[0.05, 1],
+             metallic Bernoulli(0.3), one sun and a constant environment map (importance sampled);
+  many_lights(): ground and box clutter with emissive triangles of lognormal emission.
+The closed-form scenes restate crates/verion-gpu-metal/tests/pathtrace.rs: `cube` (:38-58),
+`plane` (:61-70), `furnace_room` (:87-98), the white furnace (:118-137).
+"""
+from __future__ import annotations
+
+import math
+from typing import Dict, Tuple
+
+import numpy as np
+
+from . import scene as sc
+
+
+def cube_mesh(half: float) -> sc.Mesh:
+    """Axis-aligned cube of 12 triangles (tests/pathtrace.rs:38-58)."""
+    h = half
+    pos = [[-h, -h, -h], [h, -h, -h], [h, h, -h], [-h, h, -h], [-h, -h, h], [h, -h, h], [h, h, h], [-h, h, h]]
+    idx = [0, 1, 2, 0, 2, 3, 4, 5, 6, 4, 6, 7, 0, 1, 5, 0, 5, 4, 2, 3, 7, 2, 7, 6, 1, 2, 6, 1, 6, 5, 0, 3, 7, 0, 7, 4]
+    return sc.Mesh(np.array(pos, np.float32), np.array(idx, np.uint32))
+
+
+def plane_mesh(half: float, z: float) -> sc.Mesh:
+    """Square in the plane z (tests/pathtrace.rs:61-70)."""
+    h = half
+    return sc.Mesh(np.array([[-h, -h, z], [h, -h, z], [h, h, z], [-h, h, z]], np.float32),
+                   np.array([0, 1, 2, 0, 2, 3], np.uint32))
+
+
+def camera_at_z(z: float) -> sc.Camera:
+    return sc.Camera(transform=((1.0, 0.0, 0.0, 0.0), (0.0, 1.0, 0.0, 0.0), (0.0, 0.0, 1.0, z)))
+
+
+def furnace_room(rho: float, e: float) -> sc.Scene:
+    """Camera inside a closed cube whose walls emit e and reflect rho (tests/pathtrace.rs:87-98)."""
+    s = sc.Scene()
+    m = s.add_mesh(cube_mesh(1.0))
+    mat = s.add_material(sc.lambert((rho, rho, rho), (e, e, e)))
+    s.add_instance(sc.Instance(m, mat))
+    return s
+
+
+def white_furnace(material: sc.BaseBsdf = None, sky: float = 1.0) -> sc.Scene:
+    """Convex object under a uniform sky (tests/pathtrace.rs:118-137); albedo-1 Lambert by default."""
+    s = sc.Scene()
+    m = s.add_mesh(cube_mesh(0.5))
+    mat = s.add_material(material if material is not None else sc.lambert((1.0, 1.0, 1.0)))
+    s.add_instance(sc.Instance(m, mat))
+    s.add_light(sc.Environment((sky, sky, sky)))
+    s.camera = camera_at_z(3.0)
+    return s
+
+
+def _quad(p0, p1, p2, p3):
+    return [p0, p1, p2, p3], [0, 1, 2, 0, 2, 3]
+
+
+def _box(cx: float, cz: float, w: float, h: float, angle: float):
+    """Closed box of 12 triangles standing on y = 0, rotated about y."""
+    c, s_ = math.cos(angle), math.sin(angle)
+    corners = []
+    for y in (0.0, h):
+        for dx, dz in ((-1, -1), (1, -1), (1, 1), (-1, 1)):
+            x, z = dx * w / 2, dz * w / 2
+            corners.append([cx + c * x + s_ * z, y, cz - s_ * x + c * z])
+    idx = [0, 2, 1, 0, 3, 2, 4, 5, 6, 4, 6, 7, 0, 1, 5, 0, 5, 4, 1, 2, 6, 1, 6, 5, 2, 3, 7, 2, 7, 6, 3, 0, 4, 3, 4, 7]
+    return corners, idx
+
+
+def cornell() -> sc.Scene:
+    """The Cornell box workload: 36 triangles in a 1 m cube, camera on the open side."""
+    s = sc.Scene()
+    white = s.add_material(sc.lambert((0.73, 0.73, 0.73)))
+    red = s.add_material(sc.lambert((0.63, 0.065, 0.05)))
+    green = s.add_material(sc.lambert((0.14, 0.45, 0.09)))
+    light = s.add_material(sc.lambert((0.0, 0.0, 0.0), (17.0, 12.0, 4.0)))
+
+    def add(verts, idx, mat):
+        m = s.add_mesh(sc.Mesh(np.array(verts, np.float32), np.array(idx, np.uint32)))
+        s.add_instance(sc.Instance(m, mat))
+
+    add(*_quad([0, 0, 0], [1, 0, 0], [1, 0, 1], [0, 0, 1]), white)      # floor
+    add(*_quad([0, 1, 0], [0, 1, 1], [1, 1, 1], [1, 1, 0]), white)      # ceiling
+    add(*_quad([0, 0, 0], [0, 1, 0], [1, 1, 0], [1, 0, 0]), white)      # back wall
+    add(*_quad([0, 0, 0], [0, 0, 1], [0, 1, 1], [0, 1, 0]), red)        # left wall
+    add(*_quad([1, 0, 0], [1, 1, 0], [1, 1, 1], [1, 0, 1]), green)      # right wall
+    add(*_box(0.68, 0.62, 0.3, 0.3, -0.29), white)                       # short box
+    add(*_box(0.32, 0.35, 0.3, 0.6, 0.29), white)                        # tall box
+    add(*_quad([0.25, 0.999, 0.25], [0.25, 0.999, 0.75], [0.75, 0.999, 0.75], [0.75, 0.999, 0.25]), light)
+    s.camera = sc.Camera(transform=((1.0, 0.0, 0.0, 0.5), (0.0, 1.0, 0.0, 0.5), (0.0, 0.0, 1.0, 2.2)))
+    return s
+
+
+def icosphere(level: int) -> sc.Mesh:
+    """Unit icosphere by midpoint subdivision: 20 * 4^level triangles, smooth vertex normals."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+             [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]]
+    verts = [list(np.array(v, np.float64) / np.linalg.norm(v)) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        cache: Dict[Tuple[int, int], int] = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = (np.array(verts[a]) + np.array(verts[b])) / 2.0
+                verts.append(list(m / np.linalg.norm(m)))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    v = np.array(verts, np.float32)
+    return sc.Mesh(v, np.array(faces, np.uint32).reshape(-1), normals=v.copy())
+
+
+def procedural(spheres: int = 1024, seed: int = 1, level: int = 3) -> sc.Scene:
+    """The procedural scale workload: instanced icospheres in a 20 m cube, a sun and a constant map."""
+    rng = np.random.default_rng(seed)
+    s = sc.Scene()
+    mesh = s.add_mesh(icosphere(level))
+    centres = rng.uniform(-10.0, 10.0, size=(spheres, 3))
+    radii = rng.lognormal(0.0, 0.5, size=spheres)
+    rough = rng.uniform(0.05, 1.0, size=spheres)
+    metal = rng.random(spheres) < 0.3
+    colour = rng.uniform(0.2, 0.9, size=(spheres, 3))
+    for k in range(spheres):
+        mat = s.add_material(sc.BaseBsdf(base_color=tuple(float(c) for c in colour[k]),
+                                         metallic=1.0 if metal[k] else 0.0, roughness=float(rough[k])))
+        r = float(radii[k])
+        cx, cy, cz = (float(c) for c in centres[k])
+        s.add_instance(sc.Instance(mesh, mat, ((r, 0.0, 0.0, cx), (0.0, r, 0.0, cy), (0.0, 0.0, r, cz))))
+    s.add_light(sc.DirectionalLight(direction=(-0.4, -1.0, -0.3), color=(1.0, 0.95, 0.85), intensity=3.0))
+    env = s.add_image(sc.Image(32, 16, np.tile(np.array([[0.35, 0.45, 0.6]], np.float32), (32 * 16, 1))))
+    s.add_light(sc.EnvironmentMap(env, 1.0))
+    s.camera = sc.Camera(transform=((1.0, 0.0, 0.0, 0.0), (0.0, 1.0, 0.0, 0.0), (0.0, 0.0, 1.0, 34.0)))
+    return s
+
+
+def many_lights(emitters: int = 10000, clutter_boxes: int = 41000, seed: int = 1) -> sc.Scene:
+    """Many-lights workload: a ground plane and box clutter (12 triangles per box) with `emitters`
+    emissive triangles of lognormal(0, 1) emission, all seeded."""
+    rng = np.random.default_rng(seed)
+    s = sc.Scene()
+    ground = s.add_mesh(sc.Mesh(np.array([[-30, 0, -30], [30, 0, -30], [30, 0, 30], [-30, 0, 30]], np.float32),
+                                np.array([0, 2, 1, 0, 3, 2], np.uint32)))
+    gm = s.add_material(sc.BaseBsdf(base_color=(0.6, 0.6, 0.6), roughness=0.6))
+    s.add_instance(sc.Instance(ground, gm))
+    box = s.add_mesh(cube_mesh(0.5))
+    bm = s.add_material(sc.BaseBsdf(base_color=(0.5, 0.5, 0.55), roughness=0.4))
+    # clutter: one mesh of many boxes (non-instanced) so the triangle count is the scene's
+    centres = rng.uniform(-25, 25, size=(clutter_boxes, 2))
+    sizes = rng.uniform(0.1, 0.8, size=(clutter_boxes, 3))
+    cube = cube_mesh(0.5)
+    pos = []
+    idx = []
+    for k in range(clutter_boxes):
+        p = cube.positions * sizes[k].astype(np.float32) + np.array([centres[k, 0], sizes[k, 1] / 2, centres[k, 1]], np.float32)
+        idx.append(cube.indices + 8 * k)
+        pos.append(p)
+    clutter = s.add_mesh(sc.Mesh(np.concatenate(pos), np.concatenate(idx)))
+    s.add_instance(sc.Instance(clutter, bm))
+    del box
+    # emissive triangles: small panels floating above the clutter
+    tri = []
+    for k in range(emitters):
+        c = np.array([rng.uniform(-25, 25), rng.uniform(1.5, 4.0), rng.uniform(-25, 25)])
+        a = rng.uniform(0, 2 * math.pi)
+        e1 = np.array([math.cos(a), 0, math.sin(a)]) * 0.3
+        e2 = np.array([-math.sin(a), -0.2, math.cos(a)]) * 0.3
+        tri.append([c, c + e1, c + e2])
+    emission = rng.lognormal(0.0, 1.0, size=emitters)
+    # group emitters into 16 materials by emission quantile so the table stays small
+    order = np.argsort(emission)
+    groups = np.array_split(order, 16)
+    for g in groups:
+        lev = float(np.mean(emission[g]))
+        mat = s.add_material(sc.lambert((0.0, 0.0, 0.0), (lev, lev * 0.9, lev * 0.8)))
+        verts = np.array([tri[k] for k in g], np.float32).reshape(-1, 3)
+        m = s.add_mesh(sc.Mesh(verts, np.arange(verts.shape[0], dtype=np.uint32)))
+        s.add_instance(sc.Instance(m, mat))
+    s.camera = sc.Camera(transform=((1.0, 0.0, 0.0, 0.0), (0.0, 0.9578, 0.2873, 9.0), (0.0, -0.2873, 0.9578, 32.0)))
+    return s
