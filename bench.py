@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""Msamples/s of the plain wavefront path tracer at 1080p on one MI355X.
+
+Workload (the procedural scale scene): 1,024 seeded level-3 icospheres (1,310,720 triangles) in a
+20 m cube, GGX roughness U[0.05, 1], metallic Bernoulli(0.3), a sun and a constant environment map;
+1920x1080, depth 8, wavefront integrator with compacted queues, FP32 paths and FP64 film.
+A step is one sample pass over every pixel (2,073,600 camera paths); the default 64 timed steps are
+the workload's 64 spp. Inputs (scene tables, LBVH) are resident in HBM before timing starts.
+
+Prints one JSON line with the throughput, a roofline entry for the dominant kernel (achieved
+algorithmic bytes per launch over its HIP-event launch time, against 8 TB/s) and the CPU oracle's
+throughput on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps 64] [--warmup 4]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+NODE_BYTES = 128       # one 4-wide LBVH node
+TRI_BYTES = 48         # one leaf triangle record
+STATE_WORDS = 17       # slot-indexed path state without the lighting split
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, copy-calibrated),
+    when a summary for this kernel and workload is committed; None otherwise."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_procedural_1080p.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data.get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def algorithmic_bytes(c: dict, pixels: int, delta_lights: int, env: bool):
+    """Algorithmic bytes of one sample pass, split by kernel, from the traversal counters.
+
+    extend: per live path the ray (24 B read, none at depth 0 where the ray is regenerated) and the
+            hit record (16 B write), plus every LBVH node (128 B) and leaf triangle (48 B) it visits;
+    shade:  per live path the hit (16 B), the state (68 B read beyond depth 0, 68 B write for each
+            survivor), the shading tables of a hit (instance 64 + indices 12 + positions 36 +
+            vertex normals 36 + material 96 = 244 B), 64 B per delta light, 48 B of environment
+            sampling tables, the shadow rays' nodes and triangles, and the FP64 film update
+            (48 B) once per finished path.
+    """
+    rays = c["closest_rays"]
+    hits = c["closest_hits"]
+    later = max(rays - pixels, 0)  # paths that reached depth >= 1
+    extend = 24 * later + 16 * rays + NODE_BYTES * c["closest_nodes"] + TRI_BYTES * c["closest_tris"]
+    shade = (16 * rays + 4 * STATE_WORDS * later + 4 * STATE_WORDS * later + hits * (244 + 64 * delta_lights + (48 if env else 0))
+             + NODE_BYTES * c["shadow_nodes"] + TRI_BYTES * c["shadow_tris"] + 48 * pixels)
+    return extend, shade
+
+
+def cpu_baseline(packed, width, height, gpu_film):
+    """The oracle (a scalar C port, OpenMP over pixels) on the host cores: one sample of every pixel
+    of the same workload, plus a full-size parity check of that sample against the GPU's."""
+    import oracle
+    from verion_amd import pack
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    o = oracle.OracleScene(packed)
+    build_s = time.perf_counter() - t0
+    par = pack.base_params(packed.info, packed.view, width, height, False, True)
+    t0 = time.perf_counter()
+    st = o.render(par, 1, first_sample=0, threads=threads)
+    secs = time.perf_counter() - t0
+    film = st["film"]
+    rel = np.abs(gpu_film - film) / np.maximum(np.abs(film), 1e-6)
+    parity = {"max_rel": float(rel.max()), "exact_fraction": float(np.mean(gpu_film == film)), "pixels": width * height,
+              "sample": 0}
+    base = {"value": round(width * height / secs / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"{width}x{height}, 1 spp (sample 0) of the same scene on the CPU oracle; "
+                      f"{secs:.1f} s render, BVH build {build_s:.1f} s excluded"}
+    return base, parity
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--integrator", choices=["wavefront", "megakernel"], default="wavefront")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (the film would be "
+              "sharded by sample index and reduced once per render); run it with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from verion_amd import pack, scenes
+    from verion_amd.backend import MEGAKERNEL, WAVEFRONT, Context, Renderer
+
+    W, H = args.width, args.height
+    pixels = W * H
+    t0 = time.perf_counter()
+    packed = pack.pack(scenes.procedural())
+    ctx = Context()
+    r = Renderer(ctx, packed)
+    log(f"scene + LBVH ready in {time.perf_counter() - t0:.2f} s: {r.bvh_stats()}")
+    r.set_integrator(WAVEFRONT if args.integrator == "wavefront" else MEGAKERNEL)
+
+    warm = r.new_accumulation(W, H)
+    for _ in range(args.warmup):
+        r.accumulate(warm, 1)
+    warm.close()
+
+    # timed region: K sample passes from sample 0; accumulate() returns after the GPU finished, so
+    # the clock brackets device-complete work on both sides. Stage timing records HIP events around
+    # every extend and shade launch on the library's stream.
+    r.set_stage_timing(True)
+    acc = r.new_accumulation(W, H)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.accumulate(acc, 1)
+    elapsed = time.perf_counter() - t0
+    stage = acc.stage_seconds()
+    value = args.steps * pixels / elapsed / 1e6
+    log(f"timed {args.steps} steps in {elapsed:.3f} s; stage seconds {stage}")
+
+    # traversal counters of one more pass (outside the timed region) for the algorithmic bytes
+    r.set_counters(True)
+    cnt_acc = r.new_accumulation(W, H)
+    r.accumulate(cnt_acc, 1)
+    counts = r.read_counters()
+    r.set_counters(False)
+    log(f"counters per pass: {counts}")
+    delta_lights = packed.info.light_count
+    extend_b, shade_b = algorithmic_bytes(counts, pixels, delta_lights, packed.info.env_width > 0)
+    launches = args.steps * 8
+    ext_s, sh_s = stage if stage else (float("nan"), float("nan"))
+    dom = "wf_shade" if sh_s >= ext_s else "wf_extend"
+    dom_s = max(ext_s, sh_s) / launches
+    dom_b = (shade_b if dom == "wf_shade" else extend_b) / 8
+    achieved = dom_b / dom_s / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom),
+                "launch_ms": round(dom_s * 1e3, 4), "algorithmic_bytes_per_launch": int(dom_b),
+                "extend_ms_per_launch": round(ext_s / launches * 1e3, 4),
+                "shade_ms_per_launch": round(sh_s / launches * 1e3, 4)}
+
+    line = {"metric": "Msamples/s at 1080p (plain wavefront, procedural 1.31M-tri scene)", "value": round(value, 3),
+            "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "procedural 1,024 icospheres (1,310,720 tris), sun + constant env map",
+                       "width": W, "height": H, "spp_per_step": 1, "max_depth": 8, "integrator": args.integrator,
+                       "film": "fp64"},
+            "roofline": roofline}
+    if not args.no_cpu_baseline:
+        gpu1 = r.new_accumulation(W, H)
+        r.set_stage_timing(False)
+        r.accumulate(gpu1, 1)
+        base, parity = cpu_baseline(packed, W, H, gpu1.film())
+        line["cpu_baseline"] = base
+        line["parity_full_size"] = parity
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

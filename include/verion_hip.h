@@ -40,10 +40,10 @@ typedef enum vh_status {
 } vh_status;
 
 /* Integrator (lib.rs:97-103). Both give the same image bit for bit. */
-typedef enum vh_integrator {
+typedef enum vh_integrator_kind {
     VH_MEGAKERNEL = 0,
     VH_WAVEFRONT = 1,
-} vh_integrator;
+} vh_integrator_kind;
 
 /* pack::SceneInfo (pack.rs:55-69). */
 typedef struct vh_scene_info {
@@ -159,6 +159,10 @@ int vh_set_restir(vh_renderer* r, const vh_restir* restir);
  * stream, pcg(pixel ^ pcg(sample * 0x9E3779B9 + 0x7F4A7C15)) (pt_core.slang:651); any other seed
  * XORs pcg(seed ^ 0x5851F42D) into the pixel hash. No counterpart in the reference (it has no seed). */
 int vh_set_seed(vh_renderer* r, uint32_t seed);
+/* Grid of the wavefront's grid-stride extend/shade kernels, in 256-thread blocks; 0 = 8 per CU.
+ * No counterpart in the reference (its dispatch size is the queue length, wf_args.slang:19); set
+ * by the autotuner. Images do not depend on it. */
+int vh_set_wavefront_blocks(vh_renderer* r, uint32_t blocks);
 
 /* --- accumulation (new_accumulation / accumulate, lib.rs:47-52; trait Accumulation :74-93) ---- */
 
@@ -204,6 +208,11 @@ int vh_primary_hits(vh_renderer* r, uint32_t width, uint32_t height, uint32_t sa
 int vh_trace_rays(vh_renderer* r, uint64_t n, const float* rays, float tmax, uint32_t* hits);
 /* LBVH shape: nodes, leaves, maximum depth, triangles. */
 int vh_bvh_stats(const vh_renderer* r, uint64_t out[4]);
+/* Traversal counting for the roofline's algorithmic bytes: while on, every ray query adds to
+ * [closest rays, their nodes visited, their triangles tested, shadow rays, nodes, triangles]. */
+int vh_set_counters(vh_renderer* r, int on);
+/* Reads the counters (the six above, then closest rays that hit, then 0) and clears them. VH_ERR_UNSUPPORTED while counting is off. */
+int vh_read_counters(vh_renderer* r, uint64_t out[8]);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,20 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) and the built libverion_hip.so")
+    config.addinivalue_line("markers", "slow: longer CPU runs")
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    from verion_amd.backend import Context
+
+    return Context()

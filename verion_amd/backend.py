@@ -1,0 +1,316 @@
+"""Python mirror of Verion's backend contract over the C ABI of libverion_hip.so.
+
+`Renderer` follows `trait RayTracingBackend` (crates/verion-gpu/src/lib.rs:12-58) and
+`Accumulation` follows `trait Accumulation` (lib.rs:74-93), with the same method names, argument
+meaning and error behaviour (the reference's `Error` variants become `BackendError.status`).
+There is no CPU fallback: without the built library every constructor raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+from .pack import Packed, View, pack
+from .scene import Scene
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libverion_hip.so")
+
+OK, NO_DEVICE, UNSUPPORTED, OUT_OF_MEMORY, KERNEL, SCENE, INVALID_ARGUMENT = range(7)
+MEGAKERNEL, WAVEFRONT = 0, 1
+
+
+class BackendError(RuntimeError):
+    def __init__(self, status: int, message: str, needed: int = 0, available: int = 0):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+        self.message = message
+        self.needed = needed
+        self.available = available
+
+
+@dataclass
+class Adaptive:
+    """lib.rs:110-126 (defaults threshold 0.02, min_spp 16)."""
+
+    threshold: float = 0.02
+    min_spp: int = 16
+
+    def active(self, n: float, s: float, s2: float) -> bool:
+        """Adaptive::active (lib.rs:154-161)."""
+        if n < self.min_spp:
+            return True
+        mean = s / n
+        var = max(s2 / n - mean * mean, 0.0) * n / (n - 1.0)
+        return (var / n) ** 0.5 > self.threshold * max(mean, 0.05)
+
+
+@dataclass
+class Restir:
+    """lib.rs:131-149 (defaults: DI on, GI off, 4 candidates, history 4)."""
+
+    di: bool = True
+    gi: bool = False
+    candidates: int = 4
+    history: int = 4
+
+
+_lib = None
+
+
+def library() -> C.CDLL:
+    """Loads libverion_hip.so; raises ImportError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m verion_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+    u32p, f32p, f64p = C.POINTER(C.c_uint32), C.POINTER(C.c_float), C.POINTER(C.c_double)
+    sig = {
+        "vh_last_error": (C.c_char_p, []),
+        "vh_last_error_needed": (u64, []),
+        "vh_last_error_available": (u64, []),
+        "vh_context_create": (i32, [C.POINTER(vp)]),
+        "vh_context_destroy": (None, [vp]),
+        "vh_memory": (i32, [vp, C.POINTER(abi.MemoryUsage)]),
+        "vh_renderer_create": (i32, [vp, C.POINTER(abi.SceneTables), C.POINTER(vp)]),
+        "vh_renderer_destroy": (None, [vp]),
+        "vh_renderer_update": (i32, [vp, C.POINTER(abi.SceneTables), C.POINTER(abi.UpdateReport)]),
+        "vh_set_camera": (i32, [vp, C.POINTER(abi.ViewS)]),
+        "vh_set_multiscatter": (i32, [vp, i32]),
+        "vh_set_integrator": (i32, [vp, i32]),
+        "vh_integrator": (i32, [vp]),
+        "vh_set_stage_timing": (i32, [vp, i32]),
+        "vh_set_adaptive": (i32, [vp, C.POINTER(abi.Adaptive)]),
+        "vh_set_restir": (i32, [vp, C.POINTER(abi.Restir)]),
+        "vh_set_seed": (i32, [vp, u32]),
+        "vh_set_wavefront_blocks": (i32, [vp, u32]),
+        "vh_accumulation_create": (i32, [vp, u32, u32, i32, C.POINTER(vp)]),
+        "vh_accumulation_destroy": (None, [vp]),
+        "vh_accumulate": (i32, [vp, vp, u32]),
+        "vh_acc_samples": (u32, [vp]),
+        "vh_acc_gpu_seconds": (C.c_double, [vp]),
+        "vh_acc_stage_seconds": (i32, [vp, f64p]),
+        "vh_acc_wavefront_bytes": (u64, [vp]),
+        "vh_acc_restir_bytes": (u64, [vp]),
+        "vh_acc_active_pixels": (u64, [vp]),
+        "vh_acc_read_film": (i32, [vp, f64p]),
+        "vh_acc_read_stats": (i32, [vp, f64p]),
+        "vh_acc_read_lighting": (i32, [vp, f32p]),
+        "vh_acc_mean": (i32, [vp, f32p]),
+        "vh_render_normals": (i32, [vp, u32, u32, f32p]),
+        "vh_render_aovs": (i32, [vp, u32, u32, f32p, u32p, u32p, f32p]),
+        "vh_primary_hits": (i32, [vp, u32, u32, u32, u32p]),
+        "vh_trace_rays": (i32, [vp, u64, f32p, C.c_float, u32p]),
+        "vh_bvh_stats": (i32, [vp, C.POINTER(C.c_uint64)]),
+        "vh_set_counters": (i32, [vp, i32]),
+        "vh_read_counters": (i32, [vp, C.POINTER(C.c_uint64)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != OK:
+        L = library()
+        raise BackendError(rc, L.vh_last_error().decode(), L.vh_last_error_needed(), L.vh_last_error_available())
+
+
+def _p(a: np.ndarray, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class Context:
+    """RayTracingBackend::Context: the device and its stream."""
+
+    def __init__(self):
+        L = library()
+        h = C.c_void_p()
+        _check(L.vh_context_create(C.byref(h)))
+        self.h = h
+
+    def memory(self) -> abi.MemoryUsage:
+        m = abi.MemoryUsage()
+        _check(library().vh_memory(self.h, C.byref(m)))
+        return m
+
+    def close(self):
+        if getattr(self, "h", None):
+            library().vh_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Accumulation:
+    """trait Accumulation (lib.rs:74-93) over the library's running sums."""
+
+    def __init__(self, renderer: "Renderer", width: int, height: int, lighting: bool):
+        self.renderer = renderer
+        self.width, self.height, self.lighting_on = width, height, lighting
+        h = C.c_void_p()
+        _check(library().vh_accumulation_create(renderer.h, width, height, int(lighting), C.byref(h)))
+        self.h = h
+
+    @property
+    def pixels(self) -> int:
+        return self.width * self.height
+
+    def samples(self) -> int:
+        return library().vh_acc_samples(self.h)
+
+    def gpu_seconds(self) -> float:
+        return library().vh_acc_gpu_seconds(self.h)
+
+    def stage_seconds(self):
+        out = np.zeros(2, np.float64)
+        rc = library().vh_acc_stage_seconds(self.h, _p(out, C.c_double))
+        return None if rc != OK else (float(out[0]), float(out[1]))
+
+    def wavefront_bytes(self) -> int:
+        return library().vh_acc_wavefront_bytes(self.h)
+
+    def restir_bytes(self) -> int:
+        return library().vh_acc_restir_bytes(self.h)
+
+    def film(self) -> np.ndarray:
+        out = np.zeros(3 * self.pixels, np.float64)
+        _check(library().vh_acc_read_film(self.h, _p(out, C.c_double)))
+        return out
+
+    def stats(self) -> Optional[np.ndarray]:
+        out = np.zeros(3 * self.pixels, np.float64)
+        rc = library().vh_acc_read_stats(self.h, _p(out, C.c_double))
+        return out if rc == OK else None
+
+    def lighting_sums(self) -> Optional[np.ndarray]:
+        out = np.zeros(65 * self.pixels, np.float32)
+        rc = library().vh_acc_read_lighting(self.h, _p(out, C.c_float))
+        return out if rc == OK else None
+
+    def mean(self) -> np.ndarray:
+        """Mean radiance per pixel, (pixels, 3) float32."""
+        out = np.zeros(3 * self.pixels, np.float32)
+        _check(library().vh_acc_mean(self.h, _p(out, C.c_float)))
+        return out.reshape(-1, 3)
+
+    def sample_counts(self) -> Optional[np.ndarray]:
+        st = self.stats()
+        return None if st is None else st.reshape(-1, 3)[:, 0].astype(np.uint32)
+
+    def active_pixels(self) -> int:
+        return library().vh_acc_active_pixels(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            library().vh_accumulation_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Renderer:
+    """RayTracingBackend on one MI355X: upload, LBVH, megakernel and wavefront integrators."""
+
+    def __init__(self, ctx: Context, scene):
+        self.ctx = ctx
+        self.packed: Packed = scene if isinstance(scene, Packed) else pack(scene)
+        self._tables, self._keep = abi.tables_struct(self.packed)
+        h = C.c_void_p()
+        _check(library().vh_renderer_create(ctx.h, C.byref(self._tables), C.byref(h)))
+        self.h = h
+
+    def update(self, scene) -> abi.UpdateReport:
+        self.packed = scene if isinstance(scene, Packed) else pack(scene)
+        self._tables, self._keep = abi.tables_struct(self.packed)
+        rep = abi.UpdateReport()
+        _check(library().vh_renderer_update(self.h, C.byref(self._tables), C.byref(rep)))
+        return rep
+
+    def set_camera(self, view: View):
+        v = abi.view_struct(view)
+        _check(library().vh_set_camera(self.h, C.byref(v)))
+
+    def set_multiscatter(self, on: bool):
+        _check(library().vh_set_multiscatter(self.h, int(on)))
+
+    def set_integrator(self, integrator: int):
+        _check(library().vh_set_integrator(self.h, int(integrator)))
+
+    def integrator(self) -> int:
+        return library().vh_integrator(self.h)
+
+    def set_stage_timing(self, on: bool):
+        _check(library().vh_set_stage_timing(self.h, int(on)))
+
+    def set_adaptive(self, adaptive: Optional[Adaptive]):
+        a = abi.Adaptive(adaptive.threshold, adaptive.min_spp) if adaptive else None
+        _check(library().vh_set_adaptive(self.h, C.byref(a) if a else None))
+
+    def set_restir(self, restir: Optional[Restir]):
+        r = abi.Restir(int(restir.di), int(restir.gi), restir.candidates, restir.history) if restir else None
+        _check(library().vh_set_restir(self.h, C.byref(r) if r else None))
+
+    def set_seed(self, seed: int):
+        _check(library().vh_set_seed(self.h, seed))
+
+    def set_wavefront_blocks(self, blocks: int):
+        _check(library().vh_set_wavefront_blocks(self.h, blocks))
+
+    def new_accumulation(self, width: int, height: int, lighting: bool = False) -> Accumulation:
+        return Accumulation(self, width, height, lighting)
+
+    def accumulate(self, acc: Accumulation, samples: int):
+        _check(library().vh_accumulate(self.h, acc.h, samples))
+
+    def render(self, width: int, height: int, spp: int) -> np.ndarray:
+        """One accumulation of `spp` samples, its mean (the reference tests' `render`)."""
+        acc = self.new_accumulation(width, height)
+        self.accumulate(acc, spp)
+        return acc.mean()
+
+    def primary_hits(self, width: int, height: int, sample: int = 0) -> np.ndarray:
+        out = np.zeros((width * height, 4), np.uint32)
+        _check(library().vh_primary_hits(self.h, width, height, sample, _p(out, C.c_uint32)))
+        return out
+
+    def trace_rays(self, rays: np.ndarray, tmax: float = 1.0e30) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        out = np.zeros((len(rays), 4), np.uint32)
+        _check(library().vh_trace_rays(self.h, len(rays), _p(rays, C.c_float), tmax, _p(out, C.c_uint32)))
+        return out
+
+    def bvh_stats(self) -> dict:
+        out = np.zeros(4, np.uint64)
+        _check(library().vh_bvh_stats(self.h, _p(out, C.c_uint64)))
+        return {"nodes": int(out[0]), "leaves": int(out[1]), "depth": int(out[2]), "triangles": int(out[3])}
+
+    def set_counters(self, on: bool):
+        _check(library().vh_set_counters(self.h, int(on)))
+
+    def read_counters(self) -> dict:
+        """Traversal counts since the last read (counting must be on), then cleared."""
+        out = np.zeros(8, np.uint64)
+        _check(library().vh_read_counters(self.h, _p(out, C.c_uint64)))
+        k = ["closest_rays", "closest_nodes", "closest_tris", "shadow_rays", "shadow_nodes", "shadow_tris", "closest_hits"]
+        return {name: int(v) for name, v in zip(k, out)}
+
+    def close(self):
+        if getattr(self, "h", None):
+            library().vh_renderer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,832 @@
+// vh_api.hip - the C ABI of include/verion_hip.h: context, renderer, accumulation, readback.
+//
+// Host side of the reference backends (crates/verion-gpu-vulkan/src/renderer.rs:100-919,
+// crates/verion-gpu-metal/src/renderer.rs) for HIP: the tables of pack::Packed are uploaded once,
+// the LBVH is built on the device (vh_bvh.hip), and `vh_accumulate` plans the dispatches the way
+// renderer.rs:538-852 does (megakernel blocks of up to 8 samples, 1 under ReSTIR; the wavefront's
+// per-sample, per-depth stages), launches them on one stream and waits.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "vh_internal.h"
+
+namespace vh {
+
+namespace {
+thread_local std::string g_error;
+thread_local uint64_t g_needed = 0;
+thread_local uint64_t g_available = 0;
+} // namespace
+
+int fail(int status, const std::string& msg)
+{
+    g_error = msg;
+    return status;
+}
+
+int fail_oom(uint64_t needed, uint64_t available)
+{
+    g_needed = needed;
+    g_available = available;
+    g_error = "out of GPU memory: " + std::to_string(needed) + " bytes needed, " + std::to_string(available) + " available";
+    return VH_ERR_OUT_OF_MEMORY;
+}
+
+constexpr uint32_t kMaxDepth = 8;        // pack.rs:13
+constexpr uint32_t kSamplesPerDispatch = 8; // vulkan renderer.rs:29
+
+} // namespace vh
+
+using namespace vh;
+
+struct vh_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t cus = 0;
+    uint64_t uploaded = 0;
+};
+
+struct vh_renderer {
+    vh_context* ctx = nullptr;
+    DevBuf positions, indices, uvs, vnormals, instances, materials, lights, emitters, emitCdf, envPixels, envMarginal, envCond,
+        texels, texInfo, instStart;
+    BvhBuild bvh;
+    vh_scene_info info{};
+    vh_view view{};
+    uint32_t instanceCount = 0;
+    uint32_t translucent = 0;
+    bool multiscatter = true;
+    int integrator = VH_MEGAKERNEL;
+    bool stageTiming = false;
+    bool adaptiveOn = false;
+    vh_adaptive adaptive{};
+    bool restirOn = false;
+    vh_restir restir{};
+    uint32_t seed = 0;
+    uint32_t wfBlocks = 0;
+    DevBuf counters; // 6 x u64 traversal counters while counting is on
+
+    Scene scene() const
+    {
+        Scene s;
+        s.positions = positions.as<float>();
+        s.indices = indices.as<uint32_t>();
+        s.uvs = uvs.as<float>();
+        s.vnormals = vnormals.as<float>();
+        s.instances = instances.as<uint32_t>();
+        s.materials = materials.as<float>();
+        s.lights = lights.as<uint32_t>();
+        s.emitters = emitters.as<float>();
+        s.emitCdf = emitCdf.as<float>();
+        s.envPixels = envPixels.as<float>();
+        s.envMarginal = envMarginal.as<float>();
+        s.envCond = envCond.as<float>();
+        s.texels = texels.as<float>();
+        s.texInfo = texInfo.as<uint32_t>();
+        s.bvh.nodes = bvh.nodes_buf.as<float4>();
+        s.bvh.tris = bvh.tris.as<float4>();
+        s.bvh.ntri = bvh.ntri;
+        s.bvh.counters = counters.as<unsigned long long>();
+        s.translucent = translucent;
+        return s;
+    }
+};
+
+struct vh_accumulation {
+    vh_renderer* r = nullptr;
+    uint32_t width = 0, height = 0;
+    bool lighting = false;
+    DevBuf film, stats, light, restirBuf, state0, state1, hits, queue0, counts;
+    uint32_t samples = 0;
+    double gpuSeconds = 0.0;
+    double stage[2] = {0.0, 0.0};
+    bool stageValid = false;
+    bool adaptiveOn = false;
+    vh_adaptive adaptive{};
+    bool restirOn = false;
+    vh_restir restir{};
+    bool wfReady = false;
+    uint64_t wfBytes = 0;
+    uint64_t pixels() const { return (uint64_t)width * height; }
+};
+
+namespace {
+
+int check_hip(hipError_t e, const char* what)
+{
+    if (e == hipSuccess)
+        return VH_OK;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        return fail(VH_ERR_OUT_OF_MEMORY, std::string(what) + ": " + hipGetErrorString(e));
+    return fail(VH_ERR_KERNEL, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define TRY(x)                   \
+    do {                         \
+        int rc_ = (x);           \
+        if (rc_ != VH_OK)        \
+            return rc_;          \
+    } while (0)
+#define HIPTRY(x) TRY(check_hip((x), #x))
+
+uint64_t free_bytes()
+{
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess)
+        return 0;
+    return f;
+}
+
+// Error::OutOfMemory is raised before any allocation (vulkan context.rs:371-383).
+int check_fits(uint64_t needed)
+{
+    uint64_t avail = free_bytes();
+    if (needed > avail)
+        return fail_oom(needed, avail);
+    return VH_OK;
+}
+
+template <class T>
+int upload(vh_context* ctx, DevBuf& b, const T* src, uint64_t n)
+{
+    HIPTRY(b.alloc(sizeof(T) * (n ? n : 1)));
+    if (n) {
+        HIPTRY(hipMemcpyAsync(b.p, src, sizeof(T) * n, hipMemcpyHostToDevice, ctx->stream));
+        ctx->uploaded += sizeof(T) * n;
+    } else {
+        HIPTRY(hipMemsetAsync(b.p, 0, b.bytes, ctx->stream));
+    }
+    return VH_OK;
+}
+
+// Consistency of the tables against the layouts of pt_core.slang:12-49 (the kernels trust them).
+int validate(const vh_scene_tables* t, std::vector<uint32_t>& instStart, uint64_t& ntri, uint32_t& translucent)
+{
+    if (!t)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null scene tables");
+    if (t->instance_count == 0)
+        return fail(VH_ERR_SCENE, "scene has no instances");
+    if (t->instances_len != 16 * t->instance_count)
+        return fail(VH_ERR_SCENE, "instance table is not 16 words per instance");
+    if (t->positions_len % 3 || t->indices_len % 3 || t->materials_len % 24 || t->materials_len == 0)
+        return fail(VH_ERR_SCENE, "positions, indices or materials have a partial record");
+    uint64_t nverts = t->positions_len / 3;
+    if (t->uvs_len != 2 * nverts || t->vnormals_len != 3 * nverts)
+        return fail(VH_ERR_SCENE, "uvs or vertex normals do not match the vertex count");
+    uint64_t nmat = t->materials_len / 24;
+    uint64_t ntex = t->tex_info_len / 4;
+    for (uint64_t m = 0; m < nmat; m++) {
+        for (int w : {10, 11, 12, 13, 23}) {
+            float f = t->materials[24 * m + w];
+            if (!(f >= 0.0f) || f > (float)ntex)
+                return fail(VH_ERR_SCENE, "material refers to a missing texture");
+        }
+    }
+    for (uint64_t k = 0; k < ntex; k++) {
+        uint64_t base = t->tex_info[4 * k], w = t->tex_info[4 * k + 1], h = t->tex_info[4 * k + 2], lv = t->tex_info[4 * k + 3];
+        uint64_t texels = 0, lw = w, lh = h;
+        for (uint64_t l = 0; l < lv; l++) {
+            texels += lw * lh;
+            lw = std::max<uint64_t>(lw / 2, 1);
+            lh = std::max<uint64_t>(lh / 2, 1);
+        }
+        if (w == 0 || h == 0 || lv == 0 || 3 * (base + texels) > t->texels_len)
+            return fail(VH_ERR_SCENE, "texture table out of range");
+    }
+    uint64_t need_lights = (uint64_t)t->info.light_count * 16 + kAlbedoWords + 2 * t->instance_count;
+    if (t->lights_len < need_lights)
+        return fail(VH_ERR_SCENE, "light table shorter than lights + albedo tables + link masks");
+    if (t->emitters_len != 16ull * t->info.emitter_count || t->emit_cdf_len != t->info.emitter_count)
+        return fail(VH_ERR_SCENE, "emitter table does not match the emitter count");
+    uint64_t ew = t->info.env_width, eh = t->info.env_height;
+    if ((ew == 0) != (eh == 0) || t->env_pixels_len != 3 * ew * eh || t->env_marginal_len != eh || t->env_cond_len != ew * eh)
+        return fail(VH_ERR_SCENE, "environment tables do not match the map size");
+    instStart.assign(t->instance_count, 0);
+    ntri = 0;
+    translucent = 0;
+    for (uint64_t i = 0; i < t->instance_count; i++) {
+        uint32_t mesh = t->instance_mesh[i];
+        if (mesh >= t->mesh_count)
+            return fail(VH_ERR_SCENE, "instance refers to a missing mesh");
+        const uint32_t* mr = t->meshes + 4 * mesh;
+        const uint32_t* rec = t->instances + 16 * i;
+        if (rec[0] != mr[0] || rec[1] != mr[2])
+            return fail(VH_ERR_SCENE, "instance offsets differ from its mesh range");
+        if ((uint64_t)mr[0] + mr[1] > nverts || (uint64_t)mr[2] + mr[3] > t->indices_len || mr[3] % 3)
+            return fail(VH_ERR_SCENE, "mesh range out of the vertex or index table");
+        if (rec[2] >= nmat)
+            return fail(VH_ERR_SCENE, "instance refers to a missing material");
+        if (rec[3] != 0 && (uint64_t)rec[3] - 1 + mr[3] / 3 > t->info.emitter_count)
+            return fail(VH_ERR_SCENE, "emitter base out of the emitter table");
+        for (uint64_t k = 0; k < mr[3]; k++)
+            if (t->indices[mr[2] + k] >= mr[1])
+                return fail(VH_ERR_SCENE, "triangle index out of its mesh");
+        const float* M = t->materials + 24 * rec[2];
+        if (M[22] < 1.0f || M[23] > 0.0f)
+            translucent = 1;
+        instStart[i] = (uint32_t)ntri;
+        ntri += mr[3] / 3;
+    }
+    if (ntri >= 0x7fffffffull)
+        return fail(VH_ERR_SCENE, "more than 2^31 triangles");
+    return VH_OK;
+}
+
+// pack::base_params (pack.rs:828-874)
+Params base_params(const vh_renderer* r, uint32_t w, uint32_t h, bool lighting, bool adaptiveOn, const vh_adaptive& ad, bool restirOn,
+                   const vh_restir& re)
+{
+    Params p;
+    std::memset(&p, 0, sizeof(p));
+    auto bits = [](float f) { return asu(f); };
+    p.w[0] = w;
+    p.w[1] = h;
+    p.w[2] = bits(r->view.camera[2]);
+    p.w[3] = bits(r->view.aperture);
+    p.w[4] = bits(r->view.focus);
+    p.w[5] = r->info.light_count;
+    p.w[7] = kMaxDepth;
+    for (int k = 0; k < 12; k++)
+        p.w[8 + k] = bits(r->view.camera[4 + k]);
+    for (int k = 0; k < 3; k++)
+        p.w[20 + k] = bits(r->info.sky[k]);
+    p.w[23] = r->info.emitter_count;
+    p.w[24] = r->info.env_width;
+    p.w[25] = r->info.env_height;
+    p.w[26] = bits(r->info.env_pdf_scale);
+    p.w[27] = bits(2.0f * r->view.camera[2] / (float)h);
+    p.w[41] = r->info.virtual_light ? 1u : 0u;
+    p.w[42] = r->info.transparent_background ? 1u : 0u;
+    p.w[39] = bits(r->view.shift[0]);
+    p.w[40] = bits(r->view.shift[1]);
+    p.w[28] = lighting ? 1u : 0u;
+    p.w[29] = r->multiscatter ? 1u : 0u;
+    if (adaptiveOn) {
+        p.w[32] = 1;
+        p.w[33] = bits(ad.threshold);
+        p.w[34] = ad.min_spp;
+    }
+    if (restirOn) {
+        p.w[36] = (re.di ? 1u : 0u) | (re.gi ? 2u : 0u);
+        p.w[37] = re.candidates;
+        p.w[38] = re.history;
+    }
+    p.seedMix = r->seed == 0 ? 0u : pcg(r->seed ^ 0x5851F42Du);
+    return p;
+}
+
+int upload_tables(vh_renderer* r, const vh_scene_tables* t)
+{
+    std::vector<uint32_t> instStart;
+    uint64_t ntri = 0;
+    uint32_t translucent = 0;
+    TRY(validate(t, instStart, ntri, translucent));
+    // tables + world triangles (twice during the build) + sort buffers + binary and wide nodes
+    uint64_t tables = 4 * (t->positions_len + t->indices_len + t->uvs_len + t->vnormals_len + t->instances_len + t->materials_len +
+                           t->lights_len + t->emitters_len + t->emit_cdf_len + t->env_pixels_len + t->env_marginal_len +
+                           t->env_cond_len + t->texels_len + t->tex_info_len + t->instance_count);
+    uint64_t build = ntri * (2 * 48 + 16 + 64 + 4 * 8 + 128 + 48);
+    TRY(check_fits(tables + build));
+    vh_context* c = r->ctx;
+    TRY(upload(c, r->positions, t->positions, t->positions_len));
+    TRY(upload(c, r->indices, t->indices, t->indices_len));
+    TRY(upload(c, r->uvs, t->uvs, t->uvs_len));
+    TRY(upload(c, r->vnormals, t->vnormals, t->vnormals_len));
+    TRY(upload(c, r->instances, t->instances, t->instances_len));
+    TRY(upload(c, r->materials, t->materials, t->materials_len));
+    TRY(upload(c, r->lights, t->lights, t->lights_len));
+    TRY(upload(c, r->emitters, t->emitters, t->emitters_len));
+    TRY(upload(c, r->emitCdf, t->emit_cdf, t->emit_cdf_len));
+    TRY(upload(c, r->envPixels, t->env_pixels, t->env_pixels_len));
+    TRY(upload(c, r->envMarginal, t->env_marginal, t->env_marginal_len));
+    TRY(upload(c, r->envCond, t->env_cond, t->env_cond_len));
+    TRY(upload(c, r->texels, t->texels, t->texels_len));
+    TRY(upload(c, r->texInfo, t->tex_info, t->tex_info_len));
+    TRY(upload(c, r->instStart, instStart.data(), instStart.size()));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    r->info = t->info;
+    r->view = t->view;
+    r->instanceCount = (uint32_t)t->instance_count;
+    r->translucent = translucent;
+    DeviceTables dt{r->positions.as<float>(), r->indices.as<uint32_t>(), r->instances.as<uint32_t>(), r->instStart.as<uint32_t>(),
+                    r->instanceCount, (uint32_t)ntri};
+    TRY(build_bvh(r->bvh, dt, c->stream));
+    if (3 * (r->bvh.depth + 1) + 4 > (uint32_t)kStack)
+        return fail(VH_ERR_SCENE, "LBVH deeper than the traversal stack holds (depth " + std::to_string(r->bvh.depth) + ")");
+    return VH_OK;
+}
+
+int ensure_wavefront(vh_accumulation* a)
+{
+    if (a->wfReady)
+        return VH_OK;
+    uint64_t n = a->pixels();
+    uint32_t words = 17 + (a->lighting ? 17 : 0);
+    uint64_t bytes = 2 * 4 * words * n + 16 * n + 4 * n + 64;
+    TRY(check_fits(bytes));
+    HIPTRY(a->state0.alloc(4 * words * n));
+    HIPTRY(a->state1.alloc(4 * words * n));
+    HIPTRY(a->hits.alloc(16 * n));
+    HIPTRY(a->queue0.alloc(4 * n));
+    HIPTRY(a->counts.alloc(64));
+    HIPTRY(hipMemsetAsync(a->counts.p, 0, 64, a->r->ctx->stream));
+    a->wfBytes = bytes;
+    a->wfReady = true;
+    return VH_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* vh_last_error(void) { return g_error.c_str(); }
+uint64_t vh_last_error_needed(void) { return g_needed; }
+uint64_t vh_last_error_available(void) { return g_available; }
+
+int vh_context_create(vh_context** out)
+{
+    if (!out)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null output");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return fail(VH_ERR_NO_DEVICE, "no HIP device");
+    int dev = 0;
+    HIPTRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPTRY(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(VH_ERR_UNSUPPORTED, std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
+    auto* c = new vh_context();
+    c->device = dev;
+    c->cus = (uint32_t)prop.multiProcessorCount;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return check_hip(e, "hipStreamCreateWithFlags");
+    }
+    *out = c;
+    return VH_OK;
+}
+
+void vh_context_destroy(vh_context* ctx)
+{
+    if (!ctx)
+        return;
+    if (ctx->stream)
+        (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int vh_memory(const vh_context* ctx, vh_memory_usage* out)
+{
+    if (!ctx || !out)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    size_t f = 0, t = 0;
+    HIPTRY(hipMemGetInfo(&f, &t));
+    out->budget = t;
+    out->allocated = t - f;
+    out->uploaded = ctx->uploaded;
+    return VH_OK;
+}
+
+int vh_renderer_create(vh_context* ctx, const vh_scene_tables* tables, vh_renderer** out)
+{
+    if (!ctx || !out)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    std::unique_ptr<vh_renderer> r(new vh_renderer());
+    r->ctx = ctx;
+    TRY(upload_tables(r.get(), tables));
+    *out = r.release();
+    return VH_OK;
+}
+
+void vh_renderer_destroy(vh_renderer* r)
+{
+    if (!r)
+        return;
+    (void)hipStreamSynchronize(r->ctx->stream);
+    delete r;
+}
+
+int vh_renderer_update(vh_renderer* r, const vh_scene_tables* tables, vh_update_report* report)
+{
+    if (!r)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null renderer");
+    uint64_t before = r->ctx->uploaded;
+    TRY(upload_tables(r, tables));
+    if (report) {
+        report->blases_built = tables->mesh_count;
+        report->tlas_built = 1;
+        report->bytes_uploaded = r->ctx->uploaded - before;
+    }
+    return VH_OK;
+}
+
+int vh_set_camera(vh_renderer* r, const vh_view* view)
+{
+    if (!r || !view)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    r->view = *view;
+    return VH_OK;
+}
+
+int vh_set_multiscatter(vh_renderer* r, int on)
+{
+    if (!r)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null renderer");
+    r->multiscatter = on != 0;
+    return VH_OK;
+}
+
+int vh_set_integrator(vh_renderer* r, int integrator)
+{
+    if (!r || (integrator != VH_MEGAKERNEL && integrator != VH_WAVEFRONT))
+        return fail(VH_ERR_INVALID_ARGUMENT, "bad renderer or integrator");
+    r->integrator = integrator;
+    return VH_OK;
+}
+
+int vh_integrator(const vh_renderer* r) { return r ? r->integrator : -1; }
+
+int vh_set_stage_timing(vh_renderer* r, int on)
+{
+    if (!r)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null renderer");
+    r->stageTiming = on != 0;
+    return VH_OK;
+}
+
+int vh_set_adaptive(vh_renderer* r, const vh_adaptive* adaptive)
+{
+    if (!r)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null renderer");
+    r->adaptiveOn = adaptive != nullptr;
+    if (adaptive)
+        r->adaptive = *adaptive;
+    return VH_OK;
+}
+
+int vh_set_restir(vh_renderer* r, const vh_restir* restir)
+{
+    if (!r)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null renderer");
+    if (restir && restir->gi)
+        return fail(VH_ERR_UNSUPPORTED, "ReSTIR GI is not built in this backend");
+    r->restirOn = restir != nullptr;
+    if (restir)
+        r->restir = *restir;
+    return VH_OK;
+}
+
+int vh_set_seed(vh_renderer* r, uint32_t seed)
+{
+    if (!r)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null renderer");
+    r->seed = seed;
+    return VH_OK;
+}
+
+int vh_set_wavefront_blocks(vh_renderer* r, uint32_t blocks)
+{
+    if (!r)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null renderer");
+    r->wfBlocks = blocks;
+    return VH_OK;
+}
+
+int vh_accumulation_create(vh_renderer* r, uint32_t width, uint32_t height, int lighting, vh_accumulation** out)
+{
+    if (!r || !out)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (width == 0 || height == 0)
+        return fail(VH_ERR_INVALID_ARGUMENT, "zero image size");
+    uint64_t n = (uint64_t)width * height;
+    if (n >= (1ull << 31))
+        return fail(VH_ERR_INVALID_ARGUMENT, "image larger than 2^31 pixels");
+    uint64_t bytes = 24 * n + (lighting ? 4 * kLightingStride * n : 0);
+    TRY(check_fits(bytes));
+    std::unique_ptr<vh_accumulation> a(new vh_accumulation());
+    a->r = r;
+    a->width = width;
+    a->height = height;
+    a->lighting = lighting != 0;
+    hipStream_t s = r->ctx->stream;
+    HIPTRY(a->film.alloc(24 * n));
+    HIPTRY(hipMemsetAsync(a->film.p, 0, 24 * n, s));
+    if (a->lighting) {
+        HIPTRY(a->light.alloc(4 * kLightingStride * n));
+        HIPTRY(hipMemsetAsync(a->light.p, 0, 4 * kLightingStride * n, s));
+    }
+    HIPTRY(hipStreamSynchronize(s));
+    *out = a.release();
+    return VH_OK;
+}
+
+void vh_accumulation_destroy(vh_accumulation* acc)
+{
+    if (!acc)
+        return;
+    (void)hipStreamSynchronize(acc->r->ctx->stream);
+    delete acc;
+}
+
+int vh_accumulate(vh_renderer* r, vh_accumulation* a, uint32_t samples)
+{
+    if (!r || !a || a->r != r)
+        return fail(VH_ERR_INVALID_ARGUMENT, "accumulation does not belong to this renderer");
+    hipStream_t s = r->ctx->stream;
+    uint64_t n = a->pixels();
+    if (a->samples == 0) {
+        a->adaptiveOn = r->adaptiveOn;
+        a->adaptive = r->adaptive;
+        a->restirOn = r->restirOn;
+        a->restir = r->restir;
+        if (a->restirOn && !a->restirBuf.p) {
+            uint64_t bytes = 4ull * kRestirStride * n;
+            TRY(check_fits(bytes));
+            HIPTRY(a->restirBuf.alloc(bytes));
+            HIPTRY(hipMemsetAsync(a->restirBuf.p, 0, bytes, s));
+        }
+    }
+    if (a->restirOn != r->restirOn ||
+        (a->restirOn && std::memcmp(&a->restir, &r->restir, sizeof(vh_restir)) != 0))
+        return fail(VH_ERR_SCENE, "ReSTIR was switched mid-accumulation; start a new accumulation");
+    if (a->adaptiveOn && !a->stats.p) {
+        TRY(check_fits(24 * n));
+        HIPTRY(a->stats.alloc(24 * n));
+        HIPTRY(hipMemsetAsync(a->stats.p, 0, 24 * n, s));
+    }
+    if (a->adaptiveOn != r->adaptiveOn)
+        return fail(VH_ERR_SCENE, "adaptive sampling was switched mid-accumulation; start a new accumulation");
+    if (samples == 0)
+        return VH_OK;
+    bool wave = r->integrator == VH_WAVEFRONT;
+    if (wave)
+        TRY(ensure_wavefront(a));
+
+    Scene S = r->scene();
+    Frame F{a->film.as<double>(), a->stats.as<double>(), a->light.as<float>(), a->restirBuf.as<float>()};
+    Params base = base_params(r, a->width, a->height, a->lighting, a->adaptiveOn, r->adaptive, a->restirOn, a->restir);
+    bool restir = a->restirOn;
+    bool split = a->lighting;
+
+    hipEvent_t e0, e1;
+    HIPTRY(hipEventCreate(&e0));
+    HIPTRY(hipEventCreate(&e1));
+    std::vector<hipEvent_t> stageEv;
+    bool timing = wave && r->stageTiming;
+    if (timing) {
+        stageEv.resize(4 * kMaxDepth * samples);
+        for (auto& e : stageEv)
+            HIPTRY(hipEventCreate(&e));
+    }
+    int rc = VH_OK;
+    HIPTRY(hipEventRecord(e0, s));
+    if (wave) {
+        Wavefront W;
+        W.state[0] = a->state0.as<uint32_t>();
+        W.state[1] = a->state1.as<uint32_t>();
+        W.hits = a->hits.as<uint4>();
+        W.queue0 = a->queue0.as<uint32_t>();
+        W.counts = a->counts.as<uint32_t>();
+        W.n = (uint32_t)n;
+        W.words = 17 + (split ? 17 : 0);
+        LaunchCfg cfg{s, r->wfBlocks ? r->wfBlocks : std::max(256u, r->ctx->cus * 8)};
+        for (uint32_t k = 0; k < samples && rc == VH_OK; k++) {
+            Params P = base;
+            P.w[6] = a->samples + k;
+            P.w[31] = (uint32_t)n;
+            rc = launch_wf_sample(S, F, P, W, restir, split, cfg, timing ? &stageEv[4 * kMaxDepth * k] : nullptr);
+        }
+    } else {
+        uint32_t k = 0;
+        while (k < samples && rc == VH_OK) {
+            Params P = base;
+            P.w[6] = a->samples + k;
+            P.w[35] = std::min(samples - k, restir ? 1u : kSamplesPerDispatch);
+            rc = launch_megakernel(S, F, P, P.w[35], restir, split, s);
+            k += P.w[35];
+        }
+    }
+    HIPTRY(hipEventRecord(e1, s));
+    hipError_t se = hipEventSynchronize(e1);
+    if (rc == VH_OK && se != hipSuccess)
+        rc = check_hip(se, "accumulate");
+    if (rc == VH_OK) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        a->gpuSeconds += (double)ms * 1.0e-3;
+        a->samples += samples;
+        if (timing) {
+            for (uint32_t k = 0; k < samples; k++)
+                for (uint32_t d = 0; d < kMaxDepth; d++) {
+                    hipEvent_t* ev = &stageEv[4 * kMaxDepth * k + 4 * d];
+                    float ex = 0.0f, sh = 0.0f;
+                    (void)hipEventElapsedTime(&ex, ev[0], ev[1]);
+                    (void)hipEventElapsedTime(&sh, ev[1], ev[2]);
+                    a->stage[0] += (double)ex * 1.0e-3;
+                    a->stage[1] += (double)sh * 1.0e-3;
+                }
+            a->stageValid = true;
+        }
+    }
+    for (auto& e : stageEv)
+        (void)hipEventDestroy(e);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+uint32_t vh_acc_samples(const vh_accumulation* a) { return a ? a->samples : 0; }
+double vh_acc_gpu_seconds(const vh_accumulation* a) { return a ? a->gpuSeconds : 0.0; }
+
+int vh_acc_stage_seconds(const vh_accumulation* a, double out[2])
+{
+    if (!a || !out)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    if (!a->stageValid)
+        return fail(VH_ERR_UNSUPPORTED, "stage timing was off");
+    out[0] = a->stage[0];
+    out[1] = a->stage[1];
+    return VH_OK;
+}
+
+uint64_t vh_acc_wavefront_bytes(const vh_accumulation* a) { return a ? a->wfBytes : 0; }
+uint64_t vh_acc_restir_bytes(const vh_accumulation* a) { return a && a->restirOn ? 4ull * kRestirStride * a->pixels() : 0; }
+
+int vh_acc_read_film(const vh_accumulation* a, double* out)
+{
+    if (!a || !out)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    HIPTRY(hipMemcpy(out, a->film.p, 24 * a->pixels(), hipMemcpyDeviceToHost));
+    return VH_OK;
+}
+
+int vh_acc_read_stats(const vh_accumulation* a, double* out)
+{
+    if (!a || !out)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    if (!a->stats.p)
+        return fail(VH_ERR_UNSUPPORTED, "adaptive sampling is off for this accumulation");
+    HIPTRY(hipMemcpy(out, a->stats.p, 24 * a->pixels(), hipMemcpyDeviceToHost));
+    return VH_OK;
+}
+
+int vh_acc_read_lighting(const vh_accumulation* a, float* out)
+{
+    if (!a || !out)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    if (!a->light.p)
+        return fail(VH_ERR_UNSUPPORTED, "the accumulation was made without the lighting split");
+    HIPTRY(hipMemcpy(out, a->light.p, 4ull * kLightingStride * a->pixels(), hipMemcpyDeviceToHost));
+    return VH_OK;
+}
+
+// readback::active_pixels with Adaptive::active (lib.rs:154-161), on the FP64 statistics
+uint64_t vh_acc_active_pixels(const vh_accumulation* a)
+{
+    if (!a)
+        return 0;
+    uint64_t n = a->pixels();
+    if (!a->adaptiveOn || !a->stats.p)
+        return n;
+    std::vector<double> st(3 * n);
+    if (hipMemcpy(st.data(), a->stats.p, 24 * n, hipMemcpyDeviceToHost) != hipSuccess)
+        return n;
+    uint64_t active = 0;
+    for (uint64_t p = 0; p < n; p++) {
+        double c = st[3 * p];
+        if (c < (double)a->adaptive.min_spp) {
+            active++;
+            continue;
+        }
+        double mean = st[3 * p + 1] / c;
+        double var = std::max(st[3 * p + 2] / c - mean * mean, 0.0) * c / (c - 1.0);
+        if (std::sqrt(var / c) > (double)a->adaptive.threshold * std::max(mean, 0.05))
+            active++;
+    }
+    return active;
+}
+
+// Accumulation::mean (readback.rs:18-38): the pixel's own sample count under adaptive sampling
+int vh_acc_mean(const vh_accumulation* a, float* out)
+{
+    if (!a || !out)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    uint64_t n = a->pixels();
+    std::vector<double> film(3 * n), st;
+    HIPTRY(hipMemcpy(film.data(), a->film.p, 24 * n, hipMemcpyDeviceToHost));
+    if (a->stats.p) {
+        st.resize(3 * n);
+        HIPTRY(hipMemcpy(st.data(), a->stats.p, 24 * n, hipMemcpyDeviceToHost));
+    }
+    for (uint64_t p = 0; p < n; p++) {
+        double inv = st.empty() ? 1.0 / std::max(a->samples, 1u) : 1.0 / std::max(st[3 * p], 1.0);
+        for (int k = 0; k < 3; k++)
+            out[3 * p + k] = (float)(film[3 * p + k] * inv);
+    }
+    return VH_OK;
+}
+
+int vh_primary_hits(vh_renderer* r, uint32_t width, uint32_t height, uint32_t sample, uint32_t* hits)
+{
+    if (!r || !hits || width == 0 || height == 0)
+        return fail(VH_ERR_INVALID_ARGUMENT, "bad argument");
+    uint64_t n = (uint64_t)width * height;
+    DevBuf out;
+    TRY(check_fits(16 * n));
+    HIPTRY(out.alloc(16 * n));
+    Params P = base_params(r, width, height, false, false, vh_adaptive{}, false, vh_restir{});
+    P.w[6] = sample;
+    TRY(launch_primary_hits(r->scene(), P, out.as<uint4>(), r->ctx->stream));
+    HIPTRY(hipStreamSynchronize(r->ctx->stream));
+    HIPTRY(hipMemcpy(hits, out.p, 16 * n, hipMemcpyDeviceToHost));
+    return VH_OK;
+}
+
+int vh_trace_rays(vh_renderer* r, uint64_t n, const float* rays, float tmax, uint32_t* hits)
+{
+    if (!r || (n && (!rays || !hits)))
+        return fail(VH_ERR_INVALID_ARGUMENT, "bad argument");
+    if (n == 0)
+        return VH_OK;
+    DevBuf in, out;
+    TRY(check_fits(40 * n));
+    HIPTRY(in.alloc(24 * n));
+    HIPTRY(out.alloc(16 * n));
+    HIPTRY(hipMemcpyAsync(in.p, rays, 24 * n, hipMemcpyHostToDevice, r->ctx->stream));
+    TRY(launch_trace_rays(r->scene(), in.as<float>(), n, tmax, out.as<uint4>(), r->ctx->stream));
+    HIPTRY(hipStreamSynchronize(r->ctx->stream));
+    HIPTRY(hipMemcpy(hits, out.p, 16 * n, hipMemcpyDeviceToHost));
+    return VH_OK;
+}
+
+int vh_set_counters(vh_renderer* r, int on)
+{
+    if (!r)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null renderer");
+    if (!on) {
+        r->counters.release();
+        return VH_OK;
+    }
+    if (!r->counters.p) {
+        HIPTRY(r->counters.alloc(8 * sizeof(unsigned long long)));
+        HIPTRY(hipMemsetAsync(r->counters.p, 0, 8 * sizeof(unsigned long long), r->ctx->stream));
+        HIPTRY(hipStreamSynchronize(r->ctx->stream));
+    }
+    return VH_OK;
+}
+
+int vh_read_counters(vh_renderer* r, uint64_t out[8])
+{
+    if (!r || !out)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    if (!r->counters.p)
+        return fail(VH_ERR_UNSUPPORTED, "counting is off");
+    HIPTRY(hipStreamSynchronize(r->ctx->stream));
+    HIPTRY(hipMemcpy(out, r->counters.p, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    HIPTRY(hipMemset(r->counters.p, 0, 8 * sizeof(uint64_t)));
+    return VH_OK;
+}
+
+int vh_bvh_stats(const vh_renderer* r, uint64_t out[4])
+{
+    if (!r || !out)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    out[0] = r->bvh.nodes;
+    out[1] = r->bvh.leaves;
+    out[2] = r->bvh.depth;
+    out[3] = r->bvh.ntri;
+    return VH_OK;
+}
+
+int vh_render_normals(vh_renderer* r, uint32_t width, uint32_t height, float* normals)
+{
+    (void)width;
+    (void)height;
+    (void)normals;
+    if (!r)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null renderer");
+    return fail(VH_ERR_UNSUPPORTED, "render_normals is not built yet");
+}
+
+int vh_render_aovs(vh_renderer* r, uint32_t width, uint32_t height, float* albedo_depth, uint32_t* object_id, uint32_t* material_id,
+                   float* coverage)
+{
+    (void)width;
+    (void)height;
+    (void)albedo_depth;
+    (void)object_id;
+    (void)material_id;
+    (void)coverage;
+    if (!r)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null renderer");
+    return fail(VH_ERR_UNSUPPORTED, "render_aovs is not built yet");
+}
+
+} // extern "C"

@@ -313,8 +313,9 @@ __global__ __launch_bounds__(256) void k_boxes(const float4* __restrict__ tris, 
                 if (cs[s] & 0x80000000u) {
                     triBox(tris, cs[s] & 0x7fffffffu, l, h);
                 } else {
-                    float4 bl = __builtin_nontemporal_load(&box[2 * cs[s]]);
-                    float4 bh = __builtin_nontemporal_load(&box[2 * cs[s] + 1]);
+                    // the fence above orders this read after the sibling's write (device scope)
+                    float4 bl = box[2 * cs[s]];
+                    float4 bh = box[2 * cs[s] + 1];
                     l[0] = bl.x; l[1] = bl.y; l[2] = bl.z;
                     h[0] = bh.x; h[1] = bh.y; h[2] = bh.z;
                 }

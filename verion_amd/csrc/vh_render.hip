@@ -19,8 +19,7 @@ namespace vh {
 namespace {
 
 constexpr uint32_t kWfBlock = 256;
-constexpr uint32_t kBaseWords = 17;  // 16 words of wf_state.slang:75 + the pixel id
-constexpr uint32_t kSplitWords = 17; // wf_state.slang:12 (WF_SPLIT)
+constexpr uint32_t kBaseWords = 17; // 16 words of wf_state.slang:75 + the pixel id; WF_SPLIT (17) more follow
 
 template <bool RESTIR, bool SPLIT>
 __global__ __launch_bounds__(64) void k_pathtrace(Scene S, Frame F, Params P, uint32_t samples)

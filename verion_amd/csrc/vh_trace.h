@@ -28,6 +28,9 @@ struct Bvh {
     const float4* nodes; // 8 float4 per node; node 0 is the root
     const float4* tris;  // 3 float4 per triangle, in leaf order
     uint32_t ntri;
+    // Optional traversal counters (null when off): [0] closest-hit rays, [1] their nodes visited,
+    // [2] their triangles tested, [3..5] the same for shadow (any-hit) rays, [6] rays that hit.
+    unsigned long long* counters;
 };
 
 // Watertight ray/triangle test. Inputs are the world-space vertices; on a hit with 0 < t < tmax
@@ -90,7 +93,7 @@ VH_HD bool intersect(const RayPre& r, V3 o, V3 p0, V3 p1, V3 p2, float tmax, flo
 
 __device__ __forceinline__ V3 xyz(float4 a) { return v3(a.x, a.y, a.z); }
 
-constexpr int kStack = 64;
+constexpr int kStack = 96;
 
 // Box test of the four children: returns a 4-bit mask of hits and their entry distances.
 __device__ __forceinline__ uint32_t slab4(const float4* n, float ix, float iy, float iz, float ox, float oy, float oz,
@@ -118,6 +121,19 @@ __device__ __forceinline__ float safe_inv(float d)
     return 1.0f / (fabsf(d) > 1.0e-30f ? d : (d < 0.0f ? -1.0e-30f : 1.0e-30f));
 }
 
+template <bool ANY>
+__device__ __forceinline__ void count_traversal(const Bvh& bvh, uint32_t visited, uint32_t tested, bool hit)
+{
+    if (bvh.counters) {
+        unsigned long long* c = bvh.counters + (ANY ? 3 : 0);
+        atomicAdd(c, 1ull);
+        atomicAdd(c + 1, (unsigned long long)visited);
+        atomicAdd(c + 2, (unsigned long long)tested);
+        if (hit && !ANY)
+            atomicAdd(bvh.counters + 6, 1ull);
+    }
+}
+
 // Closest hit (any = false) or the first hit found (any = true) over (0, tmax).
 template <bool ANY>
 __device__ Hit trace(const Bvh& bvh, V3 o, V3 d, float tmax)
@@ -137,8 +153,10 @@ __device__ Hit trace(const Bvh& bvh, V3 o, V3 d, float tmax)
     float stackT[kStack];
     int sp = 0;
     uint32_t node = 0;
+    uint32_t visited = 0, tested = 0;
     while (true) {
         const float4* n = bvh.nodes + 8 * (size_t)node;
+        visited++;
         float tn[4];
         uint32_t mask = slab4(n, ix, iy, iz, ox, oy, oz, best, tn);
         uint4 child = *reinterpret_cast<const uint4*>(n + 6);
@@ -158,12 +176,14 @@ __device__ Hit trace(const Bvh& bvh, V3 o, V3 d, float tmax)
                 innerT[ni] = tn[c];
                 ni++;
             } else if (!(h.hit && tn[c] > h.t)) {
+                tested += cn[c];
                 for (uint32_t k = 0; k < cn[c]; k++) {
                     const float4* tr = bvh.tris + 3 * (size_t)(ch[c] + k);
                     float4 a = tr[0], b = tr[1], cc = tr[2];
                     uint32_t gid = asu(a.w);
                     float t, b1, b2;
-                    float lim = h.hit ? __builtin_nextafterf(h.t, __builtin_inff()) : tmax;
+                    // ties are kept for the id comparison: test up to the next float above the best t (> 0)
+                    float lim = h.hit ? asf(asu(h.t) + 1u) : tmax;
                     if (intersect(pre, o, xyz(a), xyz(b), xyz(cc), lim, t, b1, b2) && t < tmax &&
                         (!h.hit || t < h.t || (t == h.t && gid < h.gid))) {
                         h.hit = true;
@@ -174,8 +194,10 @@ __device__ Hit trace(const Bvh& bvh, V3 o, V3 d, float tmax)
                         h.inst = asu(b.w);
                         h.prim = asu(cc.w);
                         best = t;
-                        if (ANY)
+                        if (ANY) {
+                            count_traversal<ANY>(bvh, visited, tested, true);
                             return h;
+                        }
                     }
                 }
             }
@@ -209,6 +231,7 @@ __device__ Hit trace(const Bvh& bvh, V3 o, V3 d, float tmax)
         if (!found)
             break;
     }
+    count_traversal<ANY>(bvh, visited, tested, h.hit);
     return h;
 }
 
