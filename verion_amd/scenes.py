@@ -60,6 +60,31 @@ def white_furnace(material: sc.BaseBsdf = None, sky: float = 1.0) -> sc.Scene:
     return s
 
 
+def floor_scene(light, occluder: bool = False) -> sc.Scene:
+    """Albedo-1 Lambert floor lit by one light, seen from z = 3 (tests/pathtrace.rs:232-263); the
+    occluder is a small black triangle between the light and the floor point under the camera."""
+    s = sc.Scene()
+    floor = s.add_mesh(plane_mesh(5.0, 0.0))
+    s.add_instance(sc.Instance(floor, s.add_material(sc.lambert((1.0, 1.0, 1.0)))))
+    if occluder:
+        tri = s.add_mesh(sc.Mesh(np.array([[0.4, -0.3, 0.5], [0.6, -0.3, 0.5], [0.5, 0.3, 0.5]], np.float32),
+                                 np.array([0, 1, 2], np.uint32)))
+        s.add_instance(sc.Instance(tri, s.add_material(sc.lambert((0.0, 0.0, 0.0)))))
+    s.add_light(light)
+    s.camera = camera_at_z(3.0)
+    return s
+
+
+def sky_scene(mesh: sc.Mesh, material: sc.BaseBsdf, camera_z: float = 3.0, sky: float = 1.0) -> sc.Scene:
+    """One object under a uniform sky, seen from the +z axis (tests/pathtrace.rs:543-560)."""
+    s = sc.Scene()
+    m = s.add_mesh(mesh)
+    s.add_instance(sc.Instance(m, s.add_material(material)))
+    s.add_light(sc.Environment((sky, sky, sky)))
+    s.camera = camera_at_z(camera_z)
+    return s
+
+
 def _quad(p0, p1, p2, p3):
     return [p0, p1, p2, p3], [0, 1, 2, 0, 2, 3]
 
