@@ -1,0 +1,237 @@
+"""GPU parity beyond the plain image: every shading feature, adaptive sampling, the lighting split,
+ReSTIR DI, seeds, progressive accumulation and the error paths of the backend contract.
+Oracle comparisons hold to 1e-4 relative per value (the build reproduces the oracle's bits);
+integrator comparisons are bit-exact (crates/verion-gpu-metal/tests/pathtrace.rs:151-230,
+restir.rs:111-137, tests/gpu.rs:52-88)."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import params
+from verion_amd import scenes
+from verion_amd.backend import (INVALID_ARGUMENT, MEGAKERNEL, SCENE, UNSUPPORTED, WAVEFRONT, Adaptive, BackendError,
+                                Renderer, Restir)
+
+pytestmark = pytest.mark.gpu
+TOL = 1.0e-4
+
+
+def rel(got, want):
+    got, want = np.asarray(got, np.float64), np.asarray(want, np.float64)
+    return float((np.abs(got - want) / np.maximum(np.abs(want), 1.0e-6)).max())
+
+
+@pytest.fixture(scope="module")
+def zoo(ctx):
+    r = Renderer(ctx, scenes.zoo())
+    return r, oracle.OracleScene(r.packed)
+
+
+@pytest.mark.parametrize("integrator", [MEGAKERNEL, WAVEFRONT])
+def test_zoo_matches_oracle(zoo, integrator):
+    r, o = zoo
+    r.set_integrator(integrator)
+    acc = r.new_accumulation(96, 64)
+    r.accumulate(acc, 8)
+    want = o.render(params(r.packed, 96, 64), 8)["film"]
+    got = acc.film()
+    assert rel(got, want) <= TOL, f"exact {np.mean(got == want):.4f}"
+    r.set_integrator(MEGAKERNEL)
+
+
+@pytest.mark.parametrize("integrator", [MEGAKERNEL, WAVEFRONT])
+def test_zoo_without_multiscatter(zoo, integrator):
+    r, o = zoo
+    r.set_integrator(integrator)
+    r.set_multiscatter(False)
+    acc = r.new_accumulation(48, 32)
+    r.accumulate(acc, 4)
+    want = o.render(params(r.packed, 48, 32, multiscatter=False), 4)["film"]
+    r.set_multiscatter(True)
+    r.set_integrator(MEGAKERNEL)
+    assert rel(acc.film(), want) <= TOL
+
+
+@pytest.mark.parametrize("integrator", [MEGAKERNEL, WAVEFRONT])
+def test_lighting_split_matches_oracle(zoo, integrator):
+    r, o = zoo
+    r.set_integrator(integrator)
+    acc = r.new_accumulation(64, 48, lighting=True)
+    r.accumulate(acc, 4)
+    st = o.render(params(r.packed, 64, 48, lighting=True), 4)
+    r.set_integrator(MEGAKERNEL)
+    assert rel(acc.film(), st["film"]) <= TOL
+    got = acc.lighting_sums()
+    assert np.abs(got - st["lighting"]).max() <= TOL * np.maximum(np.abs(st["lighting"]), 1.0).max()
+
+
+def test_lighting_split_integrators_agree_bit_for_bit(zoo):
+    r, _ = zoo
+    out = []
+    for integ in (MEGAKERNEL, WAVEFRONT):
+        r.set_integrator(integ)
+        acc = r.new_accumulation(64, 48, lighting=True)
+        r.accumulate(acc, 3)
+        out.append((acc.film(), acc.lighting_sums()))
+    r.set_integrator(MEGAKERNEL)
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("integrator", [MEGAKERNEL, WAVEFRONT])
+def test_adaptive_matches_oracle_with_counts(ctx, integrator):
+    r = Renderer(ctx, scenes.furnace_room(0.5, 1.0))
+    r.set_integrator(integrator)
+    ad = Adaptive(threshold=0.05, min_spp=4)
+    r.set_adaptive(ad)
+    acc = r.new_accumulation(16, 16)
+    r.accumulate(acc, 32)
+    st = oracle.OracleScene(r.packed).render(params(r.packed, 16, 16, adaptive=(0.05, 4)), 32)
+    assert np.array_equal(acc.stats(), st["stats"])
+    assert rel(acc.film(), st["film"]) <= TOL
+    counts = acc.sample_counts()
+    assert (counts < 32).any() and (counts == 32).any()
+
+
+def test_adaptive_gives_the_same_image_with_both_integrators(ctx):
+    r = Renderer(ctx, scenes.furnace_room(0.5, 1.0))
+    r.set_adaptive(Adaptive(threshold=0.05, min_spp=4))
+    accs = []
+    for integ in (MEGAKERNEL, WAVEFRONT):
+        r.set_integrator(integ)
+        a = r.new_accumulation(16, 16)
+        r.accumulate(a, 32)
+        accs.append(a)
+    assert np.array_equal(accs[0].mean(), accs[1].mean())
+    assert np.array_equal(accs[0].sample_counts(), accs[1].sample_counts())
+
+
+def test_adaptive_with_a_zero_threshold_is_uniform(ctx):
+    r = Renderer(ctx, scenes.furnace_room(0.5, 1.0))
+    r.set_integrator(WAVEFRONT)
+    uniform = r.render(16, 16, 8)
+    r.set_adaptive(Adaptive(threshold=0.0, min_spp=2))
+    acc = r.new_accumulation(16, 16)
+    r.accumulate(acc, 8)
+    assert np.array_equal(acc.mean(), uniform)
+    assert acc.sample_counts().tolist() == [8] * 256
+    assert acc.active_pixels() == 256
+
+
+def test_adaptive_stops_converged_pixels_at_the_minimum(ctx):
+    r = Renderer(ctx, scenes.furnace_room(0.5, 1.0))
+    r.set_adaptive(Adaptive(threshold=1e9, min_spp=4))
+    acc = r.new_accumulation(16, 16)
+    r.accumulate(acc, 8)
+    assert acc.sample_counts().tolist() == [4] * 256
+    assert acc.active_pixels() == 0
+    assert abs(acc.mean()[:, 0].mean() - 1.99) < 0.2
+    r.set_adaptive(None)
+    with pytest.raises(BackendError) as e:
+        r.accumulate(acc, 1)
+    assert e.value.status == SCENE
+
+
+@pytest.fixture(scope="module")
+def many(ctx):
+    s = scenes.many_lights(emitters=600, clutter_boxes=400)
+    r = Renderer(ctx, s)
+    return r, oracle.OracleScene(r.packed)
+
+
+@pytest.mark.parametrize("integrator", [MEGAKERNEL, WAVEFRONT])
+def test_restir_di_matches_oracle(many, integrator):
+    r, o = many
+    r.set_integrator(integrator)
+    r.set_restir(Restir(di=True, gi=False, candidates=8, history=4))
+    acc = r.new_accumulation(64, 48)
+    for _ in range(4):  # frames: each reuses the previous frame's reservoirs
+        r.accumulate(acc, 1)
+    st = o.render(params(r.packed, 64, 48, restir=(True, False, 8, 4)), 4)
+    r.set_restir(None)
+    r.set_integrator(MEGAKERNEL)
+    assert rel(acc.film(), st["film"]) <= TOL, f"exact {np.mean(acc.film() == st['film']):.4f}"
+
+
+def test_restir_integrators_agree(many):
+    r, _ = many
+    r.set_restir(Restir(candidates=8))
+    out = []
+    for integ in (MEGAKERNEL, WAVEFRONT):
+        r.set_integrator(integ)
+        a = r.new_accumulation(64, 48)
+        r.accumulate(a, 3)
+        out.append(a.film())
+    r.set_restir(None)
+    r.set_integrator(MEGAKERNEL)
+    assert np.array_equal(out[0], out[1])
+
+
+def test_restir_switch_mid_accumulation_is_an_error(many):
+    r, _ = many
+    acc = r.new_accumulation(8, 8)
+    r.accumulate(acc, 1)
+    r.set_restir(Restir())
+    with pytest.raises(BackendError) as e:
+        r.accumulate(acc, 1)
+    r.set_restir(None)
+    assert e.value.status == SCENE
+
+
+def test_restir_gi_is_unsupported(many):
+    r, _ = many
+    with pytest.raises(BackendError) as e:
+        r.set_restir(Restir(gi=True))
+    assert e.value.status == UNSUPPORTED
+
+
+def test_progressive_accumulation_equals_one_shot(ctx):
+    # pathtrace.rs:151-161
+    r = Renderer(ctx, scenes.cornell())
+    a = r.new_accumulation(32, 32)
+    r.accumulate(a, 8)
+    r.accumulate(a, 8)
+    b = r.new_accumulation(32, 32)
+    r.accumulate(b, 16)
+    assert a.samples() == 16 and a.gpu_seconds() > 0.0
+    assert np.array_equal(a.film(), b.film())
+
+
+def test_seed_zero_is_the_reference_stream_and_seeds_differ(ctx):
+    r = Renderer(ctx, scenes.cornell())
+    o = oracle.OracleScene(r.packed)
+    r.set_seed(1)
+    acc = r.new_accumulation(32, 32)
+    r.accumulate(acc, 2)
+    assert rel(acc.film(), o.render(params(r.packed, 32, 32), 2, seed=1)["film"]) <= TOL
+    r.set_seed(0)
+    acc0 = r.new_accumulation(32, 32)
+    r.accumulate(acc0, 2)
+    assert not np.array_equal(acc.film(), acc0.film())
+    assert rel(acc0.film(), o.render(params(r.packed, 32, 32), 2)["film"]) <= TOL
+
+
+def test_error_paths(ctx):
+    r = Renderer(ctx, scenes.cornell())
+    with pytest.raises(BackendError) as e:
+        r.new_accumulation(0, 16)
+    assert e.value.status == INVALID_ARGUMENT
+    from verion_amd import scene as sc
+
+    empty = sc.Scene()
+    with pytest.raises((BackendError, ValueError)):
+        Renderer(ctx, empty)
+    with pytest.raises(BackendError) as e:
+        r.set_integrator(7)
+    assert e.value.status == INVALID_ARGUMENT
+
+
+def test_wavefront_memory_is_reported(ctx):
+    r = Renderer(ctx, scenes.cornell())
+    r.set_integrator(WAVEFRONT)
+    acc = r.new_accumulation(64, 32)
+    assert acc.wavefront_bytes() == 0
+    r.accumulate(acc, 1)
+    n = 64 * 32
+    assert acc.wavefront_bytes() == 2 * 4 * 17 * n + 16 * n + 4 * n + 64
+    m = ctx.memory()
+    assert m.budget > m.allocated > 0 and m.uploaded > 0

@@ -130,30 +130,60 @@ def _p(a: np.ndarray, t):
     return a.ctypes.data_as(C.POINTER(t))
 
 
-class Context:
+class _Handle:
+    """A library handle. Handles nest (accumulation in renderer in context) and the library wants
+    them destroyed children first; freeing a handle frees its live children before it, so the
+    order in which the garbage collector finalizes the Python objects does not matter."""
+
+    __slots__ = ("h", "destroy", "kids", "parent")
+
+    def __init__(self, h, destroy, parent: Optional["_Handle"] = None):
+        self.h, self.destroy, self.kids, self.parent = h, destroy, [], parent
+        if parent is not None:
+            parent.kids.append(self)
+
+    def free(self):
+        if self.h is None:
+            return
+        for k in list(self.kids):
+            k.free()
+        self.destroy(self.h)
+        self.h = None
+        if self.parent is not None and self in self.parent.kids:
+            self.parent.kids.remove(self)
+
+
+class _Owned:
+    _handle: Optional[_Handle] = None
+
+    @property
+    def h(self):
+        return self._handle.h if self._handle is not None else None
+
+    def close(self):
+        if self._handle is not None:
+            self._handle.free()
+
+    def __del__(self):
+        self.close()
+
+
+class Context(_Owned):
     """RayTracingBackend::Context: the device and its stream."""
 
     def __init__(self):
         L = library()
         h = C.c_void_p()
         _check(L.vh_context_create(C.byref(h)))
-        self.h = h
+        self._handle = _Handle(h, L.vh_context_destroy)
 
     def memory(self) -> abi.MemoryUsage:
         m = abi.MemoryUsage()
         _check(library().vh_memory(self.h, C.byref(m)))
         return m
 
-    def close(self):
-        if getattr(self, "h", None):
-            library().vh_context_destroy(self.h)
-            self.h = None
 
-    def __del__(self):
-        self.close()
-
-
-class Accumulation:
+class Accumulation(_Owned):
     """trait Accumulation (lib.rs:74-93) over the library's running sums."""
 
     def __init__(self, renderer: "Renderer", width: int, height: int, lighting: bool):
@@ -161,7 +191,7 @@ class Accumulation:
         self.width, self.height, self.lighting_on = width, height, lighting
         h = C.c_void_p()
         _check(library().vh_accumulation_create(renderer.h, width, height, int(lighting), C.byref(h)))
-        self.h = h
+        self._handle = _Handle(h, library().vh_accumulation_destroy, renderer._handle)
 
     @property
     def pixels(self) -> int:
@@ -212,16 +242,8 @@ class Accumulation:
     def active_pixels(self) -> int:
         return library().vh_acc_active_pixels(self.h)
 
-    def close(self):
-        if getattr(self, "h", None):
-            library().vh_accumulation_destroy(self.h)
-            self.h = None
 
-    def __del__(self):
-        self.close()
-
-
-class Renderer:
+class Renderer(_Owned):
     """RayTracingBackend on one MI355X: upload, LBVH, megakernel and wavefront integrators."""
 
     def __init__(self, ctx: Context, scene):
@@ -230,7 +252,7 @@ class Renderer:
         self._tables, self._keep = abi.tables_struct(self.packed)
         h = C.c_void_p()
         _check(library().vh_renderer_create(ctx.h, C.byref(self._tables), C.byref(h)))
-        self.h = h
+        self._handle = _Handle(h, library().vh_renderer_destroy, ctx._handle)
 
     def update(self, scene) -> abi.UpdateReport:
         self.packed = scene if isinstance(scene, Packed) else pack(scene)
@@ -306,11 +328,3 @@ class Renderer:
         _check(library().vh_read_counters(self.h, _p(out, C.c_uint64)))
         k = ["closest_rays", "closest_nodes", "closest_tris", "shadow_rays", "shadow_nodes", "shadow_tris", "closest_hits"]
         return {name: int(v) for name, v in zip(k, out)}
-
-    def close(self):
-        if getattr(self, "h", None):
-            library().vh_renderer_destroy(self.h)
-            self.h = None
-
-    def __del__(self):
-        self.close()

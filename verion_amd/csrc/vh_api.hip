@@ -36,6 +36,7 @@ int fail_oom(uint64_t needed, uint64_t available)
 
 constexpr uint32_t kMaxDepth = 8;        // pack.rs:13
 constexpr uint32_t kSamplesPerDispatch = 8; // vulkan renderer.rs:29
+constexpr uint32_t kCounterBanks = 64;      // banks of the traversal counters (vh_trace.h)
 
 } // namespace vh
 
@@ -775,8 +776,8 @@ int vh_set_counters(vh_renderer* r, int on)
         return VH_OK;
     }
     if (!r->counters.p) {
-        HIPTRY(r->counters.alloc(8 * sizeof(unsigned long long)));
-        HIPTRY(hipMemsetAsync(r->counters.p, 0, 8 * sizeof(unsigned long long), r->ctx->stream));
+        HIPTRY(r->counters.alloc(kCounterBanks * 8 * sizeof(unsigned long long)));
+        HIPTRY(hipMemsetAsync(r->counters.p, 0, kCounterBanks * 8 * sizeof(unsigned long long), r->ctx->stream));
         HIPTRY(hipStreamSynchronize(r->ctx->stream));
     }
     return VH_OK;
@@ -788,9 +789,15 @@ int vh_read_counters(vh_renderer* r, uint64_t out[8])
         return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
     if (!r->counters.p)
         return fail(VH_ERR_UNSUPPORTED, "counting is off");
+    std::vector<uint64_t> banks(kCounterBanks * 8);
     HIPTRY(hipStreamSynchronize(r->ctx->stream));
-    HIPTRY(hipMemcpy(out, r->counters.p, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost));
-    HIPTRY(hipMemset(r->counters.p, 0, 8 * sizeof(uint64_t)));
+    HIPTRY(hipMemcpy(banks.data(), r->counters.p, banks.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    HIPTRY(hipMemset(r->counters.p, 0, banks.size() * sizeof(uint64_t)));
+    for (int k = 0; k < 8; k++) {
+        out[k] = 0;
+        for (uint32_t b = 0; b < kCounterBanks; b++)
+            out[k] += banks[8 * b + k];
+    }
     return VH_OK;
 }
 

@@ -113,12 +113,12 @@ __device__ __forceinline__ void loadPath(const uint32_t* s, uint32_t n, uint32_t
     }
 }
 
-// wf_args at depth 0 (wf_args.slang:12-17): the depth-0 count and empty later queues
+// wf_args at depth 0 (wf_args.slang:12-17, and its clear of counts[0] at :21-22): the depth-0
+// count (every pixel, or none yet when wf_select fills the queue) and empty later queues
 __global__ void k_wf_reset(uint32_t* counts, uint32_t n, uint32_t adaptive, uint32_t maxDepth)
 {
     if (threadIdx.x == 0) {
-        if (!adaptive)
-            counts[0] = n;
+        counts[0] = adaptive ? 0u : n;
         for (uint32_t k = 1; k <= maxDepth; k++)
             counts[k] = 0;
     }
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_select(Frame F, Params P, Wavef
         uint32_t p = base + threadIdx.x;
         bool active = p < n && pixelActive(c, p);
         uint32_t slot = compact(active, &W.counts[0]);
-        if (active)
+        if (active && slot < n) // slots stay below n: counts[0] starts the sample at 0 (k_wf_reset)
             W.queue0[slot] = p;
     }
 }
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(Scene S, Frame F, Params 
                 finishPath<SPLIT>(c, st, pixel);
         }
         uint32_t slot = compact(keep, &W.counts[d + 1]);
-        if (keep)
+        if (keep && slot < W.n) // survivors of depth d never outnumber its count (<= n)
             storePath<SPLIT>(out, W.n, slot, st, pixel);
     }
 }

@@ -125,12 +125,14 @@ template <bool ANY>
 __device__ __forceinline__ void count_traversal(const Bvh& bvh, uint32_t visited, uint32_t tested, bool hit)
 {
     if (bvh.counters) {
-        unsigned long long* c = bvh.counters + (ANY ? 3 : 0);
+        // 64 banks of 8 counters, picked by block, keep the atomics from serialising on one address
+        unsigned long long* bank = bvh.counters + 8 * (blockIdx.x & 63u);
+        unsigned long long* c = bank + (ANY ? 3 : 0);
         atomicAdd(c, 1ull);
         atomicAdd(c + 1, (unsigned long long)visited);
         atomicAdd(c + 2, (unsigned long long)tested);
         if (hit && !ANY)
-            atomicAdd(bvh.counters + 6, 1ull);
+            atomicAdd(bank + 6, 1ull);
     }
 }
 

@@ -125,6 +125,60 @@ def cornell() -> sc.Scene:
     return s
 
 
+def _textured_quad(cx, cz, half, y=0.0):
+    """Quad at height y with uvs 0..2 (repeat wrap) and +y vertex normals."""
+    v = np.array([[cx - half, y, cz - half], [cx + half, y, cz - half], [cx + half, y, cz + half], [cx - half, y, cz + half]], np.float32)
+    uv = np.array([[0, 0], [2, 0], [2, 2], [0, 2]], np.float32)
+    return sc.Mesh(v, np.array([0, 2, 1, 0, 3, 2], np.uint32), uvs=uv, normals=np.tile([[0, 1, 0]], (4, 1)).astype(np.float32))
+
+
+def zoo(seed: int = 7) -> sc.Scene:
+    """A material and light test scene: every BSDF feature of pt_core.slang (metal, rough and smooth
+    dielectric, dispersion, clear coat, sheen, translucent opacity with an opacity texture, base /
+    metallic-roughness / emission / normal textures), a point, a spot and a directional light,
+    emissive triangles and an importance-sampled environment map."""
+    rng = np.random.default_rng(seed)
+    s = sc.Scene()
+    n = 16
+    checker = np.array([[0.9, 0.2, 0.2] if (i // 4 + j // 4) % 2 else [0.2, 0.8, 0.3] for j in range(n) for i in range(n)], np.float32)
+    base_tex = s.add_image(sc.Image(n, n, checker))
+    mr = s.add_image(sc.Image(n, n, np.stack([np.zeros(n * n), rng.uniform(0.2, 1.0, n * n), rng.uniform(0, 1, n * n)], 1)))
+    em = s.add_image(sc.Image(n, n, rng.uniform(0.0, 2.0, (n * n, 3))))
+    nrm = s.add_image(sc.Image(n, n, np.clip(np.stack([0.5 + 0.2 * rng.normal(size=n * n), 0.5 + 0.2 * rng.normal(size=n * n), np.ones(n * n)], 1), 0, 1)))
+    opa = s.add_image(sc.Image(n, n, np.repeat(rng.uniform(0.0, 1.0, (n * n, 1)), 3, 1)))
+    env = s.add_image(sc.Image(16, 8, rng.uniform(0.1, 1.5, (128, 3))))
+    floor = s.add_mesh(_textured_quad(0.0, 0.0, 4.0, 0.0))
+    s.add_instance(sc.Instance(floor, s.add_material(sc.BaseBsdf(base_color=(0.8, 0.8, 0.8), base_color_texture=base_tex,
+                                                                  normal_texture=nrm, normal_scale=0.7))))
+    mats = [
+        sc.BaseBsdf(base_color=(0.95, 0.7, 0.3), metallic=1.0, roughness=0.35, metallic_roughness_texture=mr),
+        sc.BaseBsdf(base_color=(1.0, 1.0, 1.0), transmission=1.0, roughness=0.0, ior=1.5),
+        sc.BaseBsdf(base_color=(0.9, 1.0, 0.95), transmission=1.0, roughness=0.4, ior=1.7, dispersion=1.0),
+        sc.BaseBsdf(base_color=(0.2, 0.3, 0.8), roughness=0.6, clearcoat=1.0, clearcoat_roughness=0.1),
+        sc.BaseBsdf(base_color=(0.5, 0.1, 0.1), roughness=0.8, sheen_color=(0.9, 0.8, 0.7), sheen_roughness=0.4),
+        sc.BaseBsdf(base_color=(0.7, 0.7, 0.2), roughness=0.5, opacity=0.8, opacity_texture=opa),
+        sc.BaseBsdf(base_color=(0.1, 0.1, 0.1), emission=(1.0, 1.0, 1.0), emission_texture=em),
+        sc.BaseBsdf(base_color=(0.0, 0.0, 0.0), emission=(6.0, 5.0, 4.0), ior=1.0),
+    ]
+    ico = icosphere(2)
+    sphere = s.add_mesh(sc.Mesh(ico.positions, ico.indices, uvs=(ico.positions[:, :2] * 0.5 + 0.5), normals=ico.normals))
+    box = s.add_mesh(cube_mesh(0.5))
+    for k, m in enumerate(mats):
+        mi = s.add_material(m)
+        x = -3.0 + 6.0 * (k % 4) / 3.0
+        z = -1.2 if k < 4 else 1.2
+        mesh = sphere if k % 2 == 0 else box
+        r = 0.6
+        s.add_instance(sc.Instance(mesh, mi, ((r, 0.0, 0.0, x), (0.0, r, 0.0, 0.7), (0.0, 0.0, r, z))))
+    s.add_light(sc.PointLight(position=(0.0, 3.0, 2.0), color=(1.0, 0.9, 0.8), intensity=6.0))
+    s.add_light(sc.SpotLight(position=(2.0, 3.0, 0.0), direction=(-0.5, -1.0, 0.0), intensity=8.0, inner_cone=0.2, outer_cone=0.6))
+    s.add_light(sc.DirectionalLight(direction=(0.3, -1.0, -0.5), intensity=1.5))
+    s.add_light(sc.EnvironmentMap(env, 0.6))
+    s.add_light(sc.Environment((0.05, 0.05, 0.08)))
+    s.camera = sc.Camera(transform=((1.0, 0.0, 0.0, 0.0), (0.0, 0.9397, 0.342, 3.0), (0.0, -0.342, 0.9397, 6.5)))
+    return s
+
+
 def icosphere(level: int) -> sc.Mesh:
     """Unit icosphere by midpoint subdivision: 20 * 4^level triangles, smooth vertex normals."""
     t = (1.0 + math.sqrt(5.0)) / 2.0
