@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--integrator", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-counting", action="store_true",
+                    help="skip the traversal-counting pass (for profiler runs: every launch is then a timed-workload launch)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (the film would be "
@@ -135,11 +137,15 @@ def main():
     log(f"timed {args.steps} steps in {elapsed:.3f} s; stage seconds {stage}")
 
     # traversal counters of one more pass (outside the timed region) for the algorithmic bytes
-    r.set_counters(True)
-    cnt_acc = r.new_accumulation(W, H)
-    r.accumulate(cnt_acc, 1)
-    counts = r.read_counters()
-    r.set_counters(False)
+    if args.no_counting:
+        counts = {k: 0 for k in ("closest_rays", "closest_nodes", "closest_tris", "shadow_rays", "shadow_nodes",
+                                 "shadow_tris", "closest_hits")}
+    else:
+        r.set_counters(True)
+        cnt_acc = r.new_accumulation(W, H)
+        r.accumulate(cnt_acc, 1)
+        counts = r.read_counters()
+        r.set_counters(False)
     log(f"counters per pass: {counts}")
     delta_lights = packed.info.light_count
     extend_b, shade_b = algorithmic_bytes(counts, pixels, delta_lights, packed.info.env_width > 0)
