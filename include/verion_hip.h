@@ -214,6 +214,42 @@ int vh_set_counters(vh_renderer* r, int on);
 /* Reads the counters (the six above, then closest rays that hit, then 0) and clears them. VH_ERR_UNSUPPORTED while counting is off. */
 int vh_read_counters(vh_renderer* r, uint64_t out[8]);
 
+/* --- neural radiance cache (no counterpart in the reference: its milestone M7 plans one,
+ *     docs/implementation/M7-nrc-adaptive-integrator/README.md rows 2-5) ------------------------
+ * A 96-64-64-64-64-64-16 MLP (ReLU hidden, linear output, first 3 outputs RGB) over a 12-frequency
+ * position encoding, degree-3 real SH of the outgoing direction, the shading normal and the
+ * roughness; BF16 MFMA with FP32 accumulation, FP32 master weights, Adam, relative L2 loss.
+ * Parameters are 23,888 floats: W0 (96x64 row-major [k][n]), b0, W1..W4 (64x64), b1..b4, W5 (64x16), b5.
+ * Raw inputs are 12 floats per query: position (in [0,1]^3 for the debug calls), roughness,
+ * direction xyz, 0, normal xyz, 0. */
+typedef struct vh_nrc_config {
+    uint32_t train_paths;     /* training paths per frame (65,536) */
+    uint32_t train_steps;     /* Adam steps per frame */
+    float learning_rate;      /* 1e-2 */
+    uint32_t terminate_depth; /* render paths end in the cache at their first opaque vertex at this depth (1) */
+} vh_nrc_config;
+typedef struct vh_nrc vh_nrc;
+
+/* NULL config = {65536, 1, 1e-2, 1}. The cache starts with zero parameters: set them first. */
+int vh_nrc_create(vh_renderer* r, const vh_nrc_config* config, vh_nrc** out);
+void vh_nrc_destroy(vh_nrc* nrc);
+/* Replaces the FP32 master parameters (n = 23888) and resets Adam. */
+int vh_nrc_set_params(vh_nrc* nrc, const float* params, uint64_t n);
+int vh_nrc_get_params(vh_nrc* nrc, float* params, uint64_t n);
+/* Fused inference of n raw queries: 3 floats out per query. */
+int vh_nrc_infer(vh_nrc* nrc, uint64_t n, const float* raw12, float* out3);
+/* Gradient of the mean relative L2 loss over a batch (23888 floats) and the loss; no update. */
+int vh_nrc_gradient(vh_nrc* nrc, uint64_t n, const float* raw12, const float* target3, float* grad, float* loss);
+/* `steps` Adam steps on one batch; the loss of the last step's forward pass. */
+int vh_nrc_train(vh_nrc* nrc, uint64_t n, const float* raw12, const float* target3, uint32_t steps, float* loss);
+/* The NRC integrator: per frame, train_paths plain training paths on random pixels (every opaque
+ * vertex becomes a sample), train_steps Adam steps, then one sample of every pixel that ends in the
+ * cache at its first opaque vertex at terminate_depth, into `acc` (FP64 film). Not with the
+ * lighting split, adaptive sampling or ReSTIR (VH_ERR_UNSUPPORTED). */
+int vh_nrc_accumulate(vh_nrc* nrc, vh_accumulation* acc, uint32_t frames);
+/* Last loss, last training-sample count, total training seconds, total rendering seconds. */
+int vh_nrc_stats(const vh_nrc* nrc, double out[4]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,11 @@ class Restir(C.Structure):
     _fields_ = [("di", C.c_uint32), ("gi", C.c_uint32), ("candidates", C.c_uint32), ("history", C.c_uint32)]
 
 
+class NrcConfig(C.Structure):
+    _fields_ = [("train_paths", C.c_uint32), ("train_steps", C.c_uint32), ("learning_rate", C.c_float),
+                ("terminate_depth", C.c_uint32)]
+
+
 class MemoryUsage(C.Structure):
     _fields_ = [("budget", C.c_uint64), ("allocated", C.c_uint64), ("uploaded", C.c_uint64)]
 

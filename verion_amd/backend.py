@@ -111,6 +111,15 @@ def library() -> C.CDLL:
         "vh_bvh_stats": (i32, [vp, C.POINTER(C.c_uint64)]),
         "vh_set_counters": (i32, [vp, i32]),
         "vh_read_counters": (i32, [vp, C.POINTER(C.c_uint64)]),
+        "vh_nrc_create": (i32, [vp, C.POINTER(abi.NrcConfig), C.POINTER(vp)]),
+        "vh_nrc_destroy": (None, [vp]),
+        "vh_nrc_set_params": (i32, [vp, f32p, u64]),
+        "vh_nrc_get_params": (i32, [vp, f32p, u64]),
+        "vh_nrc_infer": (i32, [vp, u64, f32p, f32p]),
+        "vh_nrc_gradient": (i32, [vp, u64, f32p, f32p, f32p, f32p]),
+        "vh_nrc_train": (i32, [vp, u64, f32p, f32p, u32, f32p]),
+        "vh_nrc_accumulate": (i32, [vp, vp, u32]),
+        "vh_nrc_stats": (i32, [vp, f64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -328,3 +337,57 @@ class Renderer(_Owned):
         _check(library().vh_read_counters(self.h, _p(out, C.c_uint64)))
         k = ["closest_rays", "closest_nodes", "closest_tris", "shadow_rays", "shadow_nodes", "shadow_tris", "closest_hits"]
         return {name: int(v) for name, v in zip(k, out)}
+
+
+class Nrc(_Owned):
+    """The neural radiance cache of a renderer (vh_nrc_*): a 96-64x5-16 MLP trained online on the
+    tracer's own paths with BF16-MFMA kernels and FP32 Adam; `accumulate` is the NRC integrator."""
+
+    PARAMS = 23888
+
+    def __init__(self, renderer: Renderer, train_paths: int = 65536, train_steps: int = 1, learning_rate: float = 1e-2,
+                 terminate_depth: int = 1):
+        self.renderer = renderer
+        cfg = abi.NrcConfig(train_paths, train_steps, learning_rate, terminate_depth)
+        h = C.c_void_p()
+        _check(library().vh_nrc_create(renderer.h, C.byref(cfg), C.byref(h)))
+        self._handle = _Handle(h, library().vh_nrc_destroy, renderer._handle)
+
+    def set_params(self, theta: np.ndarray):
+        theta = np.ascontiguousarray(theta, np.float32)
+        _check(library().vh_nrc_set_params(self.h, _p(theta, C.c_float), theta.size))
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.PARAMS, np.float32)
+        _check(library().vh_nrc_get_params(self.h, _p(out, C.c_float), out.size))
+        return out
+
+    def infer(self, raw: np.ndarray) -> np.ndarray:
+        raw = np.ascontiguousarray(raw, np.float32).reshape(-1, 12)
+        out = np.zeros((len(raw), 3), np.float32)
+        _check(library().vh_nrc_infer(self.h, len(raw), _p(raw, C.c_float), _p(out, C.c_float)))
+        return out
+
+    def gradient(self, raw: np.ndarray, target: np.ndarray):
+        raw = np.ascontiguousarray(raw, np.float32).reshape(-1, 12)
+        target = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+        g = np.zeros(self.PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        _check(library().vh_nrc_gradient(self.h, len(raw), _p(raw, C.c_float), _p(target, C.c_float), _p(g, C.c_float),
+                                         _p(loss, C.c_float)))
+        return g, float(loss[0])
+
+    def train(self, raw: np.ndarray, target: np.ndarray, steps: int = 1) -> float:
+        raw = np.ascontiguousarray(raw, np.float32).reshape(-1, 12)
+        target = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+        loss = np.zeros(1, np.float32)
+        _check(library().vh_nrc_train(self.h, len(raw), _p(raw, C.c_float), _p(target, C.c_float), steps, _p(loss, C.c_float)))
+        return float(loss[0])
+
+    def accumulate(self, acc: Accumulation, frames: int):
+        _check(library().vh_nrc_accumulate(self.h, acc.h, frames))
+
+    def stats(self) -> dict:
+        out = np.zeros(4, np.float64)
+        _check(library().vh_nrc_stats(self.h, _p(out, C.c_double)))
+        return {"loss": float(out[0]), "samples": int(out[1]), "train_seconds": float(out[2]), "render_seconds": float(out[3])}

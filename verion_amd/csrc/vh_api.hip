@@ -34,87 +34,6 @@ int fail_oom(uint64_t needed, uint64_t available)
     return VH_ERR_OUT_OF_MEMORY;
 }
 
-constexpr uint32_t kMaxDepth = 8;        // pack.rs:13
-constexpr uint32_t kSamplesPerDispatch = 8; // vulkan renderer.rs:29
-constexpr uint32_t kCounterBanks = 64;      // banks of the traversal counters (vh_trace.h)
-
-} // namespace vh
-
-using namespace vh;
-
-struct vh_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    uint32_t cus = 0;
-    uint64_t uploaded = 0;
-};
-
-struct vh_renderer {
-    vh_context* ctx = nullptr;
-    DevBuf positions, indices, uvs, vnormals, instances, materials, lights, emitters, emitCdf, envPixels, envMarginal, envCond,
-        texels, texInfo, instStart;
-    BvhBuild bvh;
-    vh_scene_info info{};
-    vh_view view{};
-    uint32_t instanceCount = 0;
-    uint32_t translucent = 0;
-    bool multiscatter = true;
-    int integrator = VH_MEGAKERNEL;
-    bool stageTiming = false;
-    bool adaptiveOn = false;
-    vh_adaptive adaptive{};
-    bool restirOn = false;
-    vh_restir restir{};
-    uint32_t seed = 0;
-    uint32_t wfBlocks = 0;
-    DevBuf counters; // 6 x u64 traversal counters while counting is on
-
-    Scene scene() const
-    {
-        Scene s;
-        s.positions = positions.as<float>();
-        s.indices = indices.as<uint32_t>();
-        s.uvs = uvs.as<float>();
-        s.vnormals = vnormals.as<float>();
-        s.instances = instances.as<uint32_t>();
-        s.materials = materials.as<float>();
-        s.lights = lights.as<uint32_t>();
-        s.emitters = emitters.as<float>();
-        s.emitCdf = emitCdf.as<float>();
-        s.envPixels = envPixels.as<float>();
-        s.envMarginal = envMarginal.as<float>();
-        s.envCond = envCond.as<float>();
-        s.texels = texels.as<float>();
-        s.texInfo = texInfo.as<uint32_t>();
-        s.bvh.nodes = bvh.nodes_buf.as<float4>();
-        s.bvh.tris = bvh.tris.as<float4>();
-        s.bvh.ntri = bvh.ntri;
-        s.bvh.counters = counters.as<unsigned long long>();
-        s.translucent = translucent;
-        return s;
-    }
-};
-
-struct vh_accumulation {
-    vh_renderer* r = nullptr;
-    uint32_t width = 0, height = 0;
-    bool lighting = false;
-    DevBuf film, stats, light, restirBuf, state0, state1, hits, queue0, counts;
-    uint32_t samples = 0;
-    double gpuSeconds = 0.0;
-    double stage[2] = {0.0, 0.0};
-    bool stageValid = false;
-    bool adaptiveOn = false;
-    vh_adaptive adaptive{};
-    bool restirOn = false;
-    vh_restir restir{};
-    bool wfReady = false;
-    uint64_t wfBytes = 0;
-    uint64_t pixels() const { return (uint64_t)width * height; }
-};
-
-namespace {
-
 int check_hip(hipError_t e, const char* what)
 {
     if (e == hipSuccess)
@@ -338,7 +257,9 @@ int ensure_wavefront(vh_accumulation* a)
     return VH_OK;
 }
 
-} // namespace
+} // namespace vh
+
+using namespace vh;
 
 extern "C" {
 

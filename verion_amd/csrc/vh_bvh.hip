@@ -526,6 +526,10 @@ int build_bvh(BvhBuild& out, const DeviceTables& T, hipStream_t s)
     VH_CHECK(hipStreamSynchronize(s));
     float scale = unord(host_bnd[6]);
     out.pad = scale * 1.0e-5f + 1.0e-6f;
+    for (int k = 0; k < 3; k++) {
+        out.lo[k] = unord(host_bnd[k]);
+        out.hi[k] = unord(host_bnd[3 + k]);
+    }
 
     if (n <= kLeafMax) {
         VH_CHECK(out.nodes_buf.alloc(sizeof(float4) * 8));

@@ -79,6 +79,7 @@ struct BvhBuild {
     uint32_t depth = 0;
     uint32_t ntri = 0;
     float pad = 0.0f;
+    float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {1.0f, 1.0f, 1.0f}; // scene box of all vertices
 };
 
 int build_bvh(BvhBuild& out, const DeviceTables& T, hipStream_t s);
@@ -100,10 +101,116 @@ struct LaunchCfg {
     uint32_t wfGrid;   // blocks of the grid-stride wavefront kernels
 };
 
+constexpr uint32_t kMaxDepth = 8;           // pack.rs:13
+constexpr uint32_t kSamplesPerDispatch = 8; // vulkan renderer.rs:29
+constexpr uint32_t kCounterBanks = 64;      // banks of the traversal counters (vh_trace.h)
+
 int launch_megakernel(const Scene& S, const Frame& F, const Params& P, uint32_t samples, bool restir, bool split, hipStream_t s);
 int launch_wf_sample(const Scene& S, const Frame& F, const Params& P, const Wavefront& W, bool restir, bool split, const LaunchCfg& cfg,
                      hipEvent_t* stageEvents);
 int launch_primary_hits(const Scene& S, const Params& P, uint4* out, hipStream_t s);
+int launch_nrc_train_paths(const Scene& S, const Params& P, const Wavefront& W, uint32_t* trainPix, uint32_t nTrain, uint32_t npix,
+                           uint32_t frame, float4* vertices, float4* samples, uint32_t* sampleCount, uint32_t sampleCap,
+                           const LaunchCfg& cfg);
+int launch_wf_reset(const Params& P, const Wavefront& W, hipStream_t s);
+int launch_wf_depth(const Scene& S, const Frame& F, const Params& P, const Wavefront& W, uint32_t d, bool shade, const LaunchCfg& cfg);
+int launch_nrc_query_shade(const Scene& S, const Frame& F, const Params& P, const Wavefront& W, float4* queries, uint32_t d,
+                           const LaunchCfg& cfg);
 int launch_trace_rays(const Scene& S, const float* rays, uint64_t n, float tmax, uint4* out, hipStream_t s);
 
+} // namespace vh
+
+// ---------------------------------------------------------------- the handles of the C ABI
+
+struct vh_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t cus = 0;
+    uint64_t uploaded = 0;
+};
+
+struct vh_renderer {
+    vh_context* ctx = nullptr;
+    vh::DevBuf positions, indices, uvs, vnormals, instances, materials, lights, emitters, emitCdf, envPixels, envMarginal,
+        envCond, texels, texInfo, instStart;
+    vh::BvhBuild bvh;
+    vh_scene_info info{};
+    vh_view view{};
+    uint32_t instanceCount = 0;
+    uint32_t translucent = 0;
+    bool multiscatter = true;
+    int integrator = VH_MEGAKERNEL;
+    bool stageTiming = false;
+    bool adaptiveOn = false;
+    vh_adaptive adaptive{};
+    bool restirOn = false;
+    vh_restir restir{};
+    uint32_t seed = 0;
+    uint32_t wfBlocks = 0;
+    vh::DevBuf counters; // traversal counter banks while counting is on
+
+    vh::Scene scene() const
+    {
+        vh::Scene s;
+        s.positions = positions.as<float>();
+        s.indices = indices.as<uint32_t>();
+        s.uvs = uvs.as<float>();
+        s.vnormals = vnormals.as<float>();
+        s.instances = instances.as<uint32_t>();
+        s.materials = materials.as<float>();
+        s.lights = lights.as<uint32_t>();
+        s.emitters = emitters.as<float>();
+        s.emitCdf = emitCdf.as<float>();
+        s.envPixels = envPixels.as<float>();
+        s.envMarginal = envMarginal.as<float>();
+        s.envCond = envCond.as<float>();
+        s.texels = texels.as<float>();
+        s.texInfo = texInfo.as<uint32_t>();
+        s.bvh.nodes = bvh.nodes_buf.as<float4>();
+        s.bvh.tris = bvh.tris.as<float4>();
+        s.bvh.ntri = bvh.ntri;
+        s.bvh.counters = counters.as<unsigned long long>();
+        s.translucent = translucent;
+        return s;
+    }
+    uint32_t wavefrontBlocks() const { return wfBlocks ? wfBlocks : (ctx->cus * 8 > 256 ? ctx->cus * 8 : 256); }
+};
+
+struct vh_accumulation {
+    vh_renderer* r = nullptr;
+    uint32_t width = 0, height = 0;
+    bool lighting = false;
+    vh::DevBuf film, stats, light, restirBuf, state0, state1, hits, queue0, counts;
+    uint32_t samples = 0;
+    double gpuSeconds = 0.0;
+    double stage[2] = {0.0, 0.0};
+    bool stageValid = false;
+    bool adaptiveOn = false;
+    vh_adaptive adaptive{};
+    bool restirOn = false;
+    vh_restir restir{};
+    bool wfReady = false;
+    uint64_t wfBytes = 0;
+    uint64_t pixels() const { return (uint64_t)width * height; }
+    vh::Wavefront wavefront(bool split) const
+    {
+        vh::Wavefront W;
+        W.state[0] = state0.as<uint32_t>();
+        W.state[1] = state1.as<uint32_t>();
+        W.hits = hits.as<uint4>();
+        W.queue0 = queue0.as<uint32_t>();
+        W.counts = counts.as<uint32_t>();
+        W.n = (uint32_t)pixels();
+        W.words = 17 + (split ? 17 : 0);
+        return W;
+    }
+};
+
+namespace vh {
+// shared host helpers (vh_api.hip)
+int check_hip(hipError_t e, const char* what);
+int check_fits(uint64_t needed);
+int ensure_wavefront(vh_accumulation* a);
+Params base_params(const vh_renderer* r, uint32_t w, uint32_t h, bool lighting, bool adaptiveOn, const vh_adaptive& ad, bool restirOn,
+                   const vh_restir& re);
 } // namespace vh

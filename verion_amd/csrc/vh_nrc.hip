@@ -1,0 +1,730 @@
+// vh_nrc.hip - neural radiance cache: encoding, fused BF16-MFMA inference, fused forward/backward
+// training, gradient reduction and Adam, and the NRC integrator of the C ABI.
+//
+// No reference code exists for this (docs/implementation/M7-nrc-adaptive-integrator/README.md
+// plans it, rows 2-5 "not started"). The network is the workload's: 12-frequency position
+// encoding + degree-3 real SH of the outgoing direction + normal + roughness (96 inputs), five
+// hidden ReLU layers of 64, 16 linear outputs of which 3 are RGB; relative L2 loss; Adam with FP32
+// master weights. Matrix products run on v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
+//
+// MFMA operand convention used throughout (16x16 output tile, 32-deep k step, wave64):
+//   lane l holds row r = l % 16 of A and column r of B, and k-slice g = l / 16 (8 values);
+//   both operands are read as 8 contiguous bf16 along k, so A is stored [row][k] and B is stored
+//   [col][k]; the accumulator's element t of lane l is C[4 g + t][r].
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "vh_internal.h"
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+namespace vh {
+
+namespace {
+
+constexpr int kIn = 96, kHid = 64, kOut = 16, kNL = 6;
+constexpr int kRows = 64;   // rows per block tile (4 waves x 16)
+constexpr int kLdX = 104;   // padded leading dimension of 96-wide rows in LDS
+constexpr int kLdH = 72;    // padded leading dimension of 64-wide rows in LDS
+
+__host__ __device__ constexpr int layerK(int l) { return l == 0 ? kIn : kHid; }
+__host__ __device__ constexpr int layerN(int l) { return l == kNL - 1 ? kOut : kHid; }
+__host__ __device__ constexpr int layerNpad(int l) { return l == kNL - 1 ? 32 : kHid; } // k-depth of the backward product
+__host__ __device__ constexpr int wOff(int l) { return l == 0 ? 0 : wOff(l - 1) + layerK(l - 1) * layerN(l - 1) + layerN(l - 1); }
+__host__ __device__ constexpr int bOff(int l) { return wOff(l) + layerK(l) * layerN(l); }
+__host__ __device__ constexpr int wtOff(int l) { return l == 0 ? 0 : wtOff(l - 1) + layerK(l - 1) * layerN(l - 1); }
+__host__ __device__ constexpr int wbOff(int l) { return l == 0 ? 0 : wbOff(l - 1) + layerK(l - 1) * layerNpad(l - 1); }
+constexpr int kParams = bOff(kNL - 1) + kOut; // 23888
+constexpr int kWt = wtOff(kNL);               // 23552 bf16
+constexpr int kWb = wbOff(kNL);               // 24576 bf16
+constexpr int kBias = kParams - kWt;          // 336
+static_assert(kParams == 23888, "parameter layout");
+
+struct NrcNet {
+    const float* theta;     // fp32 master parameters
+    const __bf16* wt;       // per layer [n][k] (forward B operand)
+    const __bf16* wb;       // per layer [k][npad] (backward B operand)
+    float lo[3], inv[3];    // position normalisation to [0, 1]^3
+};
+
+__device__ __forceinline__ f32x4_t mma16(const __bf16* a, int lda, const __bf16* b, int ldb, int K, f32x4_t c)
+{
+    int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const __bf16* pa = a + r * lda + g * 8;
+    const __bf16* pb = b + r * ldb + g * 8;
+    for (int k0 = 0; k0 < K; k0 += 32)
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8_t*>(pa + k0),
+                                                     *reinterpret_cast<const bf16x8_t*>(pb + k0), c, 0, 0, 0);
+    return c;
+}
+
+// 24 of the 96 encoded inputs of one query: part 0..2 the frequency features of one position axis,
+// part 3 the 16 SH terms, the normal, the roughness and 4 zeros. Layout of oracle/nrc.py `encode`.
+__device__ void encodePart(const NrcNet& net, float4 q0, float4 q1, float4 q2, int part, __bf16* dst)
+{
+    if (part < 3) {
+        float p = part == 0 ? q0.x : (part == 1 ? q0.y : q0.z);
+        p = fminf(fmaxf((p - net.lo[part]) * net.inv[part], 0.0f), 1.0f);
+        float base = kPi * p;
+#pragma unroll
+        for (int k = 0; k < 12; k++) {
+            float s, c;
+            sincosf((float)(1u << k) * base, &s, &c);
+            dst[2 * k] = (__bf16)s;
+            dst[2 * k + 1] = (__bf16)c;
+        }
+    } else {
+        float x = q1.x, y = q1.y, z = q1.z;
+        float v[24] = {0.28209479177387814f,
+                       -0.4886025119029199f * y, 0.4886025119029199f * z, -0.4886025119029199f * x,
+                       1.0925484305920792f * x * y, -1.0925484305920792f * y * z, 0.31539156525252005f * (3.0f * z * z - 1.0f),
+                       -1.0925484305920792f * x * z, 0.5462742152960396f * (x * x - y * y),
+                       -0.5900435899266435f * y * (3.0f * x * x - y * y), 2.890611442640554f * x * y * z,
+                       -0.4570457994644658f * y * (5.0f * z * z - 1.0f), 0.3731763325901154f * z * (5.0f * z * z - 3.0f),
+                       -0.4570457994644658f * x * (5.0f * z * z - 1.0f), 1.445305721320277f * z * (x * x - y * y),
+                       -0.5900435899266435f * x * (x * x - 3.0f * y * y),
+                       q2.x, q2.y, q2.z, q0.w, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int k = 0; k < 24; k++)
+            dst[k] = (__bf16)v[k];
+    }
+}
+
+// MODE 0: out[3 q + c] = prediction of raw query q (3 float4 per record).
+// MODE 1: render: query records of the terminating depth (5 float4: inputs, pixel, throughput,
+//         total); film[pixel] += total + throughput * max(prediction, 0).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_nrc_infer(NrcNet net, const float4* rec, const uint32_t* countPtr, uint32_t countImm,
+                                                   float* out, double* film)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 s_w[kWt];
+    __shared__ float s_b[kBias];
+    __shared__ __attribute__((aligned(16))) __bf16 s_act[4][2][16 * kLdX];
+    for (int i = threadIdx.x; i < kWt / 8; i += 256)
+        reinterpret_cast<bf16x8_t*>(s_w)[i] = reinterpret_cast<const bf16x8_t*>(net.wt)[i];
+    for (int l = 0, o = 0; l < kNL; l++) {
+        for (int i = threadIdx.x; i < layerN(l); i += 256)
+            s_b[o + i] = net.theta[bOff(l) + i];
+        o += layerN(l);
+    }
+    __syncthreads();
+    const uint32_t n = countPtr ? *countPtr : countImm;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    const int stride = MODE == 0 ? 3 : (int)kNrcQueryWords;
+    for (uint32_t tile = blockIdx.x * kRows; tile < n; tile += gridDim.x * kRows) {
+        uint32_t q = tile + wave * 16 + r;
+        bool valid = q < n && (MODE == 0 || rec[(size_t)q * stride + 4].w != 0.0f);
+        __bf16* a0 = s_act[wave][0] + r * kLdX + g * 24;
+        if (valid) {
+            const float4* qr = rec + (size_t)q * stride;
+            encodePart(net, qr[0], qr[1], qr[2], g, a0);
+        } else {
+            for (int k = 0; k < 24; k++)
+                a0[k] = (__bf16)0.0f;
+        }
+        __syncthreads();
+        int bo = 0;
+#pragma unroll
+        for (int l = 0; l < kNL; l++) {
+            const int K = layerK(l), N = layerN(l);
+            const __bf16* in = s_act[wave][l & 1];
+            __bf16* nx = s_act[wave][(l + 1) & 1];
+            const int ldin = kLdX;
+            for (int nt = 0; nt < N / 16; nt++) {
+                f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
+                c = mma16(in, ldin, s_w + wtOff(l) + nt * 16 * K, K, K, c);
+                int col = nt * 16 + r;
+                float bias = s_b[bo + col];
+                if (l < kNL - 1) {
+#pragma unroll
+                    for (int t = 0; t < 4; t++)
+                        nx[(g * 4 + t) * kLdX + col] = (__bf16)fmaxf(c[t] + bias, 0.0f);
+                } else if (r < 3) {
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        uint32_t qq = tile + wave * 16 + g * 4 + t;
+                        float v = c[t] + bias;
+                        if (MODE == 0) {
+                            if (qq < n)
+                                out[(size_t)qq * 3 + r] = v;
+                        } else if (qq < n) {
+                            const float4* qr = rec + (size_t)qq * stride;
+                            float4 q4 = qr[4];
+                            if (q4.w != 0.0f) {
+                                uint32_t pixel = asu(qr[1].w);
+                                float4 thr = qr[3];
+                                float tv = r == 0 ? thr.x : (r == 1 ? thr.y : thr.z);
+                                float tot = r == 0 ? q4.x : (r == 1 ? q4.y : q4.z);
+                                film[3 * (size_t)pixel + r] += (double)(tot + tv * fmaxf(v, 0.0f));
+                            }
+                        }
+                    }
+                }
+            }
+            bo += N;
+            __syncthreads();
+        }
+    }
+}
+
+// One layer of the backward pass for a 64-row tile: dW (and db) accumulate into this wave's tiles,
+// then (l > 0) dZ of the layer below = (dZ W^T) * relu'(input).
+template <int L, int NT>
+__device__ __forceinline__ void backLayer(const NrcNet& net, const __bf16* hin, int ldin, __bf16* dz, __bf16* dzNext, __bf16* xt,
+                                          __bf16* dzt, f32x4_t (&acc)[NT], float& db)
+{
+    constexpr int K = layerK(L), N = layerN(L), NP = layerNpad(L);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    // transposes: xt[k][i] = hin[i][k], dzt[n][i] = dz[i][n]
+    for (int e = threadIdx.x; e < K * kRows; e += 256) {
+        int k = e / kRows, i = e % kRows;
+        xt[k * kLdH + i] = hin[i * ldin + k];
+    }
+    for (int e = threadIdx.x; e < N * kRows; e += 256) {
+        int nn = e / kRows, i = e % kRows;
+        dzt[nn * kLdH + i] = dz[i * kLdH + nn];
+    }
+    __syncthreads();
+    constexpr int NTILE = N / 16;
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+        int t = wave + 4 * j;
+        int kt = t / NTILE, nt = t % NTILE;
+        acc[j] = mma16(xt + kt * 16 * kLdH, kLdH, dzt + nt * 16 * kLdH, kLdH, kRows, acc[j]);
+    }
+    if ((int)threadIdx.x < N) {
+        float s = 0.0f;
+        for (int i = 0; i < kRows; i++)
+            s += (float)dz[i * kLdH + threadIdx.x];
+        db += s;
+    }
+    if constexpr (L > 0) {
+        const __bf16* W = net.wb + wbOff(L);
+        for (int kt = 0; kt < K / 16; kt++) {
+            f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
+            c = mma16(dz + wave * 16 * kLdH, kLdH, W + kt * 16 * NP, NP, NP, c);
+            int col = kt * 16 + r;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                int row = wave * 16 + g * 4 + t;
+                float h = (float)hin[row * ldin + col];
+                dzNext[row * kLdH + col] = (__bf16)(h > 0.0f ? c[t] : 0.0f);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <int L, int NT>
+__device__ __forceinline__ void storeLayer(float* part, const f32x4_t (&acc)[NT], float db)
+{
+    constexpr int N = layerN(L);
+    constexpr int NTILE = N / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+        int t = wave + 4 * j;
+        int kt = t / NTILE, nt = t % NTILE;
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            part[wOff(L) + (kt * 16 + g * 4 + e) * N + nt * 16 + r] = acc[j][e];
+    }
+    if ((int)threadIdx.x < N)
+        part[bOff(L) + threadIdx.x] = db;
+}
+
+// Fused forward + backward over 64-row tiles of training samples (4 float4 each: 3 of inputs, the
+// RGB target). Writes this block's gradient partial sums (kParams floats) and loss sum.
+__global__ __launch_bounds__(256) void k_nrc_train(NrcNet net, const float4* samples, const uint32_t* countPtr, uint32_t countImm,
+                                                   uint32_t cap, float* partials, float* lossPartials)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 s_x[kRows * kLdX];
+    __shared__ __attribute__((aligned(16))) __bf16 s_h[5][kRows * kLdH];
+    __shared__ float s_y[kRows][kOut];
+    __shared__ float s_t[kRows][4];
+    __shared__ __attribute__((aligned(16))) __bf16 s_dz[2][kRows * kLdH];
+    __shared__ __attribute__((aligned(16))) __bf16 s_xt[kIn * kLdH];
+    __shared__ __attribute__((aligned(16))) __bf16 s_dzt[kHid * kLdH];
+    __shared__ float s_loss[256];
+    uint32_t n = countPtr ? *countPtr : countImm;
+    if (n > cap)
+        n = cap;
+    const float invB = 1.0f / (float)(n > 0 ? n : 1);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    f32x4_t a0[6], a1[4], a2[4], a3[4], a4[4], a5[1];
+    float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f, d3 = 0.0f, d4 = 0.0f, d5 = 0.0f;
+    for (auto& v : a0) v = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (auto& v : a1) v = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (auto& v : a2) v = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (auto& v : a3) v = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (auto& v : a4) v = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    a5[0] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    float lossAcc = 0.0f;
+    for (uint32_t tile = blockIdx.x * kRows; tile < n; tile += gridDim.x * kRows) {
+        // encode
+        uint32_t q = tile + wave * 16 + r;
+        __bf16* xr = s_x + (wave * 16 + r) * kLdX + g * 24;
+        if (q < n) {
+            const float4* s = samples + (size_t)q * 4;
+            encodePart(net, s[0], s[1], s[2], g, xr);
+            if (g == 0) {
+                float4 t = s[3];
+                s_t[wave * 16 + r][0] = t.x;
+                s_t[wave * 16 + r][1] = t.y;
+                s_t[wave * 16 + r][2] = t.z;
+                s_t[wave * 16 + r][3] = 1.0f;
+            }
+        } else {
+            for (int k = 0; k < 24; k++)
+                xr[k] = (__bf16)0.0f;
+            if (g == 0)
+                s_t[wave * 16 + r][3] = 0.0f;
+        }
+        __syncthreads();
+        // forward; B operands stream from the global bf16 copy (L2-resident, 47 KB)
+#pragma unroll
+        for (int l = 0; l < kNL; l++) {
+            const int K = layerK(l), N = layerN(l);
+            const __bf16* in = l == 0 ? s_x : s_h[l - 1];
+            const int ldin = l == 0 ? kLdX : kLdH;
+            for (int nt = 0; nt < N / 16; nt++) {
+                f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
+                c = mma16(in + wave * 16 * ldin, ldin, net.wt + wtOff(l) + nt * 16 * K, K, K, c);
+                int col = nt * 16 + r;
+                float bias = net.theta[bOff(l) + col];
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    int row = wave * 16 + g * 4 + t;
+                    if (l < kNL - 1)
+                        s_h[l][row * kLdH + col] = (__bf16)fmaxf(c[t] + bias, 0.0f);
+                    else
+                        s_y[row][col] = c[t] + bias;
+                }
+            }
+            __syncthreads();
+        }
+        // relative L2 loss gradient, d(loss)/dy = 2 (y - t) / (y^2 + 0.01) / B for RGB
+        if (threadIdx.x < kRows) {
+            int i = threadIdx.x;
+            bool ok = s_t[i][3] != 0.0f;
+            for (int c = 0; c < 32; c++) {
+                float gv = 0.0f;
+                if (ok && c < 3) {
+                    float y = s_y[i][c], t = s_t[i][c];
+                    float den = y * y + 0.01f;
+                    gv = 2.0f * (y - t) / den * invB;
+                    lossAcc += (y - t) * (y - t) / den;
+                }
+                s_dz[0][i * kLdH + c] = (__bf16)gv;
+            }
+        }
+        __syncthreads();
+        backLayer<5, 1>(net, s_h[4], kLdH, s_dz[0], s_dz[1], s_xt, s_dzt, a5, d5);
+        backLayer<4, 4>(net, s_h[3], kLdH, s_dz[1], s_dz[0], s_xt, s_dzt, a4, d4);
+        backLayer<3, 4>(net, s_h[2], kLdH, s_dz[0], s_dz[1], s_xt, s_dzt, a3, d3);
+        backLayer<2, 4>(net, s_h[1], kLdH, s_dz[1], s_dz[0], s_xt, s_dzt, a2, d2);
+        backLayer<1, 4>(net, s_h[0], kLdH, s_dz[0], s_dz[1], s_xt, s_dzt, a1, d1);
+        backLayer<0, 6>(net, s_x, kLdX, s_dz[1], s_dz[0], s_xt, s_dzt, a0, d0);
+    }
+    float* part = partials + (size_t)blockIdx.x * kParams;
+    storeLayer<0, 6>(part, a0, d0);
+    storeLayer<1, 4>(part, a1, d1);
+    storeLayer<2, 4>(part, a2, d2);
+    storeLayer<3, 4>(part, a3, d3);
+    storeLayer<4, 4>(part, a4, d4);
+    storeLayer<5, 1>(part, a5, d5);
+    s_loss[threadIdx.x] = lossAcc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float s = 0.0f;
+        for (int i = 0; i < 256; i++)
+            s += s_loss[i];
+        lossPartials[blockIdx.x] = s;
+    }
+}
+
+// grad[p] = sum over blocks of the partials (in block order: deterministic); loss likewise.
+__global__ void k_nrc_reduce(const float* partials, uint32_t blocks, float* grad, const float* lossPartials, float* loss,
+                             const uint32_t* countPtr, uint32_t countImm, uint32_t cap)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < (uint32_t)kParams) {
+        float s = 0.0f;
+        for (uint32_t b = 0; b < blocks; b++)
+            s += partials[(size_t)b * kParams + p];
+        grad[p] = s;
+    }
+    if (p == 0) {
+        uint32_t n = countPtr ? *countPtr : countImm;
+        if (n > cap)
+            n = cap;
+        float s = 0.0f;
+        for (uint32_t b = 0; b < blocks; b++)
+            s += lossPartials[b];
+        loss[0] = s / (float)(n > 0 ? n : 1);
+        loss[1] = (float)n;
+    }
+}
+
+// Parameter p -> its bf16 copies (forward [n][k] and backward [k][npad]); biases have none.
+__device__ __forceinline__ void writeCopies(uint32_t p, float v, __bf16* wt, __bf16* wb)
+{
+#pragma unroll
+    for (int l = 0; l < kNL; l++) {
+        const int K = layerK(l), N = layerN(l);
+        if (p >= (uint32_t)wOff(l) && p < (uint32_t)bOff(l)) {
+            int e = (int)p - wOff(l);
+            int k = e / N, nn = e % N;
+            wt[wtOff(l) + nn * K + k] = (__bf16)v;
+            wb[wbOff(l) + k * layerNpad(l) + nn] = (__bf16)v;
+        }
+    }
+}
+
+__global__ void k_nrc_adam(float* theta, float* m, float* v, const float* grad, float lr, float b1, float b2, float c1, float c2,
+                           float eps, __bf16* wt, __bf16* wb)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (uint32_t)kParams)
+        return;
+    float g = grad[p];
+    float mm = b1 * m[p] + (1.0f - b1) * g;
+    float vv = b2 * v[p] + (1.0f - b2) * g * g;
+    m[p] = mm;
+    v[p] = vv;
+    float th = theta[p] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    theta[p] = th;
+    writeCopies(p, th, wt, wb);
+}
+
+__global__ void k_nrc_copies(const float* theta, __bf16* wt, __bf16* wb)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < (uint32_t)kParams)
+        writeCopies(p, theta[p], wt, wb);
+}
+
+__global__ void k_zero_bf16(__bf16* p, uint32_t n)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = (__bf16)0.0f;
+}
+
+constexpr uint32_t kTrainBlocks = 256;
+
+} // namespace
+
+} // namespace vh
+
+using namespace vh;
+
+struct vh_nrc {
+    vh_renderer* r = nullptr;
+    vh_nrc_config cfg{};
+    DevBuf theta, m, v, grad, wt, wb, partials, lossPart, loss;
+    DevBuf trainPix, vertices, samples, sampleCount, tstate0, tstate1, thits, tcounts, queries;
+    uint32_t step = 0;
+    uint32_t frames = 0;
+    double trainSeconds = 0.0, renderSeconds = 0.0;
+    float lastLoss = 0.0f;
+    uint32_t lastSamples = 0;
+    NrcNet net(bool unitBox) const
+    {
+        NrcNet n;
+        n.theta = theta.as<float>();
+        n.wt = wt.as<__bf16>();
+        n.wb = wb.as<__bf16>();
+        for (int k = 0; k < 3; k++) {
+            float lo = unitBox ? 0.0f : r->bvh.lo[k], hi = unitBox ? 1.0f : r->bvh.hi[k];
+            n.lo[k] = lo;
+            n.inv[k] = 1.0f / fmaxf(hi - lo, 1.0e-20f);
+        }
+        return n;
+    }
+};
+
+namespace {
+
+#define TRY(x)            \
+    do {                  \
+        int rc_ = (x);    \
+        if (rc_ != VH_OK) \
+            return rc_;   \
+    } while (0)
+#define HIPTRY(x) TRY(check_hip((x), #x))
+
+int train_steps(vh_nrc* nn, const float4* samples, const uint32_t* countPtr, uint32_t countImm, uint32_t cap, uint32_t steps,
+                bool adam, hipStream_t s)
+{
+    NrcNet net = nn->net(countPtr == nullptr);
+    for (uint32_t k = 0; k < steps; k++) {
+        k_nrc_train<<<kTrainBlocks, 256, 0, s>>>(net, samples, countPtr, countImm, cap, nn->partials.as<float>(), nn->lossPart.as<float>());
+        k_nrc_reduce<<<(kParams + 255) / 256, 256, 0, s>>>(nn->partials.as<float>(), kTrainBlocks, nn->grad.as<float>(),
+                                                           nn->lossPart.as<float>(), nn->loss.as<float>(), countPtr, countImm, cap);
+        if (adam) {
+            nn->step++;
+            float c1 = 1.0f - powf(0.9f, (float)nn->step), c2 = 1.0f - powf(0.99f, (float)nn->step);
+            k_nrc_adam<<<(kParams + 255) / 256, 256, 0, s>>>(nn->theta.as<float>(), nn->m.as<float>(), nn->v.as<float>(),
+                                                             nn->grad.as<float>(), nn->cfg.learning_rate, 0.9f, 0.99f, c1, c2, 1.0e-8f,
+                                                             nn->wt.as<__bf16>(), nn->wb.as<__bf16>());
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return fail(VH_ERR_KERNEL, std::string("NRC training: ") + hipGetErrorString(e));
+    }
+    return VH_OK;
+}
+
+int upload_batch(vh_nrc* nn, uint64_t n, const float* raw12, const float* target3, DevBuf& buf, hipStream_t s)
+{
+    // sample records: (p, rough) (wo, 0) (n, 0) (target, 1)
+    std::vector<float> host(16 * n, 0.0f);
+    for (uint64_t i = 0; i < n; i++) {
+        for (int k = 0; k < 12; k++)
+            host[16 * i + k] = raw12[12 * i + k];
+        for (int c = 0; c < 3; c++)
+            host[16 * i + 12 + c] = target3 ? target3[3 * i + c] : 0.0f;
+        host[16 * i + 15] = 1.0f;
+    }
+    HIPTRY(buf.alloc(sizeof(float) * host.size()));
+    HIPTRY(hipMemcpyAsync(buf.p, host.data(), sizeof(float) * host.size(), hipMemcpyHostToDevice, s));
+    (void)nn;
+    return VH_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int vh_nrc_create(vh_renderer* r, const vh_nrc_config* cfg, vh_nrc** out)
+{
+    if (!r || !out)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    std::unique_ptr<vh_nrc> nn(new vh_nrc());
+    nn->r = r;
+    nn->cfg = cfg ? *cfg : vh_nrc_config{65536, 1, 1.0e-2f, 1};
+    if (nn->cfg.train_paths == 0 || nn->cfg.train_steps == 0)
+        return fail(VH_ERR_INVALID_ARGUMENT, "train_paths and train_steps must be positive");
+    hipStream_t s = r->ctx->stream;
+    uint64_t tp = nn->cfg.train_paths;
+    uint64_t bytes = 4ull * kParams * 4 + 2ull * (kWt + kWb) + 4ull * kParams * kTrainBlocks + tp * (4 + 16 * 5 * kMaxDepthNrc + 64 * kMaxDepthNrc + 2 * 4 * 17 + 16);
+    TRY(check_fits(bytes));
+    HIPTRY(nn->theta.alloc(4ull * kParams));
+    HIPTRY(nn->m.alloc(4ull * kParams));
+    HIPTRY(nn->v.alloc(4ull * kParams));
+    HIPTRY(nn->grad.alloc(4ull * kParams));
+    HIPTRY(nn->wt.alloc(2ull * kWt));
+    HIPTRY(nn->wb.alloc(2ull * kWb));
+    HIPTRY(nn->partials.alloc(4ull * kParams * kTrainBlocks));
+    HIPTRY(nn->lossPart.alloc(4ull * kTrainBlocks));
+    HIPTRY(nn->loss.alloc(16));
+    HIPTRY(hipMemsetAsync(nn->theta.p, 0, 4ull * kParams, s));
+    HIPTRY(hipMemsetAsync(nn->m.p, 0, 4ull * kParams, s));
+    HIPTRY(hipMemsetAsync(nn->v.p, 0, 4ull * kParams, s));
+    k_zero_bf16<<<(kWb + 255) / 256, 256, 0, s>>>(nn->wb.as<__bf16>(), kWb);
+    k_zero_bf16<<<(kWt + 255) / 256, 256, 0, s>>>(nn->wt.as<__bf16>(), kWt);
+    HIPTRY(nn->trainPix.alloc(4 * tp));
+    HIPTRY(nn->vertices.alloc(16ull * kNrcVertexWords * kMaxDepthNrc * tp));
+    HIPTRY(nn->samples.alloc(64ull * kMaxDepthNrc * tp));
+    HIPTRY(nn->sampleCount.alloc(16));
+    HIPTRY(nn->tstate0.alloc(4ull * 17 * tp));
+    HIPTRY(nn->tstate1.alloc(4ull * 17 * tp));
+    HIPTRY(nn->thits.alloc(16 * tp));
+    HIPTRY(nn->tcounts.alloc(64));
+    HIPTRY(hipStreamSynchronize(s));
+    *out = nn.release();
+    return VH_OK;
+}
+
+void vh_nrc_destroy(vh_nrc* nn)
+{
+    if (!nn)
+        return;
+    (void)hipStreamSynchronize(nn->r->ctx->stream);
+    delete nn;
+}
+
+int vh_nrc_set_params(vh_nrc* nn, const float* theta, uint64_t n)
+{
+    if (!nn || !theta)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    if (n != (uint64_t)kParams)
+        return fail(VH_ERR_INVALID_ARGUMENT, "expected " + std::to_string(kParams) + " parameters");
+    hipStream_t s = nn->r->ctx->stream;
+    HIPTRY(hipMemcpyAsync(nn->theta.p, theta, 4ull * kParams, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemsetAsync(nn->m.p, 0, 4ull * kParams, s));
+    HIPTRY(hipMemsetAsync(nn->v.p, 0, 4ull * kParams, s));
+    k_nrc_copies<<<(kParams + 255) / 256, 256, 0, s>>>(nn->theta.as<float>(), nn->wt.as<__bf16>(), nn->wb.as<__bf16>());
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipStreamSynchronize(s));
+    nn->step = 0;
+    return VH_OK;
+}
+
+int vh_nrc_get_params(vh_nrc* nn, float* theta, uint64_t n)
+{
+    if (!nn || !theta || n != (uint64_t)kParams)
+        return fail(VH_ERR_INVALID_ARGUMENT, "bad argument");
+    HIPTRY(hipStreamSynchronize(nn->r->ctx->stream));
+    HIPTRY(hipMemcpy(theta, nn->theta.p, 4ull * kParams, hipMemcpyDeviceToHost));
+    return VH_OK;
+}
+
+int vh_nrc_infer(vh_nrc* nn, uint64_t n, const float* raw12, float* out3)
+{
+    if (!nn || (n && (!raw12 || !out3)))
+        return fail(VH_ERR_INVALID_ARGUMENT, "bad argument");
+    if (n == 0)
+        return VH_OK;
+    hipStream_t s = nn->r->ctx->stream;
+    std::vector<float> host(12 * n);
+    std::copy(raw12, raw12 + 12 * n, host.begin());
+    DevBuf in, out;
+    HIPTRY(in.alloc(4 * host.size()));
+    HIPTRY(out.alloc(12 * n));
+    HIPTRY(hipMemcpyAsync(in.p, host.data(), 4 * host.size(), hipMemcpyHostToDevice, s));
+    uint32_t grid = (uint32_t)std::min<uint64_t>((n + kRows - 1) / kRows, 4096);
+    k_nrc_infer<0><<<grid, 256, 0, s>>>(nn->net(true), in.as<float4>(), nullptr, (uint32_t)n, out.as<float>(), nullptr);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipStreamSynchronize(s));
+    HIPTRY(hipMemcpy(out3, out.p, 12 * n, hipMemcpyDeviceToHost));
+    return VH_OK;
+}
+
+int vh_nrc_gradient(vh_nrc* nn, uint64_t n, const float* raw12, const float* target3, float* grad, float* loss)
+{
+    if (!nn || !raw12 || !target3 || !grad || n == 0 || n > (1ull << 24))
+        return fail(VH_ERR_INVALID_ARGUMENT, "bad argument");
+    hipStream_t s = nn->r->ctx->stream;
+    DevBuf batch;
+    TRY(upload_batch(nn, n, raw12, target3, batch, s));
+    TRY(train_steps(nn, batch.as<float4>(), nullptr, (uint32_t)n, (uint32_t)n, 1, false, s));
+    HIPTRY(hipStreamSynchronize(s));
+    HIPTRY(hipMemcpy(grad, nn->grad.p, 4ull * kParams, hipMemcpyDeviceToHost));
+    if (loss)
+        HIPTRY(hipMemcpy(loss, nn->loss.p, 4, hipMemcpyDeviceToHost));
+    return VH_OK;
+}
+
+int vh_nrc_train(vh_nrc* nn, uint64_t n, const float* raw12, const float* target3, uint32_t steps, float* loss)
+{
+    if (!nn || !raw12 || !target3 || n == 0 || n > (1ull << 24))
+        return fail(VH_ERR_INVALID_ARGUMENT, "bad argument");
+    hipStream_t s = nn->r->ctx->stream;
+    DevBuf batch;
+    TRY(upload_batch(nn, n, raw12, target3, batch, s));
+    TRY(train_steps(nn, batch.as<float4>(), nullptr, (uint32_t)n, (uint32_t)n, steps, true, s));
+    HIPTRY(hipStreamSynchronize(s));
+    if (loss)
+        HIPTRY(hipMemcpy(loss, nn->loss.p, 4, hipMemcpyDeviceToHost));
+    return VH_OK;
+}
+
+int vh_nrc_accumulate(vh_nrc* nn, vh_accumulation* a, uint32_t frames)
+{
+    if (!nn || !a || a->r != nn->r)
+        return fail(VH_ERR_INVALID_ARGUMENT, "accumulation does not belong to the cache's renderer");
+    vh_renderer* r = nn->r;
+    if (a->lighting || r->adaptiveOn || r->restirOn || a->adaptiveOn || a->restirOn)
+        return fail(VH_ERR_UNSUPPORTED, "the radiance cache runs without the lighting split, adaptive sampling and ReSTIR");
+    hipStream_t s = r->ctx->stream;
+    TRY(ensure_wavefront(a));
+    uint64_t npix = a->pixels();
+    if (!nn->queries.p || nn->queries.bytes < 16ull * kNrcQueryWords * npix)
+        HIPTRY(nn->queries.alloc(16ull * kNrcQueryWords * npix));
+    Scene S = r->scene();
+    Frame F{a->film.as<double>(), nullptr, nullptr, nullptr};
+    Wavefront W = a->wavefront(false);
+    Wavefront T;
+    T.state[0] = nn->tstate0.as<uint32_t>();
+    T.state[1] = nn->tstate1.as<uint32_t>();
+    T.hits = nn->thits.as<uint4>();
+    T.queue0 = nullptr;
+    T.counts = nn->tcounts.as<uint32_t>();
+    T.n = nn->cfg.train_paths;
+    T.words = 17;
+    LaunchCfg cfg{s, r->wavefrontBlocks()};
+    Params base = base_params(r, a->width, a->height, false, false, vh_adaptive{}, false, vh_restir{});
+    NrcNet net = nn->net(false);
+    const uint32_t termDepth = 1;
+    uint32_t sampleCap = nn->cfg.train_paths * kMaxDepthNrc;
+    hipEvent_t e0, e1, e2;
+    HIPTRY(hipEventCreate(&e0));
+    HIPTRY(hipEventCreate(&e1));
+    HIPTRY(hipEventCreate(&e2));
+    int rc = VH_OK;
+    float trainMs = 0.0f, renderMs = 0.0f;
+    for (uint32_t f = 0; f < frames && rc == VH_OK; f++) {
+        uint32_t frame = nn->frames + f;
+        (void)hipEventRecord(e0, s);
+        // 1. training paths on a random pixel subset (their own sample stream) and Adam steps
+        Params PT = base;
+        PT.w[6] = 0x40000000u + frame;
+        rc = launch_nrc_train_paths(S, PT, T, nn->trainPix.as<uint32_t>(), nn->cfg.train_paths, (uint32_t)npix, frame,
+                                    nn->vertices.as<float4>(), nn->samples.as<float4>(), nn->sampleCount.as<uint32_t>(), sampleCap, cfg);
+        if (rc == VH_OK)
+            rc = train_steps(nn, nn->samples.as<float4>(), nn->sampleCount.as<uint32_t>(), 0, sampleCap, nn->cfg.train_steps, true, s);
+        (void)hipEventRecord(e1, s);
+        // 2. one sample of every pixel, terminated into the cache at the first bounce
+        Params P = base;
+        P.w[6] = a->samples + f;
+        P.w[31] = (uint32_t)npix;
+        if (rc == VH_OK)
+            rc = launch_wf_reset(P, W, s);
+        for (uint32_t d = 0; d < termDepth && rc == VH_OK; d++)
+            rc = launch_wf_depth(S, F, P, W, d, true, cfg);
+        if (rc == VH_OK)
+            rc = launch_wf_depth(S, F, P, W, termDepth, false, cfg);
+        if (rc == VH_OK)
+            rc = launch_nrc_query_shade(S, F, P, W, nn->queries.as<float4>(), termDepth, cfg);
+        if (rc == VH_OK) {
+            uint32_t grid = std::min<uint32_t>((uint32_t)((npix + kRows - 1) / kRows), r->ctx->cus * 2);
+            k_nrc_infer<1><<<grid, 256, 0, s>>>(net, nn->queries.as<float4>(), W.counts + termDepth, 0u, nullptr, F.film);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess)
+                rc = fail(VH_ERR_KERNEL, std::string("NRC inference: ") + hipGetErrorString(e));
+        }
+        for (uint32_t d = termDepth + 1; d < P.w[7] && rc == VH_OK; d++)
+            rc = launch_wf_depth(S, F, P, W, d, true, cfg);
+        (void)hipEventRecord(e2, s);
+        if (hipEventSynchronize(e2) != hipSuccess && rc == VH_OK)
+            rc = fail(VH_ERR_KERNEL, "NRC frame failed on the device");
+        float t1 = 0.0f, t2 = 0.0f;
+        (void)hipEventElapsedTime(&t1, e0, e1);
+        (void)hipEventElapsedTime(&t2, e1, e2);
+        trainMs += t1;
+        renderMs += t2;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    TRY(rc);
+    float lossHost[2];
+    HIPTRY(hipMemcpy(lossHost, nn->loss.p, 8, hipMemcpyDeviceToHost));
+    nn->lastLoss = lossHost[0];
+    nn->lastSamples = (uint32_t)lossHost[1];
+    nn->frames += frames;
+    nn->trainSeconds += (double)trainMs * 1.0e-3;
+    nn->renderSeconds += (double)renderMs * 1.0e-3;
+    a->samples += frames;
+    a->gpuSeconds += (double)(trainMs + renderMs) * 1.0e-3;
+    return VH_OK;
+}
+
+int vh_nrc_stats(const vh_nrc* nn, double out[4])
+{
+    if (!nn || !out)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    out[0] = (double)nn->lastLoss;
+    out[1] = nn->lastSamples;
+    out[2] = nn->trainSeconds;
+    out[3] = nn->renderSeconds;
+    return VH_OK;
+}
+
+} // extern "C"

@@ -226,6 +226,180 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(Scene S, Frame F, Params 
     }
 }
 
+// Block-wide exclusive prefix of per-thread counts k, one atomic per block on `counter`; returns
+// this thread's first index.
+__device__ __forceinline__ uint32_t compactN(uint32_t k, uint32_t* counter)
+{
+    __shared__ uint32_t s_wave[kWfBlock / 64];
+    __shared__ uint32_t s_base;
+    uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t x = k;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off)
+            x += y;
+    }
+    if (lane == 63)
+        s_wave[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t w = 0; w < kWfBlock / 64; w++) {
+            uint32_t c = s_wave[w];
+            s_wave[w] = run;
+            run += c;
+        }
+        s_base = run ? atomicAdd(counter, run) : 0u;
+    }
+    __syncthreads();
+    uint32_t r = s_base + s_wave[wave] + x - k;
+    __syncthreads();
+    return r;
+}
+
+// ---- neural radiance cache: training paths (plain paths of a pixel subset, every vertex recorded)
+
+__global__ void k_nrc_pick(uint32_t* trainPix, uint32_t nTrain, uint32_t npix, uint32_t frame, uint32_t* counts, uint32_t* sampleCount)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nTrain)
+        trainPix[i] = pcg(i ^ pcg(frame * 0x9E3779B9u + 0x2545F491u)) % npix;
+    if (i == 0) {
+        counts[0] = nTrain;
+        for (uint32_t k = 1; k <= kMaxDepthNrc; k++)
+            counts[k] = 0;
+        *sampleCount = 0;
+    }
+}
+
+__global__ __launch_bounds__(kWfBlock) void k_nrc_t_extend(Scene S, Params P, Wavefront W, const uint32_t* trainPix, uint32_t d)
+{
+    uint32_t count = W.counts[d];
+    const uint32_t* st = W.state[d & 1u];
+    for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < count; i += gridDim.x * kWfBlock) {
+        V3 org, dir;
+        if (d == 0) {
+            uint32_t p = trainPix[i];
+            PathState s;
+            generatePath(P, p % P.w[0], p / P.w[0], P.w[6], s);
+            org = s.org;
+            dir = s.dir;
+        } else {
+            org = get3(st, W.n, 0, i);
+            dir = get3(st, W.n, 3, i);
+        }
+        Hit h = trace<false>(S.bvh, org, dir, 1.0e30f);
+        W.hits[i] = make_uint4(h.hit ? 1u : 0u, h.inst, h.prim, asu(h.t));
+    }
+}
+
+// Shades training paths; a finished path turns each recorded vertex into a training sample
+// (inputs, target = (final total - total at the vertex) / throughput at the vertex).
+__global__ __launch_bounds__(kWfBlock) void k_nrc_t_shade(Scene S, Params P, Wavefront W, const uint32_t* trainPix, float4* vertices,
+                                                          float4* samples, uint32_t* sampleCount, uint32_t sampleCap, uint32_t d)
+{
+    Frame F{};
+    Ctx c{S, F, P};
+    uint32_t count = W.counts[d];
+    const uint32_t* in = W.state[d & 1u];
+    uint32_t* out = W.state[(d + 1) & 1u];
+    for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += gridDim.x * kWfBlock) {
+        uint32_t i = base + threadIdx.x;
+        bool keep = false, finished = false;
+        PathState st;
+        uint32_t tp = 0;
+        if (i < count) {
+            if (d == 0) {
+                tp = i;
+                uint32_t p = trainPix[i];
+                generatePath(P, p % P.w[0], p / P.w[0], P.w[6], st);
+                for (uint32_t j = 0; j < kMaxDepthNrc; j++)
+                    vertices[((size_t)tp * kMaxDepthNrc + j) * kNrcVertexWords + 4].w = 0.0f;
+            } else {
+                loadPath<false>(in, W.n, i, st, tp);
+            }
+            uint4 hw = W.hits[i];
+            Hit h;
+            h.hit = hw.x != 0;
+            h.inst = hw.y;
+            h.prim = hw.z;
+            h.t = asf(hw.w);
+            h.b1 = h.b2 = 0.0f;
+            h.gid = 0;
+            NrcIo io{vertices, nullptr, tp, i, 0u, false};
+            keep = shadePath<false, false, 1>(c, st, h, d, tp, &io) && d + 1 < P.w[7];
+            finished = !keep;
+        }
+        uint32_t nvalid = 0;
+        const float4* vrec = vertices + (size_t)tp * kMaxDepthNrc * kNrcVertexWords;
+        if (finished && !bad(st.parts.total)) {
+            for (uint32_t j = 0; j < kMaxDepthNrc; j++)
+                nvalid += vrec[j * kNrcVertexWords + 4].w != 0.0f ? 1u : 0u;
+        }
+        uint32_t first = compactN(nvalid, sampleCount);
+        if (nvalid) {
+            V3 T = st.parts.total;
+            uint32_t o = first;
+            for (uint32_t j = 0; j < kMaxDepthNrc && o < sampleCap; j++) {
+                const float4* v = vrec + j * kNrcVertexWords;
+                if (v[4].w == 0.0f)
+                    continue;
+                float4 thr = v[3], tae = v[4];
+                float4 tgt = make_float4(thr.x > 1.0e-6f ? (T.x - tae.x) / thr.x : 0.0f, thr.y > 1.0e-6f ? (T.y - tae.y) / thr.y : 0.0f,
+                                         thr.z > 1.0e-6f ? (T.z - tae.z) / thr.z : 0.0f, 1.0f);
+                float4* s = samples + (size_t)o * 4;
+                s[0] = v[0];
+                s[1] = v[1];
+                s[2] = v[2];
+                s[3] = tgt;
+                o++;
+            }
+        }
+        uint32_t slot = compact(keep, &W.counts[d + 1]);
+        if (keep && slot < W.n)
+            storePath<false>(out, W.n, slot, st, tp);
+    }
+}
+
+// ---- neural radiance cache: render paths, shaded at the terminating depth
+
+__global__ __launch_bounds__(kWfBlock) void k_nrc_q_shade(Scene S, Frame F, Params P, Wavefront W, float4* queries, uint32_t d)
+{
+    Ctx c{S, F, P};
+    uint32_t count = W.counts[d];
+    const uint32_t* in = W.state[d & 1u];
+    uint32_t* out = W.state[(d + 1) & 1u];
+    for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += gridDim.x * kWfBlock) {
+        uint32_t i = base + threadIdx.x;
+        bool keep = false;
+        PathState st;
+        uint32_t pixel = 0;
+        if (i < count) {
+            loadPath<false>(in, W.n, i, st, pixel);
+            uint4 hw = W.hits[i];
+            Hit h;
+            h.hit = hw.x != 0;
+            h.inst = hw.y;
+            h.prim = hw.z;
+            h.t = asf(hw.w);
+            h.b1 = h.b2 = 0.0f;
+            h.gid = 0;
+            NrcIo io{nullptr, queries, 0u, i, d, false};
+            bool alive = shadePath<false, false, 2>(c, st, h, d, pixel, &io);
+            keep = alive && d + 1 < P.w[7];
+            if (!io.terminated) {
+                queries[(size_t)i * kNrcQueryWords + 4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f); // no query in this slot
+                if (!keep)
+                    finishPath<false>(c, st, pixel);
+            }
+        }
+        uint32_t slot = compact(keep, &W.counts[d + 1]);
+        if (keep && slot < W.n)
+            storePath<false>(out, W.n, slot, st, pixel);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_primary(Scene S, Params P, uint4* out)
 {
     uint32_t n = P.w[0] * P.w[1];
@@ -299,6 +473,50 @@ int launch_wf_sample(const Scene& S, const Frame& F, const Params& P0, const Wav
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("wavefront launch: ") + hipGetErrorString(e));
+}
+
+int launch_nrc_train_paths(const Scene& S, const Params& P, const Wavefront& W, uint32_t* trainPix, uint32_t nTrain, uint32_t npix,
+                           uint32_t frame, float4* vertices, float4* samples, uint32_t* sampleCount, uint32_t sampleCap,
+                           const LaunchCfg& cfg)
+{
+    hipStream_t s = cfg.stream;
+    uint32_t grid = blocks_for(nTrain, kWfBlock, cfg.wfGrid ? cfg.wfGrid : 1024);
+    k_nrc_pick<<<(nTrain + 255) / 256, 256, 0, s>>>(trainPix, nTrain, npix, frame, W.counts, sampleCount);
+    for (uint32_t d = 0; d < P.w[7]; d++) {
+        k_nrc_t_extend<<<grid, kWfBlock, 0, s>>>(S, P, W, trainPix, d);
+        k_nrc_t_shade<<<grid, kWfBlock, 0, s>>>(S, P, W, trainPix, vertices, samples, sampleCount, sampleCap, d);
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("NRC training paths: ") + hipGetErrorString(e));
+}
+
+int launch_wf_reset(const Params& P, const Wavefront& W, hipStream_t s)
+{
+    k_wf_reset<<<1, 64, 0, s>>>(W.counts, W.n, 0u, P.w[7]);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("wavefront reset: ") + hipGetErrorString(e));
+}
+
+int launch_wf_depth(const Scene& S, const Frame& F, const Params& P0, const Wavefront& W, uint32_t d, bool shade, const LaunchCfg& cfg)
+{
+    Params P = P0;
+    P.w[30] = d;
+    uint32_t grid = cfg.wfGrid ? cfg.wfGrid : 1024;
+    k_wf_extend<<<grid, kWfBlock, 0, cfg.stream>>>(S, P, W, d);
+    if (shade)
+        k_wf_shade<false, false><<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("wavefront depth: ") + hipGetErrorString(e));
+}
+
+int launch_nrc_query_shade(const Scene& S, const Frame& F, const Params& P0, const Wavefront& W, float4* queries, uint32_t d,
+                           const LaunchCfg& cfg)
+{
+    Params P = P0;
+    P.w[30] = d;
+    k_nrc_q_shade<<<cfg.wfGrid ? cfg.wfGrid : 1024, kWfBlock, 0, cfg.stream>>>(S, F, P, W, queries, d);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("NRC query shade: ") + hipGetErrorString(e));
 }
 
 int launch_primary_hits(const Scene& S, const Params& P, uint4* out, hipStream_t s)

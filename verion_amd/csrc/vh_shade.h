@@ -712,10 +712,30 @@ __device__ __forceinline__ void restirClear(const Ctx& c, uint32_t pixel)
     C[27] = asf(0u);
 }
 
+// ---------------------------------------------------------------- neural radiance cache hook
+
+// What shadePath exchanges with the radiance cache. NRC = 1 (training paths): every opaque vertex
+// records, after its own emission is added, the cache's inputs (position, outgoing direction,
+// shading normal, roughness) with the path throughput and radiance total at that point; the
+// radiance the rest of the path adds, divided by that throughput, is the vertex's training target.
+// NRC = 2 (render paths): the first opaque vertex at `terminateDepth` hands the same inputs, the
+// throughput and the total to the cache (a query record) and the path ends there.
+constexpr uint32_t kNrcVertexWords = 5; // float4 per recorded vertex
+constexpr uint32_t kNrcQueryWords = 5;  // float4 per query
+struct NrcIo {
+    float4* vertices; // NRC 1: [path][kMaxDepthNrc][kNrcVertexWords]
+    float4* queries;  // NRC 2: [slot][kNrcQueryWords]
+    uint32_t path;    // NRC 1: training path id
+    uint32_t slot;    // NRC 2: the path's queue slot at this depth
+    uint32_t terminateDepth;
+    bool terminated;  // out (NRC 2): the path was handed to the cache
+};
+constexpr uint32_t kMaxDepthNrc = 8;
+
 // ---------------------------------------------------------------- shadePath (pt_core.slang:992-1543)
 
-template <bool RESTIR, bool SPLIT>
-__device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t depth, uint32_t pixel)
+template <bool RESTIR, bool SPLIT, int NRC = 0>
+__device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t depth, uint32_t pixel, NrcIo* io = nullptr)
 {
     const Scene& S = c.S;
     const Params& P = c.P;
@@ -843,6 +863,29 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
                 wB = st.qPrev * st.qPrev / (st.qPrev * st.qPrev + pl * pl);
             }
             partsAdd(st.parts, depth, cls, thr * le * wB, SPLIT);
+        }
+        if constexpr (NRC == 1) {
+            if (depth < kMaxDepthNrc) {
+                float4* v = io->vertices + ((size_t)io->path * kMaxDepthNrc + depth) * kNrcVertexWords;
+                v[0] = make_float4(hit.x, hit.y, hit.z, M[7] * texRMx);
+                v[1] = make_float4(wo.x, wo.y, wo.z, 0.0f);
+                v[2] = make_float4(n.x, n.y, n.z, 0.0f);
+                v[3] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+                v[4] = make_float4(st.parts.total.x, st.parts.total.y, st.parts.total.z, 1.0f); // w: recorded
+            }
+        }
+        if constexpr (NRC == 2) {
+            if (depth >= io->terminateDepth) {
+                float4* q = io->queries + (size_t)io->slot * kNrcQueryWords;
+                q[0] = make_float4(hit.x, hit.y, hit.z, M[7] * texRMx);
+                q[1] = make_float4(wo.x, wo.y, wo.z, asf(pixel));
+                q[2] = make_float4(n.x, n.y, n.z, 0.0f);
+                q[3] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+                q[4] = make_float4(st.parts.total.x, st.parts.total.y, st.parts.total.z, 1.0f); // w: a query
+                io->terminated = true;
+                alive = false;
+                break;
+            }
         }
         float metallic = M[6] * texRMy;
         float rough = M[7] * texRMx;
