@@ -7,9 +7,11 @@ Workload (the procedural scale scene): 1,024 seeded level-3 icospheres (1,310,72
 A step is one sample pass over every pixel (2,073,600 camera paths); the default 64 timed steps are
 the workload's 64 spp. Inputs (scene tables, LBVH) are resident in HBM before timing starts.
 
-Prints one JSON line with the throughput, a roofline entry for the dominant kernel (achieved
-algorithmic bytes per launch over its HIP-event launch time, against 8 TB/s) and the CPU oracle's
-throughput on a bounded sample of the same workload.
+Prints one JSON line with the throughput (`value`, plain wavefront), a roofline entry for the
+dominant kernel (achieved algorithmic bytes per launch over its HIP-event launch time, against
+8 TB/s), the CPU oracle's throughput on a bounded sample of the same workload, and `time_to_rmse`:
+GPU seconds to relMSE 1e-3 of the neural radiance cache and of the plain tracer on the same scene
+at 1080p, 1 spp per frame, against a plain reference render (4096 spp by default).
 
     python bench.py [--gpus 1] [--steps 64] [--warmup 4]
 """
@@ -98,6 +100,9 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--integrator", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-nrc", action="store_true", help="skip the NRC vs plain time-to-relMSE measurement")
+    ap.add_argument("--nrc-ref-spp", type=int, default=4096,
+                    help="spp of the plain reference render the relMSE ladders are scored against")
     ap.add_argument("--no-counting", action="store_true",
                     help="skip the traversal-counting pass (for profiler runs: every launch is then a timed-workload launch)")
     args = ap.parse_args()
@@ -161,7 +166,7 @@ def main():
                 "extend_ms_per_launch": round(ext_s / launches * 1e3, 4),
                 "shade_ms_per_launch": round(sh_s / launches * 1e3, 4)}
 
-    line = {"metric": "Msamples/s at 1080p (plain wavefront, procedural 1.31M-tri scene)", "value": round(value, 3),
+    line = {"metric": "Msamples/s at 1080p + time-to-rMSE 1e-3: NRC vs plain wavefront, 1 MI355X", "value": round(value, 3),
             "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -176,7 +181,30 @@ def main():
         base, parity = cpu_baseline(packed, W, H, gpu1.film())
         line["cpu_baseline"] = base
         line["parity_full_size"] = parity
+    if not args.no_nrc:
+        line["time_to_rmse"] = time_to_rmse(W, H, args.nrc_ref_spp)
     print(json.dumps(line), flush=True)
+
+
+def time_to_rmse(width, height, ref_spp):
+    """The second half of the metric: GPU seconds to relMSE 1e-3 of the NRC and of the plain
+    wavefront tracer, both 1 spp per frame on the same workload, scored against a plain reference
+    render of `ref_spp` spp with an independent seed; and the NRC's time to the plain tracer's
+    relMSE at 256 frames as a fraction of the plain tracer's time."""
+    from verion_amd import nrc_eval
+
+    res = nrc_eval.evaluate("procedural", width, height, ref_spp=ref_spp, max_frames=256, plain_max=2048,
+                            match_frames=256, log=log)
+    v = res["nrc"][0]
+    tp, tn = res["plain"]["time_to_target_s"], v["time_to_target_s"]
+    m = v["at_plain_error"] or {}
+    return {"target_relmse": res["target_relmse"], "relmse": "mean over pixels and channels of (x-r)^2/(r^2+0.01)",
+            "reference_spp": ref_spp, "seconds": "GPU, HIP events on the library stream (NRC: training included)",
+            "plain_s": tp, "nrc_s": tn, "nrc_over_plain": (tn / tp) if (tn and tp) else None,
+            "plain_relmse_256": m.get("plain_relmse"), "plain_s_256": m.get("plain_seconds"),
+            "nrc_s_to_plain_relmse_256": m.get("nrc_seconds"), "fraction_at_plain_relmse_256": m.get("fraction"),
+            "nrc_config": v["config"], "nrc_train_s": v["train_seconds"], "nrc_render_s": v["render_seconds"],
+            "plain_ladder": res["plain"]["ladder"], "nrc_ladder": v["ladder"]}
 
 
 if __name__ == "__main__":

@@ -226,17 +226,22 @@ typedef struct vh_nrc_config {
     uint32_t train_paths;     /* training paths per frame (65,536) */
     uint32_t train_steps;     /* Adam steps per frame */
     float learning_rate;      /* 1e-2 */
-    uint32_t terminate_depth; /* render paths end in the cache at their first opaque vertex at this depth (1) */
+    uint32_t terminate_depth; /* render paths end in the cache at their first opaque vertex at this depth (1..7) */
+    float ema_decay;          /* inference uses the bias-corrected exponential moving average of the
+                                 parameters over Adam steps with this decay; 0 = the parameters themselves */
+    uint32_t warmup_frames;   /* training-only frames (paths + Adam steps, no film) before the first
+                                 rendered frame of the cache; their time counts as the render's */
 } vh_nrc_config;
 typedef struct vh_nrc vh_nrc;
 
-/* NULL config = {65536, 1, 1e-2, 1}. The cache starts with zero parameters: set them first. */
+/* NULL config = {65536, 8, 1e-2, 1, 0.95, 16}. The cache starts with zero parameters: set them first. */
 int vh_nrc_create(vh_renderer* r, const vh_nrc_config* config, vh_nrc** out);
 void vh_nrc_destroy(vh_nrc* nrc);
-/* Replaces the FP32 master parameters (n = 23888) and resets Adam. */
+/* Replaces the FP32 master parameters (n = 23888), their moving average, and resets Adam. */
 int vh_nrc_set_params(vh_nrc* nrc, const float* params, uint64_t n);
 int vh_nrc_get_params(vh_nrc* nrc, float* params, uint64_t n);
-/* Fused inference of n raw queries: 3 floats out per query. */
+/* Fused inference of n raw queries with the parameters the integrator uses (the moving average):
+ * 3 floats out per query. */
 int vh_nrc_infer(vh_nrc* nrc, uint64_t n, const float* raw12, float* out3);
 /* Gradient of the mean relative L2 loss over a batch (23888 floats) and the loss; no update. */
 int vh_nrc_gradient(vh_nrc* nrc, uint64_t n, const float* raw12, const float* target3, float* grad, float* loss);
@@ -247,7 +252,7 @@ int vh_nrc_train(vh_nrc* nrc, uint64_t n, const float* raw12, const float* targe
  * cache at its first opaque vertex at terminate_depth, into `acc` (FP64 film). Not with the
  * lighting split, adaptive sampling or ReSTIR (VH_ERR_UNSUPPORTED). */
 int vh_nrc_accumulate(vh_nrc* nrc, vh_accumulation* acc, uint32_t frames);
-/* Last loss, last training-sample count, total training seconds, total rendering seconds. */
+/* Last loss, last training-sample count, total training seconds (warm-up included), total rendering seconds. */
 int vh_nrc_stats(const vh_nrc* nrc, double out[4]);
 
 #ifdef __cplusplus

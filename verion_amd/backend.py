@@ -345,10 +345,10 @@ class Nrc(_Owned):
 
     PARAMS = 23888
 
-    def __init__(self, renderer: Renderer, train_paths: int = 65536, train_steps: int = 1, learning_rate: float = 1e-2,
-                 terminate_depth: int = 1):
+    def __init__(self, renderer: Renderer, train_paths: int = 65536, train_steps: int = 8, learning_rate: float = 1e-2,
+                 terminate_depth: int = 1, ema_decay: float = 0.95, warmup_frames: int = 16):
         self.renderer = renderer
-        cfg = abi.NrcConfig(train_paths, train_steps, learning_rate, terminate_depth)
+        cfg = abi.NrcConfig(train_paths, train_steps, learning_rate, terminate_depth, ema_decay, warmup_frames)
         h = C.c_void_p()
         _check(library().vh_nrc_create(renderer.h, C.byref(cfg), C.byref(h)))
         self._handle = _Handle(h, library().vh_nrc_destroy, renderer._handle)

@@ -369,7 +369,8 @@ __global__ void k_nrc_reduce(const float* partials, uint32_t blocks, float* grad
     }
 }
 
-// Parameter p -> its bf16 copies (forward [n][k] and backward [k][npad]); biases have none.
+// Parameter p -> its bf16 copies (forward [n][k] and, when wb is given, backward [k][npad]);
+// biases have none.
 __device__ __forceinline__ void writeCopies(uint32_t p, float v, __bf16* wt, __bf16* wb)
 {
 #pragma unroll
@@ -379,13 +380,25 @@ __device__ __forceinline__ void writeCopies(uint32_t p, float v, __bf16* wt, __b
             int e = (int)p - wOff(l);
             int k = e / N, nn = e % N;
             wt[wtOff(l) + nn * K + k] = (__bf16)v;
-            wb[wbOff(l) + k * layerNpad(l) + nn] = (__bf16)v;
+            if (wb)
+                wb[wbOff(l) + k * layerNpad(l) + nn] = (__bf16)v;
         }
     }
 }
 
+// The moving average the integrator infers with: avg = decay avg + (1 - decay) theta over Adam
+// steps from avg = 0, read as avg / (1 - decay^step) (bias-corrected like Adam's moments);
+// decay 0 makes it theta itself.
+struct NrcAverage {
+    float* avg;    // running sum
+    float* eval;   // corrected average, fp32 (biases are read from here)
+    __bf16* ewt;   // its forward bf16 copy
+    float decay;
+    float correction; // 1 - decay^step
+};
+
 __global__ void k_nrc_adam(float* theta, float* m, float* v, const float* grad, float lr, float b1, float b2, float c1, float c2,
-                           float eps, __bf16* wt, __bf16* wb)
+                           float eps, __bf16* wt, __bf16* wb, NrcAverage A)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (uint32_t)kParams)
@@ -398,13 +411,24 @@ __global__ void k_nrc_adam(float* theta, float* m, float* v, const float* grad, 
     float th = theta[p] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
     theta[p] = th;
     writeCopies(p, th, wt, wb);
+    float a = A.decay * A.avg[p] + (1.0f - A.decay) * th;
+    A.avg[p] = a;
+    float e = A.decay > 0.0f ? a / A.correction : th;
+    A.eval[p] = e;
+    writeCopies(p, e, A.ewt, nullptr);
 }
 
-__global__ void k_nrc_copies(const float* theta, __bf16* wt, __bf16* wb)
+// theta -> its bf16 copies; the average restarts at theta (its running sum at zero).
+__global__ void k_nrc_copies(const float* theta, __bf16* wt, __bf16* wb, NrcAverage A)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < (uint32_t)kParams)
-        writeCopies(p, theta[p], wt, wb);
+    if (p < (uint32_t)kParams) {
+        float th = theta[p];
+        writeCopies(p, th, wt, wb);
+        A.avg[p] = 0.0f;
+        A.eval[p] = th;
+        writeCopies(p, th, A.ewt, nullptr);
+    }
 }
 
 __global__ void k_zero_bf16(__bf16* p, uint32_t n)
@@ -426,24 +450,51 @@ struct vh_nrc {
     vh_renderer* r = nullptr;
     vh_nrc_config cfg{};
     DevBuf theta, m, v, grad, wt, wb, partials, lossPart, loss;
-    DevBuf trainPix, vertices, samples, sampleCount, tstate0, tstate1, thits, tcounts, queries;
+    DevBuf avg, evalTheta, ewt; // moving average of theta and its forward copy (inference)
+    DevBuf trainPix, vertices, tstate0, tstate1, thits, tcounts, queries;
+    DevBuf samples[2], sampleCount[2]; // training samples of alternate training frames
+    // Training paths run on their own stream beside the render; pathsDone[b] / adamDone[b] order
+    // the writes and reads of samples[b] between the two streams.
+    hipStream_t side = nullptr;
+    hipEvent_t pathsDone[2] = {nullptr, nullptr}, adamDone[2] = {nullptr, nullptr};
+    bool adamIssued[2] = {false, false};
     uint32_t step = 0;
     uint32_t frames = 0;
+    uint32_t trainFrames = 0; // frames of training paths drawn so far (warm-up included)
     double trainSeconds = 0.0, renderSeconds = 0.0;
     float lastLoss = 0.0f;
     uint32_t lastSamples = 0;
-    NrcNet net(bool unitBox) const
+    NrcAverage average() const
+    {
+        float corr = 1.0f - powf(cfg.ema_decay, (float)step);
+        return NrcAverage{avg.as<float>(), evalTheta.as<float>(), ewt.as<__bf16>(), cfg.ema_decay, corr > 0.0f ? corr : 1.0f};
+    }
+    // the training network (master parameters) or the inference network (their moving average)
+    NrcNet net(bool unitBox, bool inference = false) const
     {
         NrcNet n;
-        n.theta = theta.as<float>();
-        n.wt = wt.as<__bf16>();
-        n.wb = wb.as<__bf16>();
+        n.theta = inference ? evalTheta.as<float>() : theta.as<float>();
+        n.wt = inference ? ewt.as<__bf16>() : wt.as<__bf16>();
+        n.wb = inference ? nullptr : wb.as<__bf16>();
         for (int k = 0; k < 3; k++) {
             float lo = unitBox ? 0.0f : r->bvh.lo[k], hi = unitBox ? 1.0f : r->bvh.hi[k];
             n.lo[k] = lo;
             n.inv[k] = 1.0f / fmaxf(hi - lo, 1.0e-20f);
         }
         return n;
+    }
+    ~vh_nrc()
+    {
+        if (side)
+            (void)hipStreamSynchronize(side);
+        for (int b = 0; b < 2; b++) {
+            if (pathsDone[b])
+                (void)hipEventDestroy(pathsDone[b]);
+            if (adamDone[b])
+                (void)hipEventDestroy(adamDone[b]);
+        }
+        if (side)
+            (void)hipStreamDestroy(side);
     }
 };
 
@@ -470,7 +521,7 @@ int train_steps(vh_nrc* nn, const float4* samples, const uint32_t* countPtr, uin
             float c1 = 1.0f - powf(0.9f, (float)nn->step), c2 = 1.0f - powf(0.99f, (float)nn->step);
             k_nrc_adam<<<(kParams + 255) / 256, 256, 0, s>>>(nn->theta.as<float>(), nn->m.as<float>(), nn->v.as<float>(),
                                                              nn->grad.as<float>(), nn->cfg.learning_rate, 0.9f, 0.99f, c1, c2, 1.0e-8f,
-                                                             nn->wt.as<__bf16>(), nn->wb.as<__bf16>());
+                                                             nn->wt.as<__bf16>(), nn->wb.as<__bf16>(), nn->average());
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess)
@@ -507,18 +558,29 @@ int vh_nrc_create(vh_renderer* r, const vh_nrc_config* cfg, vh_nrc** out)
     *out = nullptr;
     std::unique_ptr<vh_nrc> nn(new vh_nrc());
     nn->r = r;
-    nn->cfg = cfg ? *cfg : vh_nrc_config{65536, 1, 1.0e-2f, 1};
+    nn->cfg = cfg ? *cfg : vh_nrc_config{65536, 8, 1.0e-2f, 1, 0.95f, 16};
     if (nn->cfg.train_paths == 0 || nn->cfg.train_steps == 0)
         return fail(VH_ERR_INVALID_ARGUMENT, "train_paths and train_steps must be positive");
+    if (nn->cfg.train_paths > (1u << 24))
+        return fail(VH_ERR_INVALID_ARGUMENT, "train_paths is at most 2^24");
+    if (nn->cfg.terminate_depth < 1 || nn->cfg.terminate_depth >= kMaxDepthNrc)
+        return fail(VH_ERR_INVALID_ARGUMENT, "terminate_depth must be in [1, 7]");
+    if (!(nn->cfg.ema_decay >= 0.0f && nn->cfg.ema_decay < 1.0f))
+        return fail(VH_ERR_INVALID_ARGUMENT, "ema_decay must be in [0, 1)");
+    if (!(nn->cfg.learning_rate > 0.0f))
+        return fail(VH_ERR_INVALID_ARGUMENT, "learning_rate must be positive");
     hipStream_t s = r->ctx->stream;
     uint64_t tp = nn->cfg.train_paths;
-    uint64_t bytes = 4ull * kParams * 4 + 2ull * (kWt + kWb) + 4ull * kParams * kTrainBlocks + tp * (4 + 16 * 5 * kMaxDepthNrc + 64 * kMaxDepthNrc + 2 * 4 * 17 + 16);
+    uint64_t bytes = 4ull * kParams * 6 + 2ull * (2 * kWt + kWb) + 4ull * kParams * kTrainBlocks + tp * (4 + 16 * 5 * kMaxDepthNrc + 2 * 64 * kMaxDepthNrc + 2 * 4 * 17 + 16);
     TRY(check_fits(bytes));
     HIPTRY(nn->theta.alloc(4ull * kParams));
     HIPTRY(nn->m.alloc(4ull * kParams));
     HIPTRY(nn->v.alloc(4ull * kParams));
     HIPTRY(nn->grad.alloc(4ull * kParams));
+    HIPTRY(nn->avg.alloc(4ull * kParams));
+    HIPTRY(nn->evalTheta.alloc(4ull * kParams));
     HIPTRY(nn->wt.alloc(2ull * kWt));
+    HIPTRY(nn->ewt.alloc(2ull * kWt));
     HIPTRY(nn->wb.alloc(2ull * kWb));
     HIPTRY(nn->partials.alloc(4ull * kParams * kTrainBlocks));
     HIPTRY(nn->lossPart.alloc(4ull * kTrainBlocks));
@@ -526,12 +588,17 @@ int vh_nrc_create(vh_renderer* r, const vh_nrc_config* cfg, vh_nrc** out)
     HIPTRY(hipMemsetAsync(nn->theta.p, 0, 4ull * kParams, s));
     HIPTRY(hipMemsetAsync(nn->m.p, 0, 4ull * kParams, s));
     HIPTRY(hipMemsetAsync(nn->v.p, 0, 4ull * kParams, s));
-    k_zero_bf16<<<(kWb + 255) / 256, 256, 0, s>>>(nn->wb.as<__bf16>(), kWb);
-    k_zero_bf16<<<(kWt + 255) / 256, 256, 0, s>>>(nn->wt.as<__bf16>(), kWt);
+    k_zero_bf16<<<(kWb + 255) / 256, 256, 0, s>>>(nn->wb.as<__bf16>(), kWb); // padding columns stay zero
+    k_nrc_copies<<<(kParams + 255) / 256, 256, 0, s>>>(nn->theta.as<float>(), nn->wt.as<__bf16>(), nn->wb.as<__bf16>(), nn->average());
     HIPTRY(nn->trainPix.alloc(4 * tp));
     HIPTRY(nn->vertices.alloc(16ull * kNrcVertexWords * kMaxDepthNrc * tp));
-    HIPTRY(nn->samples.alloc(64ull * kMaxDepthNrc * tp));
-    HIPTRY(nn->sampleCount.alloc(16));
+    for (int b = 0; b < 2; b++) {
+        HIPTRY(nn->samples[b].alloc(64ull * kMaxDepthNrc * tp));
+        HIPTRY(nn->sampleCount[b].alloc(16));
+        HIPTRY(hipEventCreateWithFlags(&nn->pathsDone[b], hipEventDisableTiming));
+        HIPTRY(hipEventCreateWithFlags(&nn->adamDone[b], hipEventDisableTiming));
+    }
+    HIPTRY(hipStreamCreateWithFlags(&nn->side, hipStreamNonBlocking));
     HIPTRY(nn->tstate0.alloc(4ull * 17 * tp));
     HIPTRY(nn->tstate1.alloc(4ull * 17 * tp));
     HIPTRY(nn->thits.alloc(16 * tp));
@@ -559,10 +626,10 @@ int vh_nrc_set_params(vh_nrc* nn, const float* theta, uint64_t n)
     HIPTRY(hipMemcpyAsync(nn->theta.p, theta, 4ull * kParams, hipMemcpyHostToDevice, s));
     HIPTRY(hipMemsetAsync(nn->m.p, 0, 4ull * kParams, s));
     HIPTRY(hipMemsetAsync(nn->v.p, 0, 4ull * kParams, s));
-    k_nrc_copies<<<(kParams + 255) / 256, 256, 0, s>>>(nn->theta.as<float>(), nn->wt.as<__bf16>(), nn->wb.as<__bf16>());
+    nn->step = 0;
+    k_nrc_copies<<<(kParams + 255) / 256, 256, 0, s>>>(nn->theta.as<float>(), nn->wt.as<__bf16>(), nn->wb.as<__bf16>(), nn->average());
     HIPTRY(hipGetLastError());
     HIPTRY(hipStreamSynchronize(s));
-    nn->step = 0;
     return VH_OK;
 }
 
@@ -589,7 +656,7 @@ int vh_nrc_infer(vh_nrc* nn, uint64_t n, const float* raw12, float* out3)
     HIPTRY(out.alloc(12 * n));
     HIPTRY(hipMemcpyAsync(in.p, host.data(), 4 * host.size(), hipMemcpyHostToDevice, s));
     uint32_t grid = (uint32_t)std::min<uint64_t>((n + kRows - 1) / kRows, 4096);
-    k_nrc_infer<0><<<grid, 256, 0, s>>>(nn->net(true), in.as<float4>(), nullptr, (uint32_t)n, out.as<float>(), nullptr);
+    k_nrc_infer<0><<<grid, 256, 0, s>>>(nn->net(true, true), in.as<float4>(), nullptr, (uint32_t)n, out.as<float>(), nullptr);
     HIPTRY(hipGetLastError());
     HIPTRY(hipStreamSynchronize(s));
     HIPTRY(hipMemcpy(out3, out.p, 12 * n, hipMemcpyDeviceToHost));
@@ -632,6 +699,8 @@ int vh_nrc_accumulate(vh_nrc* nn, vh_accumulation* a, uint32_t frames)
     vh_renderer* r = nn->r;
     if (a->lighting || r->adaptiveOn || r->restirOn || a->adaptiveOn || a->restirOn)
         return fail(VH_ERR_UNSUPPORTED, "the radiance cache runs without the lighting split, adaptive sampling and ReSTIR");
+    if (frames == 0)
+        return VH_OK;
     hipStream_t s = r->ctx->stream;
     TRY(ensure_wavefront(a));
     uint64_t npix = a->pixels();
@@ -649,60 +718,103 @@ int vh_nrc_accumulate(vh_nrc* nn, vh_accumulation* a, uint32_t frames)
     T.n = nn->cfg.train_paths;
     T.words = 17;
     LaunchCfg cfg{s, r->wavefrontBlocks()};
+    LaunchCfg sideCfg{nn->side, r->wavefrontBlocks()};
     Params base = base_params(r, a->width, a->height, false, false, vh_adaptive{}, false, vh_restir{});
-    NrcNet net = nn->net(false);
-    const uint32_t termDepth = 1;
-    uint32_t sampleCap = nn->cfg.train_paths * kMaxDepthNrc;
-    hipEvent_t e0, e1, e2;
-    HIPTRY(hipEventCreate(&e0));
-    HIPTRY(hipEventCreate(&e1));
-    HIPTRY(hipEventCreate(&e2));
-    int rc = VH_OK;
-    float trainMs = 0.0f, renderMs = 0.0f;
-    for (uint32_t f = 0; f < frames && rc == VH_OK; f++) {
-        uint32_t frame = nn->frames + f;
-        (void)hipEventRecord(e0, s);
-        // 1. training paths on a random pixel subset (their own sample stream) and Adam steps
+    const uint32_t termDepth = nn->cfg.terminate_depth;
+    const uint32_t maxDepth = base.w[7];
+    const uint32_t sampleCap = nn->cfg.train_paths * kMaxDepthNrc;
+    // Training frame tf: plain paths on a random pixel subset (their own sample stream) on the side
+    // stream into samples[tf & 1], after the Adam steps that last read that buffer; then Adam steps
+    // on the render stream once those paths are done.
+    auto issuePaths = [&](uint32_t tf) -> int {
+        uint32_t b = tf & 1u;
+        if (nn->adamIssued[b])
+            HIPTRY(hipStreamWaitEvent(nn->side, nn->adamDone[b], 0));
         Params PT = base;
-        PT.w[6] = 0x40000000u + frame;
-        rc = launch_nrc_train_paths(S, PT, T, nn->trainPix.as<uint32_t>(), nn->cfg.train_paths, (uint32_t)npix, frame,
-                                    nn->vertices.as<float4>(), nn->samples.as<float4>(), nn->sampleCount.as<uint32_t>(), sampleCap, cfg);
-        if (rc == VH_OK)
-            rc = train_steps(nn, nn->samples.as<float4>(), nn->sampleCount.as<uint32_t>(), 0, sampleCap, nn->cfg.train_steps, true, s);
-        (void)hipEventRecord(e1, s);
-        // 2. one sample of every pixel, terminated into the cache at the first bounce
+        PT.w[6] = 0x40000000u + tf;
+        TRY(launch_nrc_train_paths(S, PT, T, nn->trainPix.as<uint32_t>(), nn->cfg.train_paths, (uint32_t)npix, tf,
+                                   nn->vertices.as<float4>(), nn->samples[b].as<float4>(), nn->sampleCount[b].as<uint32_t>(), sampleCap,
+                                   sideCfg));
+        HIPTRY(hipEventRecord(nn->pathsDone[b], nn->side));
+        return VH_OK;
+    };
+    auto adam = [&](uint32_t tf) -> int {
+        uint32_t b = tf & 1u;
+        HIPTRY(hipStreamWaitEvent(s, nn->pathsDone[b], 0));
+        TRY(train_steps(nn, nn->samples[b].as<float4>(), nn->sampleCount[b].as<uint32_t>(), 0, sampleCap, nn->cfg.train_steps, true, s));
+        HIPTRY(hipEventRecord(nn->adamDone[b], s));
+        nn->adamIssued[b] = true;
+        return VH_OK;
+    };
+    // one sample of every pixel, ending in the cache at its first opaque vertex at termDepth
+    auto render = [&](uint32_t f) -> int {
+        const NrcNet net = nn->net(false, true);
         Params P = base;
         P.w[6] = a->samples + f;
         P.w[31] = (uint32_t)npix;
+        TRY(launch_wf_reset(P, W, s));
+        for (uint32_t d = 0; d < std::min(termDepth, maxDepth); d++)
+            TRY(launch_wf_depth(S, F, P, W, d, true, cfg));
+        if (termDepth >= maxDepth)
+            return VH_OK;
+        TRY(launch_wf_depth(S, F, P, W, termDepth, false, cfg));
+        TRY(launch_nrc_query_shade(S, F, P, W, nn->queries.as<float4>(), termDepth, cfg));
+        uint32_t grid = std::min<uint32_t>((uint32_t)((npix + kRows - 1) / kRows), r->ctx->cus * 2);
+        k_nrc_infer<1><<<grid, 256, 0, s>>>(net, nn->queries.as<float4>(), W.counts + termDepth, 0u, nullptr, F.film);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return fail(VH_ERR_KERNEL, std::string("NRC inference: ") + hipGetErrorString(e));
+        for (uint32_t d = termDepth + 1; d < maxDepth; d++)
+            TRY(launch_wf_depth(S, F, P, W, d, true, cfg));
+        return VH_OK;
+    };
+    // Training frames of this call (the warm-up on the cache's first call, then one per frame); the
+    // paths of training frame j + 1 run beside the Adam steps and render of frame j. Timing events
+    // on the render stream: ev[0] at the start, then per training frame (after Adam, after render).
+    const uint32_t warm = nn->frames == 0 ? nn->cfg.warmup_frames : 0;
+    const uint32_t total = warm + frames;
+    std::vector<hipEvent_t> ev(1 + 2 * (size_t)total, nullptr);
+    for (auto& e : ev)
+        HIPTRY(hipEventCreate(&e));
+    int rc = check_hip(hipEventRecord(ev[0], s), "hipEventRecord");
+    // the side stream starts after everything issued before this call (scene uploads, parameters)
+    if (rc == VH_OK)
+        rc = check_hip(hipStreamWaitEvent(nn->side, ev[0], 0), "hipStreamWaitEvent");
+    const uint32_t tf0 = nn->trainFrames;
+    if (rc == VH_OK)
+        rc = issuePaths(tf0);
+    for (uint32_t j = 0; j < total && rc == VH_OK; j++) {
+        rc = adam(tf0 + j);
         if (rc == VH_OK)
-            rc = launch_wf_reset(P, W, s);
-        for (uint32_t d = 0; d < termDepth && rc == VH_OK; d++)
-            rc = launch_wf_depth(S, F, P, W, d, true, cfg);
+            rc = check_hip(hipEventRecord(ev[1 + 2 * j], s), "hipEventRecord");
+        if (rc == VH_OK && j + 1 < total)
+            rc = issuePaths(tf0 + j + 1);
+        if (rc == VH_OK && j >= warm)
+            rc = render(j - warm);
         if (rc == VH_OK)
-            rc = launch_wf_depth(S, F, P, W, termDepth, false, cfg);
-        if (rc == VH_OK)
-            rc = launch_nrc_query_shade(S, F, P, W, nn->queries.as<float4>(), termDepth, cfg);
-        if (rc == VH_OK) {
-            uint32_t grid = std::min<uint32_t>((uint32_t)((npix + kRows - 1) / kRows), r->ctx->cus * 2);
-            k_nrc_infer<1><<<grid, 256, 0, s>>>(net, nn->queries.as<float4>(), W.counts + termDepth, 0u, nullptr, F.film);
-            hipError_t e = hipGetLastError();
-            if (e != hipSuccess)
-                rc = fail(VH_ERR_KERNEL, std::string("NRC inference: ") + hipGetErrorString(e));
-        }
-        for (uint32_t d = termDepth + 1; d < P.w[7] && rc == VH_OK; d++)
-            rc = launch_wf_depth(S, F, P, W, d, true, cfg);
-        (void)hipEventRecord(e2, s);
-        if (hipEventSynchronize(e2) != hipSuccess && rc == VH_OK)
-            rc = fail(VH_ERR_KERNEL, "NRC frame failed on the device");
-        float t1 = 0.0f, t2 = 0.0f;
-        (void)hipEventElapsedTime(&t1, e0, e1);
-        (void)hipEventElapsedTime(&t2, e1, e2);
-        trainMs += t1;
-        renderMs += t2;
+            rc = check_hip(hipEventRecord(ev[2 + 2 * j], s), "hipEventRecord");
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
+    if (rc == VH_OK && hipEventSynchronize(ev[2 * total]) != hipSuccess)
+        rc = fail(VH_ERR_KERNEL, "NRC frames failed on the device");
+    if (rc != VH_OK) {
+        (void)hipStreamSynchronize(nn->side);
+        (void)hipStreamSynchronize(s);
+    }
+    float trainMs = 0.0f, renderMs = 0.0f;
+    for (uint32_t j = 0; j < total && rc == VH_OK; j++) {
+        float t1 = 0.0f, t2 = 0.0f;
+        (void)hipEventElapsedTime(&t1, ev[2 * j], ev[1 + 2 * j]);
+        (void)hipEventElapsedTime(&t2, ev[1 + 2 * j], ev[2 + 2 * j]);
+        if (j < warm)
+            trainMs += t1 + t2;
+        else {
+            trainMs += t1;
+            renderMs += t2;
+        }
+    }
+    for (auto e : ev)
+        (void)hipEventDestroy(e);
+    nn->trainFrames = tf0 + total;
     TRY(rc);
     float lossHost[2];
     HIPTRY(hipMemcpy(lossHost, nn->loss.p, 8, hipMemcpyDeviceToHost));

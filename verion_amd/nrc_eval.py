@@ -22,11 +22,18 @@ from . import quality, scenes
 from .backend import WAVEFRONT, Context, Nrc, Renderer
 
 
+DEFAULT_VARIANT = {"train_paths": 65536, "train_steps": 8, "learning_rate": 1e-2, "terminate_depth": 1, "ema_decay": 0.95,
+                   "warmup_frames": 16}
+
+
 def evaluate(scene: str = "cornell", width: int = 1920, height: int = 1080, ref_spp: int = 16384, max_frames: int = 256,
-             plain_max: int = 4096, match_frames: int = 256, train_paths: int = 65536, train_steps: int = 1,
-             learning_rate: float = 1e-2, target: float = 1e-3, seed: int = 1, log=lambda *a: None) -> dict:
+             plain_max: int = 4096, match_frames: int = 256, variants=None, target: float = 1e-3, seed: int = 1,
+             log=lambda *a: None) -> dict:
+    """Renders the reference, the plain ladder, then one NRC ladder per variant (a dict of
+    `Nrc` arguments over DEFAULT_VARIANT), each from freshly initialised parameters."""
     from oracle import nrc as nrc_init  # parameter initialisation only (He-uniform, seeded)
 
+    variants = variants or [{}]
     ctx = Context()
     s = scenes.cornell() if scene == "cornell" else scenes.procedural()
     r = Renderer(ctx, s)
@@ -39,6 +46,8 @@ def evaluate(scene: str = "cornell", width: int = 1920, height: int = 1080, ref_
     while done < ref_spp:
         r.accumulate(ref_acc, min(chunk, ref_spp - done))
         done += min(chunk, ref_spp - done)
+        if done % 1024 == 0 or done == ref_spp:
+            log(f"reference: {done}/{ref_spp} spp")
     ref = ref_acc.mean().astype(np.float64)
     ref_s = ref_acc.gpu_seconds()
     ref_acc.close()
@@ -53,38 +62,43 @@ def evaluate(scene: str = "cornell", width: int = 1920, height: int = 1080, ref_
         return plain.gpu_seconds() - before
 
     pts_plain = quality.ladder(plain_step, plain.mean, ref, plain_max, stop_below=target)
+    plain.close()
     log("plain", [(f, round(t, 4), e) for f, t, e in pts_plain])
-
-    cache = Nrc(r, train_paths=train_paths, train_steps=train_steps, learning_rate=learning_rate)
-    cache.set_params(nrc_init.init(seed))
-    acc = r.new_accumulation(width, height)
-
-    def nrc_step(k):
-        before = acc.gpu_seconds()
-        cache.accumulate(acc, k)
-        return acc.gpu_seconds() - before
-
-    pts_nrc = quality.ladder(nrc_step, acc.mean, ref, max_frames, stop_below=target)
-    log("nrc", [(f, round(t, 4), e) for f, t, e in pts_nrc], cache.stats())
-
     tp = quality.time_to([(t, e) for _, t, e in pts_plain], target)
-    tn = quality.time_to([(t, e) for _, t, e in pts_nrc], target)
     at = [p for p in pts_plain if p[0] == match_frames]
-    match = None
-    if at:
-        e_match, t_match = at[0][2], at[0][1]
-        t_nrc = quality.time_to([(t, e) for _, t, e in pts_nrc], e_match)
-        match = {"plain_frames": match_frames, "plain_relmse": e_match, "plain_seconds": t_match,
-                 "nrc_seconds": t_nrc, "fraction": (t_nrc / t_match) if t_nrc else None}
+
+    results = []
+    for v in variants:
+        cfg = dict(DEFAULT_VARIANT, **v)
+        cache = Nrc(r, **cfg)
+        cache.set_params(nrc_init.init(seed))
+        acc = r.new_accumulation(width, height)
+
+        def nrc_step(k):
+            before = acc.gpu_seconds()
+            cache.accumulate(acc, k)
+            return acc.gpu_seconds() - before
+
+        pts_nrc = quality.ladder(nrc_step, acc.mean, ref, max_frames, stop_below=target)
+        acc.close()
+        st = cache.stats()
+        cache.close()
+        log("nrc", cfg, [(f, round(t, 4), e) for f, t, e in pts_nrc], st)
+        tn = quality.time_to([(t, e) for _, t, e in pts_nrc], target)
+        match = None
+        if at:
+            e_match, t_match = at[0][2], at[0][1]
+            t_nrc = quality.time_to([(t, e) for _, t, e in pts_nrc], e_match)
+            match = {"plain_frames": match_frames, "plain_relmse": e_match, "plain_seconds": t_match,
+                     "nrc_seconds": t_nrc, "fraction": (t_nrc / t_match) if t_nrc else None}
+        results.append({"config": cfg, "time_to_target_s": tn, "floor_relmse": min(e for _, _, e in pts_nrc),
+                        "ladder": [[f, round(t, 5), e] for f, t, e in pts_nrc], **st, "at_plain_error": match})
     return {
         "scene": scene, "width": width, "height": height, "reference_spp": ref_spp, "relmse_eps": quality.EPS,
         "target_relmse": target,
         "plain": {"time_to_target_s": tp, "ladder": [[f, round(t, 5), e] for f, t, e in pts_plain],
                   "msamples_per_s": width * height * pts_plain[-1][0] / pts_plain[-1][1] / 1e6},
-        "nrc": {"time_to_target_s": tn, "floor_relmse": min(e for _, _, e in pts_nrc),
-                "ladder": [[f, round(t, 5), e] for f, t, e in pts_nrc], "train_paths": train_paths,
-                "train_steps": train_steps, "learning_rate": learning_rate, **cache.stats()},
-        "nrc_at_plain_error": match,
+        "nrc": results,
     }
 
 
@@ -96,12 +110,12 @@ def main():
     ap.add_argument("--ref-spp", type=int, default=16384)
     ap.add_argument("--frames", type=int, default=256)
     ap.add_argument("--plain-max", type=int, default=4096)
-    ap.add_argument("--train-paths", type=int, default=65536)
-    ap.add_argument("--train-steps", type=int, default=1)
-    ap.add_argument("--lr", type=float, default=1e-2)
+    ap.add_argument("--variant", action="append", default=[],
+                    help='JSON object of Nrc arguments, e.g. \'{"ema_decay": 0.99, "warmup_frames": 16}\'; repeatable')
     a = ap.parse_args()
-    res = evaluate(a.scene, a.width, a.height, a.ref_spp, a.frames, a.plain_max, min(a.frames, 256), a.train_paths,
-                   a.train_steps, a.lr, log=lambda *x: print(*x, file=sys.stderr, flush=True))
+    variants = [json.loads(v) for v in a.variant] or [{}]
+    res = evaluate(a.scene, a.width, a.height, a.ref_spp, a.frames, a.plain_max, min(a.frames, 256), variants,
+                   log=lambda *x: print(*x, file=sys.stderr, flush=True))
     print(json.dumps(res), flush=True)
 
 
