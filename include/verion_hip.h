@@ -192,9 +192,16 @@ int vh_acc_mean(const vh_accumulation* acc, float* out);
 
 /* --- primary-hit AOVs (render_normals / render_aovs, lib.rs:54-57) ---------------------------- */
 
-/* 3 floats per pixel. */
+/* render_normals (lib.rs:55; shaders/normals.slang:39-110): the pixel-centre ray's interpolated
+ * vertex normal (the geometric normal for meshes without normals; normal maps not applied), on the
+ * geometric normal's side and facing the viewer; 0 where the ray misses. 3 floats per pixel,
+ * row-major from the top-left. VH_ERR_SCENE for a zero image size (vulkan renderer.rs:856-858). */
 int vh_render_normals(vh_renderer* r, uint32_t width, uint32_t height, float* normals);
-/* albedo_depth: 4 floats per pixel; object_id, material_id: 1 word; coverage: 1 float. */
+/* render_aovs (lib.rs:57, struct Aovs lib.rs:165-176; shaders/aov.slang:84-188), per pixel from the
+ * pixel-centre ray: albedo_depth = base colour x level-0 base texture (3 floats) and the distance
+ * along the camera's forward axis; object_id = instance + 1, material_id = material + 1 (0 on a
+ * miss); coverage = fraction of 16 stratified rays (4x4 grid) that hit, through up to 8
+ * translucent layers. VH_ERR_SCENE for a zero image size. */
 int vh_render_aovs(vh_renderer* r, uint32_t width, uint32_t height, float* albedo_depth,
                    uint32_t* object_id, uint32_t* material_id, float* coverage);
 

@@ -50,6 +50,8 @@ def lib():
         L.or_render.argtypes = [C.c_void_p, u32p, C.c_uint32, C.c_uint32, C.c_uint32, f64p, f64p, f32p, f32p, C.c_int, C.c_int]
         L.or_trace_rays.argtypes = [C.c_void_p, C.c_uint64, f32p, C.c_float, C.c_int, u32p]
         L.or_primary_hits.argtypes = [C.c_void_p, u32p, C.c_uint32, C.c_uint32, C.c_int, u32p, f32p]
+        L.or_render_normals.argtypes = [C.c_void_p, u32p, C.c_int, f32p]
+        L.or_render_aovs.argtypes = [C.c_void_p, u32p, C.c_int, f32p, u32p, u32p, f32p]
         for name in ("or_sin", "or_cos", "or_acos", "or_log2", "or_exp2"):
             getattr(L, name).restype = C.c_float
             getattr(L, name).argtypes = [C.c_float]
@@ -108,6 +110,26 @@ class OracleScene:
         lib().or_primary_hits(self.h, _p(params, C.c_uint32), seed, sample, int(brute), _p(out, C.c_uint32),
                               _p(r, C.c_float) if rays else None)
         return (out, r) if rays else out
+
+    def normals(self, params: np.ndarray, brute: bool = False) -> np.ndarray:
+        """normals.slang for the image size and camera of `params`: (pixels, 3) float32."""
+        params = np.ascontiguousarray(params, np.uint32)
+        n = int(params[0]) * int(params[1])
+        out = np.zeros((n, 3), np.float32)
+        lib().or_render_normals(self.h, _p(params, C.c_uint32), int(brute), _p(out, C.c_float))
+        return out
+
+    def aovs(self, params: np.ndarray, brute: bool = False) -> dict:
+        """aov.slang: albedo (pixels, 3), depth, object_id, material_id, coverage (pixels,)."""
+        params = np.ascontiguousarray(params, np.uint32)
+        n = int(params[0]) * int(params[1])
+        ad = np.zeros((n, 4), np.float32)
+        oid = np.zeros(n, np.uint32)
+        mid = np.zeros(n, np.uint32)
+        cov = np.zeros(n, np.float32)
+        lib().or_render_aovs(self.h, _p(params, C.c_uint32), int(brute), _p(ad, C.c_float), _p(oid, C.c_uint32),
+                             _p(mid, C.c_uint32), _p(cov, C.c_float))
+        return {"albedo": ad[:, :3].copy(), "depth": ad[:, 3].copy(), "object_id": oid, "material_id": mid, "coverage": cov}
 
     def render(self, params: np.ndarray, samples: int, first_sample: int = 0, seed: int = 0, threads: int = 0,
                brute: bool = False, state=None):

@@ -6,8 +6,9 @@
  *
  * It restates, function by function, shaders/include/pt_core.slang (bindings :12-49, pcg/Rng
  * :69-84, BSDF :115-557, occluded :580-613, generatePathFor :645-682, traceClosest :684-700,
- * ReSTIR DI :702-995, shadePath :992-1543, pixelActive :1551-1559, finishPath :1620-1667) and the
- * megakernel loop of shaders/pathtrace.slang:9-25, with these stated deviations (shared with the
+ * ReSTIR DI :702-995, shadePath :992-1543, pixelActive :1551-1559, finishPath :1620-1667), the
+ * megakernel loop of shaders/pathtrace.slang:9-25 and the primary-hit AOVs of shaders/aov.slang:84-188
+ * and shaders/normals.slang:39-110, with these stated deviations (shared with the
  * GPU library):
  *   - film and adaptive statistics accumulate in FP64 (the reference: FP32);
  *   - the driver's ray queries become a watertight ray/triangle test (Woop, Benthin and Wald,
@@ -1680,6 +1681,156 @@ void or_primary_hits(const or_scene* s, const uint32_t* params, uint32_t seed, u
             r[0] = st.org.x; r[1] = st.org.y; r[2] = st.org.z;
             r[3] = st.dir.x; r[4] = st.dir.y; r[5] = st.dir.z;
         }
+    }
+}
+
+/* ---------------------------------------------------------------- primary-hit AOVs */
+
+/* pixelDir (aov.slang:73-81; the same expressions inline in normals.slang:47-57): the camera-space
+ * unit direction through pixel position (pxx, pxy) and, in `dir`, its world-space direction. */
+static f3 pixel_dir(const ctx_t* c, float pxx, float pxy, f3* dir)
+{
+    uint32_t w = c->P[0], h = c->P[1];
+    float t = PF(2);
+    float aspect = (float)w / (float)h;
+    float u = (2.0f * pxx / (float)w - 1.0f + PF(39)) * aspect * t;
+    float v = (1.0f - 2.0f * pxy / (float)h + PF(40)) * t;
+    f3 d = normalize3(v3(u, v, -1.0f));
+    float r3[3];
+    for (uint32_t r = 0; r < 3; r++)
+        r3[r] = PF(8 + 4 * r) * d.x + PF(9 + 4 * r) * d.y + PF(10 + 4 * r) * d.z;
+    *dir = v3(r3[0], r3[1], r3[2]);
+    return d;
+}
+
+static f3 camera_origin(const ctx_t* c) { return v3(PF(11), PF(15), PF(19)); }
+
+/* World-space triangle of a hit and the barycentric weights of its vertices 1 and 2 at `hit`
+ * (normals.slang:77-95, aov.slang:117-132). */
+static void hit_frame(const ctx_t* c, uint32_t instance, uint32_t prim, f3 hit, f3 p[3], uint32_t vi[3], float* b1, float* b2)
+{
+    const or_tables* T = c->t;
+    uint32_t inst = instance * 16;
+    uint32_t tri = T->instances[inst + 1] + 3 * prim;
+    uint32_t vo = T->instances[inst];
+    for (int k = 0; k < 3; k++) {
+        vi[k] = T->indices[tri + k] + vo;
+        p[k] = transform_point(T->instances, inst + 4, vertex_pos(c, vi[k]));
+    }
+    f3 e1 = vsub(p[1], p[0]), e2 = vsub(p[2], p[0]), e3 = vsub(hit, p[0]);
+    float d11 = dot3(e1, e1), d12 = dot3(e1, e2), d22 = dot3(e2, e2);
+    float den = fmax_(d11 * d22 - d12 * d12, 1.0e-20f);
+    *b1 = (d22 * dot3(e3, e1) - d12 * dot3(e3, e2)) / den;
+    *b2 = (d11 * dot3(e3, e2) - d12 * dot3(e3, e1)) / den;
+}
+
+/* normals (normals.slang:39-110): the pixel-centre ray's interpolated vertex normal (geometric
+ * normal without vertex normals), on the geometric normal's side, facing the viewer; 0 on a miss.
+ * 3 floats per pixel. `params` is the 43-word block (words 0-1 the image size, the camera). */
+void or_render_normals(const or_scene* s, const uint32_t* params, int brute, float* out)
+{
+    ctx_t cc = {s, &s->t, params, NULL, NULL, NULL, NULL, 0u, brute};
+    const ctx_t* c = &cc;
+    uint32_t w = params[0], h = params[1];
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t pi = 0; pi < (int64_t)w * h; pi++) {
+        uint32_t x = (uint32_t)(pi % w), y = (uint32_t)(pi / w);
+        f3 dir;
+        pixel_dir(c, (float)x + 0.5f, (float)y + 0.5f, &dir);
+        f3 org = camera_origin(c);
+        dir = normalize3(dir);
+        or_hit q = trace(c, org, dir, 1.0e30f, 0);
+        f3 n = v3(0.0f, 0.0f, 0.0f);
+        if (q.hit) {
+            f3 p[3];
+            uint32_t vi[3];
+            float b1, b2;
+            hit_frame(c, q.inst, q.prim, vadd(org, vscale(dir, q.t)), p, vi, &b1, &b2);
+            f3 ng = normalize3(cross3(vsub(p[1], p[0]), vsub(p[2], p[0])));
+            n = ng;
+            f3 nsObj = vadd(vadd(sscale(1.0f - b1 - b2, vertex_normal(c, vi[0])), sscale(b1, vertex_normal(c, vi[1]))),
+                            sscale(b2, vertex_normal(c, vi[2])));
+            if (dot3(nsObj, nsObj) > 1.0e-12f) {
+                n = normalize3(transform_normal(c, q.inst * 16 + 4, nsObj));
+                if (dot3(n, ng) < 0.0f)
+                    n = vneg(n);
+            }
+            if (dot3(ng, dir) > 0.0f)
+                n = vneg(n);
+        }
+        out[3 * pi] = n.x;
+        out[3 * pi + 1] = n.y;
+        out[3 * pi + 2] = n.z;
+    }
+}
+
+/* aov (aov.slang:84-188): per pixel, from the pixel-centre ray, albedo (base colour times the
+ * level-0 base colour texture) and depth along the camera's forward axis (4 floats), instance + 1
+ * and material + 1 (0 on a miss); and the coverage of 16 stratified rays through up to 8
+ * translucent layers, summed in ray order and divided by 16. */
+void or_render_aovs(const or_scene* s, const uint32_t* params, int brute, float* albedo_depth, uint32_t* object_id,
+                    uint32_t* material_id, float* coverage)
+{
+    ctx_t cc = {s, &s->t, params, NULL, NULL, NULL, NULL, 0u, brute};
+    const ctx_t* c = &cc;
+    const or_tables* T = &s->t;
+    uint32_t w = params[0], h = params[1];
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t pi = 0; pi < (int64_t)w * h; pi++) {
+        uint32_t x = (uint32_t)(pi % w), y = (uint32_t)(pi / w);
+        f3 dir;
+        f3 d = pixel_dir(c, (float)x + 0.5f, (float)y + 0.5f, &dir);
+        f3 org = camera_origin(c);
+        dir = normalize3(dir);
+        or_hit q = trace(c, org, dir, 1.0e30f, 0);
+        f3 albedo = v3(0.0f, 0.0f, 0.0f);
+        float depth = 0.0f;
+        uint32_t id = 0, matId = 0;
+        if (q.hit) {
+            uint32_t inst = q.inst * 16;
+            id = q.inst + 1;
+            matId = T->instances[inst + 2] + 1;
+            depth = q.t * -d.z;
+            const float* M = T->materials + (size_t)T->instances[inst + 2] * 24;
+            albedo = v3(M[0], M[1], M[2]);
+            uint32_t texBase = f2u(M[10]);
+            if (texBase > 0) {
+                f3 p[3];
+                uint32_t vi[3];
+                float b1, b2;
+                hit_frame(c, q.inst, q.prim, vadd(org, vscale(dir, q.t)), p, vi, &b1, &b2);
+                float u0x = T->uvs[2 * vi[0]], u0y = T->uvs[2 * vi[0] + 1];
+                float u1x = T->uvs[2 * vi[1]], u1y = T->uvs[2 * vi[1] + 1];
+                float u2x = T->uvs[2 * vi[2]], u2y = T->uvs[2 * vi[2] + 1];
+                float ux = u0x + b1 * (u1x - u0x) + b2 * (u2x - u0x);
+                float uy = u0y + b1 * (u1y - u0y) + b2 * (u2y - u0y);
+                albedo = vmul(albedo, sample_texture(c, texBase - 1, ux, uy, 0.0f));
+            }
+        }
+        float cover = 0.0f;
+        for (uint32_t k = 0; k < 16; k++) {
+            f3 sdir;
+            pixel_dir(c, (float)x + ((float)(k % 4) + 0.5f) / 4.0f, (float)y + ((float)(k / 4) + 0.5f) / 4.0f, &sdir);
+            f3 so = org, sd = normalize3(sdir);
+            float seen = 0.0f, through = 1.0f;
+            for (uint32_t layer = 0; layer < 8 && through > 0.0f; layer++) {
+                or_hit sq = trace(c, so, sd, 1.0e30f, 0);
+                if (!sq.hit)
+                    break;
+                float op = opacity_at(c, sq.inst, sq.prim, sq.b1, sq.b2);
+                seen += through * op;
+                through *= 1.0f - op;
+                so = vadd(so, vscale(sd, sq.t + 1.0e-4f));
+            }
+            cover += seen;
+        }
+        coverage[pi] = cover / 16.0f;
+        albedo_depth[4 * pi] = albedo.x;
+        albedo_depth[4 * pi + 1] = albedo.y;
+        albedo_depth[4 * pi + 2] = albedo.z;
+        albedo_depth[4 * pi + 3] = depth;
+        object_id[pi] = id;
+        material_id[pi] = matId;
     }
 }
 

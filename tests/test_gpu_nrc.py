@@ -124,3 +124,25 @@ def test_trained_cache_beats_plain_tracing_at_equal_frames(ctx):
     assert st["samples"] > 8192
     assert np.all(np.isfinite(acc.mean()))
     assert e_nrc < e_plain, (e_nrc, e_plain, st)
+
+
+def test_inference_uses_the_bias_corrected_moving_average(ctx):
+    # after three Adam steps from theta0 the integrator's weights are
+    # (1-d)(d^2 theta1 + d theta2 + theta3) / (1 - d^3), d = 0.95
+    r = Renderer(ctx, scenes.cornell())
+    c = Nrc(r, train_paths=1024, ema_decay=0.95, learning_rate=0.05)
+    c.set_params(nrc.init(12))
+    raw, target = batch(2048, 13)
+    thetas = []
+    for _ in range(3):
+        c.train(raw, target, steps=1)
+        thetas.append(c.params().astype(np.float64))
+    d = 0.95
+    avg = (1 - d) * (d * d * thetas[0] + d * thetas[1] + thetas[2]) / (1 - d ** 3)
+    got = c.infer(raw)
+    want = nrc.forward(avg.astype(np.float32), raw, bf16_round=True)[0][:, :3]
+    last = nrc.forward(thetas[2].astype(np.float32), raw, bf16_round=True)[0][:, :3]
+    err = np.abs(got - want).max()
+    assert err < 0.02 * np.abs(want).max() + 1e-3, err
+    # and it is the average, not the last parameters
+    assert err < 0.5 * np.abs(got - last).max(), (err, np.abs(got - last).max())

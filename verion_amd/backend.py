@@ -312,6 +312,23 @@ class Renderer(_Owned):
         self.accumulate(acc, spp)
         return acc.mean()
 
+    def render_normals(self, width: int, height: int) -> np.ndarray:
+        """RayTracingBackend::render_normals (lib.rs:55): (pixels, 3) float32, row-major from the top-left."""
+        out = np.zeros((width * height, 3), np.float32)
+        _check(library().vh_render_normals(self.h, width, height, _p(out, C.c_float)))
+        return out
+
+    def render_aovs(self, width: int, height: int) -> dict:
+        """RayTracingBackend::render_aovs (lib.rs:57): the fields of `Aovs` (lib.rs:165-176)."""
+        n = width * height
+        ad = np.zeros((n, 4), np.float32)
+        oid = np.zeros(n, np.uint32)
+        mid = np.zeros(n, np.uint32)
+        cov = np.zeros(n, np.float32)
+        _check(library().vh_render_aovs(self.h, width, height, _p(ad, C.c_float), _p(oid, C.c_uint32), _p(mid, C.c_uint32),
+                                        _p(cov, C.c_float)))
+        return {"albedo": ad[:, :3].copy(), "depth": ad[:, 3].copy(), "object_id": oid, "material_id": mid, "coverage": cov}
+
     def primary_hits(self, width: int, height: int, sample: int = 0) -> np.ndarray:
         out = np.zeros((width * height, 4), np.uint32)
         _check(library().vh_primary_hits(self.h, width, height, sample, _p(out, C.c_uint32)))

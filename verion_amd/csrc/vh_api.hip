@@ -733,28 +733,51 @@ int vh_bvh_stats(const vh_renderer* r, uint64_t out[4])
     return VH_OK;
 }
 
+// RayTracingBackend::render_normals (lib.rs:55; vulkan renderer.rs:855-877)
 int vh_render_normals(vh_renderer* r, uint32_t width, uint32_t height, float* normals)
 {
-    (void)width;
-    (void)height;
-    (void)normals;
-    if (!r)
-        return fail(VH_ERR_INVALID_ARGUMENT, "null renderer");
-    return fail(VH_ERR_UNSUPPORTED, "render_normals is not built yet");
+    if (!r || !normals)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    if (width == 0 || height == 0)
+        return fail(VH_ERR_SCENE, "image size must be non-zero");
+    uint64_t n = (uint64_t)width * height;
+    if (n > (1ull << 31))
+        return fail(VH_ERR_INVALID_ARGUMENT, "image too large");
+    DevBuf out;
+    TRY(check_fits(12 * n));
+    HIPTRY(out.alloc(12 * n));
+    Params P = base_params(r, width, height, false, false, vh_adaptive{}, false, vh_restir{});
+    TRY(launch_normals(r->scene(), P, out.as<float>(), r->ctx->stream));
+    HIPTRY(hipStreamSynchronize(r->ctx->stream));
+    HIPTRY(hipMemcpy(normals, out.p, 12 * n, hipMemcpyDeviceToHost));
+    return VH_OK;
 }
 
+// RayTracingBackend::render_aovs (lib.rs:57; vulkan renderer.rs:879-924)
 int vh_render_aovs(vh_renderer* r, uint32_t width, uint32_t height, float* albedo_depth, uint32_t* object_id, uint32_t* material_id,
                    float* coverage)
 {
-    (void)width;
-    (void)height;
-    (void)albedo_depth;
-    (void)object_id;
-    (void)material_id;
-    (void)coverage;
-    if (!r)
-        return fail(VH_ERR_INVALID_ARGUMENT, "null renderer");
-    return fail(VH_ERR_UNSUPPORTED, "render_aovs is not built yet");
+    if (!r || !albedo_depth || !object_id || !material_id || !coverage)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    if (width == 0 || height == 0)
+        return fail(VH_ERR_SCENE, "image size must be non-zero");
+    uint64_t n = (uint64_t)width * height;
+    if (n > (1ull << 27))
+        return fail(VH_ERR_INVALID_ARGUMENT, "image too large");
+    DevBuf ad, oid, mid, cov;
+    TRY(check_fits(28 * n));
+    HIPTRY(ad.alloc(16 * n));
+    HIPTRY(oid.alloc(4 * n));
+    HIPTRY(mid.alloc(4 * n));
+    HIPTRY(cov.alloc(4 * n));
+    Params P = base_params(r, width, height, false, false, vh_adaptive{}, false, vh_restir{});
+    TRY(launch_aovs(r->scene(), P, ad.as<float4>(), oid.as<uint32_t>(), mid.as<uint32_t>(), cov.as<float>(), r->ctx->stream));
+    HIPTRY(hipStreamSynchronize(r->ctx->stream));
+    HIPTRY(hipMemcpy(albedo_depth, ad.p, 16 * n, hipMemcpyDeviceToHost));
+    HIPTRY(hipMemcpy(object_id, oid.p, 4 * n, hipMemcpyDeviceToHost));
+    HIPTRY(hipMemcpy(material_id, mid.p, 4 * n, hipMemcpyDeviceToHost));
+    HIPTRY(hipMemcpy(coverage, cov.p, 4 * n, hipMemcpyDeviceToHost));
+    return VH_OK;
 }
 
 } // extern "C"

@@ -105,6 +105,13 @@ constexpr uint32_t kMaxDepth = 8;           // pack.rs:13
 constexpr uint32_t kSamplesPerDispatch = 8; // vulkan renderer.rs:29
 constexpr uint32_t kCounterBanks = 64;      // banks of the traversal counters (vh_trace.h)
 
+// blocks of `block` threads for n threads of work, at least 1 and at most `cap`
+inline uint32_t blocks_for(uint64_t n, uint32_t block, uint32_t cap)
+{
+    uint64_t g = (n + block - 1) / block;
+    return (uint32_t)(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
 int launch_megakernel(const Scene& S, const Frame& F, const Params& P, uint32_t samples, bool restir, bool split, hipStream_t s);
 int launch_wf_sample(const Scene& S, const Frame& F, const Params& P, const Wavefront& W, bool restir, bool split, const LaunchCfg& cfg,
                      hipEvent_t* stageEvents);
@@ -117,6 +124,9 @@ int launch_wf_depth(const Scene& S, const Frame& F, const Params& P, const Wavef
 int launch_nrc_query_shade(const Scene& S, const Frame& F, const Params& P, const Wavefront& W, float4* queries, uint32_t d,
                            const LaunchCfg& cfg);
 int launch_trace_rays(const Scene& S, const float* rays, uint64_t n, float tmax, uint4* out, hipStream_t s);
+int launch_normals(const Scene& S, const Params& P, float* out3, hipStream_t s);
+int launch_aovs(const Scene& S, const Params& P, float4* albedoDepth, uint32_t* objectId, uint32_t* materialId, float* coverage,
+                hipStream_t s);
 
 } // namespace vh
 

@@ -420,12 +420,6 @@ __global__ __launch_bounds__(256) void k_rays(Scene S, const float* rays, uint64
     }
 }
 
-uint32_t blocks_for(uint64_t n, uint32_t block, uint32_t cap)
-{
-    uint64_t g = (n + block - 1) / block;
-    return (uint32_t)(g < 1 ? 1 : (g > cap ? cap : g));
-}
-
 } // namespace
 
 int launch_megakernel(const Scene& S, const Frame& F, const Params& P, uint32_t samples, bool restir, bool split, hipStream_t s)
