@@ -262,6 +262,53 @@ int vh_nrc_accumulate(vh_nrc* nrc, vh_accumulation* acc, uint32_t frames);
 /* Last loss, last training-sample count, total training seconds (warm-up included), total rendering seconds. */
 int vh_nrc_stats(const vh_nrc* nrc, double out[4]);
 
+/* --- batched jobs (no counterpart in the reference: its session renders one job at a time,
+ *     crates/verion-render/src/session.rs:323-336) ------------------------------------------------
+ * Every job of a batch is resident in one shared wavefront and every sample pass advances all jobs
+ * still sampling; a job's film equals rendering it alone with vh_accumulate (wavefront) bit for
+ * bit. Plain integrator only (no lighting split, ReSTIR or radiance cache). The renderers of the
+ * jobs must outlive the batch and must not be updated while it exists. */
+typedef struct vh_batch vh_batch;
+typedef struct vh_batch_job {
+    vh_renderer* renderer; /* the scene */
+    uint32_t width, height;
+    uint32_t spp;          /* samples per pixel (the most, with adaptive sampling) */
+    vh_adaptive adaptive;  /* threshold 0 = off */
+    uint32_t seed;         /* random-stream seed, as vh_set_seed (0 = the reference stream) */
+} vh_batch_job;
+/* Launch shape of the batch kernels: blocks of the grid-stride kernels, and the depth-0 pixel
+ * order (tiles of tile_width x tile_height pixels, or 0 x 0 for scanlines). */
+typedef struct vh_batch_tuning {
+    uint32_t grid_blocks;
+    uint32_t tile_width;
+    uint32_t tile_height;
+} vh_batch_tuning;
+
+/* 1 to 255 jobs; allocates every film and the shared wavefront. Tuning starts automatic. */
+int vh_batch_create(vh_context* ctx, const vh_batch_job* jobs, uint32_t count, vh_batch** out);
+void vh_batch_destroy(vh_batch* batch);
+/* Clears films, statistics and the pass counter (the next run starts from sample 0). */
+int vh_batch_reset(vh_batch* batch);
+/* Fixes the tuning; NULL returns to automatic tuning (done during the next run from its start). */
+int vh_batch_set_tuning(vh_batch* batch, const vh_batch_tuning* tuning);
+int vh_batch_get_tuning(const vh_batch* batch, vh_batch_tuning* tuning);
+/* Runs sample passes until every job has its samples (at most max_passes of them; 0 = no limit),
+ * blocking until the GPU is done. graph = 1 replays one captured pass as a hipGraph. With
+ * automatic tuning the first passes try one candidate each (the image does not depend on them)
+ * and the fastest per traced path runs the rest. */
+int vh_batch_run(vh_batch* batch, int graph, uint32_t max_passes);
+/* out[0] = GPU seconds from the start of the first run to the end of the job's last pass with
+ * work (-1 while it still has work), out[1] = its passes with work, out[2] = paths traced (depth-0
+ * paths over all passes), out[3] = GPU seconds of all runs. */
+int vh_batch_job_stats(const vh_batch* batch, uint32_t job, double out[4]);
+/* The candidates the automatic tuning tried and their milliseconds per million paths. */
+int vh_batch_tuning_report(const vh_batch* batch, vh_batch_tuning* tried, double* ms_per_mpath, uint32_t cap, uint32_t* count);
+/* Running sums of a job: film (3 doubles per pixel), statistics (3 doubles, adaptive jobs only). */
+int vh_batch_read_film(const vh_batch* batch, uint32_t job, double* film);
+int vh_batch_read_stats(const vh_batch* batch, uint32_t job, double* stats);
+/* Mean image of a job (readback.rs:18-38): 3 floats per pixel. */
+int vh_batch_mean(const vh_batch* batch, uint32_t job, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,3 +38,30 @@ def test_backend_refuses_to_run_without_the_library(tmp_path, monkeypatch):
     monkeypatch.setattr(b, "_lib", None)
     with pytest.raises(ImportError):
         b.Context()
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    # sizes and field offsets of every struct of the header, from gcc, against the ctypes mirror
+    import subprocess
+
+    from verion_amd import abi
+
+    pairs = {"vh_scene_info": abi.SceneInfo, "vh_view": abi.ViewS, "vh_scene_tables": abi.SceneTables,
+             "vh_adaptive": abi.Adaptive, "vh_restir": abi.Restir, "vh_memory_usage": abi.MemoryUsage,
+             "vh_update_report": abi.UpdateReport, "vh_nrc_config": abi.NrcConfig, "vh_batch_job": abi.BatchJob,
+             "vh_batch_tuning": abi.BatchTuning}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, py in pairs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'printf("{cname}.{field} %zu\\n", offsetof({cname}, {field}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    got = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines())
+    for cname, py in pairs.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert int(got[f"{cname}.{field}"]) == getattr(py, field).offset, (cname, field)

@@ -46,6 +46,15 @@ class NrcConfig(C.Structure):
                 ("terminate_depth", C.c_uint32), ("ema_decay", C.c_float), ("warmup_frames", C.c_uint32)]
 
 
+class BatchJob(C.Structure):
+    _fields_ = [("renderer", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("adaptive", Adaptive), ("seed", C.c_uint32)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("grid_blocks", C.c_uint32), ("tile_width", C.c_uint32), ("tile_height", C.c_uint32)]
+
+
 class MemoryUsage(C.Structure):
     _fields_ = [("budget", C.c_uint64), ("allocated", C.c_uint64), ("uploaded", C.c_uint64)]
 

@@ -120,6 +120,17 @@ def library() -> C.CDLL:
         "vh_nrc_train": (i32, [vp, u64, f32p, f32p, u32, f32p]),
         "vh_nrc_accumulate": (i32, [vp, vp, u32]),
         "vh_nrc_stats": (i32, [vp, f64p]),
+        "vh_batch_create": (i32, [vp, C.POINTER(abi.BatchJob), u32, C.POINTER(vp)]),
+        "vh_batch_destroy": (None, [vp]),
+        "vh_batch_reset": (i32, [vp]),
+        "vh_batch_set_tuning": (i32, [vp, C.POINTER(abi.BatchTuning)]),
+        "vh_batch_get_tuning": (i32, [vp, C.POINTER(abi.BatchTuning)]),
+        "vh_batch_run": (i32, [vp, i32, u32]),
+        "vh_batch_job_stats": (i32, [vp, u32, f64p]),
+        "vh_batch_tuning_report": (i32, [vp, C.POINTER(abi.BatchTuning), f64p, u32, C.POINTER(C.c_uint32)]),
+        "vh_batch_read_film": (i32, [vp, u32, f64p]),
+        "vh_batch_read_stats": (i32, [vp, u32, f64p]),
+        "vh_batch_mean": (i32, [vp, u32, f32p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -154,7 +165,7 @@ class _Handle:
     def free(self):
         if self.h is None:
             return
-        for k in list(self.kids):
+        for k in reversed(list(self.kids)):  # newest first: a batch goes before the renderers it uses
             k.free()
         self.destroy(self.h)
         self.h = None
@@ -408,3 +419,73 @@ class Nrc(_Owned):
         out = np.zeros(4, np.float64)
         _check(library().vh_nrc_stats(self.h, _p(out, C.c_double)))
         return {"loss": float(out[0]), "samples": int(out[1]), "train_seconds": float(out[2]), "render_seconds": float(out[3])}
+
+
+class Batch(_Owned):
+    """Many render jobs in one shared wavefront (vh_batch_*): `jobs` is a list of dicts with
+    renderer, width, height, spp and optionally adaptive (Adaptive or None) and seed."""
+
+    def __init__(self, ctx: Context, jobs):
+        self.ctx = ctx
+        self.jobs = [dict(j) for j in jobs]
+        self.renderers = [j["renderer"] for j in self.jobs]  # they must outlive the batch
+        arr = (abi.BatchJob * len(self.jobs))()
+        for k, j in enumerate(self.jobs):
+            a = j.get("adaptive")
+            arr[k].renderer = j["renderer"].h
+            arr[k].width, arr[k].height, arr[k].spp = j["width"], j["height"], j["spp"]
+            arr[k].adaptive = abi.Adaptive(a.threshold, a.min_spp) if a else abi.Adaptive(0.0, 0)
+            arr[k].seed = j.get("seed", 0)
+        h = C.c_void_p()
+        _check(library().vh_batch_create(ctx.h, arr, len(self.jobs), C.byref(h)))
+        self._handle = _Handle(h, library().vh_batch_destroy, ctx._handle)
+
+    def reset(self):
+        _check(library().vh_batch_reset(self.h))
+
+    def set_tuning(self, tuning):
+        """(grid_blocks, tile_width, tile_height), or None for automatic tuning."""
+        if tuning is None:
+            _check(library().vh_batch_set_tuning(self.h, None))
+        else:
+            t = abi.BatchTuning(*tuning)
+            _check(library().vh_batch_set_tuning(self.h, C.byref(t)))
+
+    def tuning(self):
+        t = abi.BatchTuning()
+        _check(library().vh_batch_get_tuning(self.h, C.byref(t)))
+        return (t.grid_blocks, t.tile_width, t.tile_height)
+
+    def run(self, graph: bool = True, max_passes: int = 0):
+        _check(library().vh_batch_run(self.h, int(graph), max_passes))
+
+    def job_stats(self, job: int) -> dict:
+        out = np.zeros(4, np.float64)
+        _check(library().vh_batch_job_stats(self.h, job, _p(out, C.c_double)))
+        return {"latency_s": float(out[0]), "passes": int(out[1]), "paths": int(out[2]), "batch_s": float(out[3])}
+
+    def tuning_report(self):
+        cap = 64
+        tried = (abi.BatchTuning * cap)()
+        ms = np.zeros(cap, np.float64)
+        n = C.c_uint32()
+        _check(library().vh_batch_tuning_report(self.h, tried, _p(ms, C.c_double), cap, C.byref(n)))
+        return [((tried[i].grid_blocks, tried[i].tile_width, tried[i].tile_height), float(ms[i])) for i in range(min(n.value, cap))]
+
+    def film(self, job: int) -> np.ndarray:
+        j = self.jobs[job]
+        out = np.zeros(3 * j["width"] * j["height"], np.float64)
+        _check(library().vh_batch_read_film(self.h, job, _p(out, C.c_double)))
+        return out
+
+    def stats(self, job: int) -> np.ndarray:
+        j = self.jobs[job]
+        out = np.zeros(3 * j["width"] * j["height"], np.float64)
+        _check(library().vh_batch_read_stats(self.h, job, _p(out, C.c_double)))
+        return out
+
+    def mean(self, job: int) -> np.ndarray:
+        j = self.jobs[job]
+        out = np.zeros((j["width"] * j["height"], 3), np.float32)
+        _check(library().vh_batch_mean(self.h, job, _p(out, C.c_float)))
+        return out
