@@ -11,7 +11,9 @@ Prints one JSON line with the throughput (`value`, plain wavefront), a roofline 
 dominant kernel (achieved algorithmic bytes per launch over its HIP-event launch time, against
 8 TB/s), the CPU oracle's throughput on a bounded sample of the same workload, and `time_to_rmse`:
 GPU seconds to relMSE 1e-3 of the neural radiance cache and of the plain tracer on the same scene
-at 1080p, 1 spp per frame, against a plain reference render (4096 spp by default).
+at 1080p, 1 spp per frame, against a plain reference render (4096 spp by default); and `batch`:
+p50/p99 job latency and aggregate Msamples/s of 32 uneven jobs in one shared wavefront against
+one job at a time.
 
     python bench.py [--gpus 1] [--steps 64] [--warmup 4]
 """
@@ -101,6 +103,7 @@ def main():
     ap.add_argument("--integrator", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-nrc", action="store_true", help="skip the NRC vs plain time-to-relMSE measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 32-job batch measurement")
     ap.add_argument("--nrc-ref-spp", type=int, default=4096,
                     help="spp of the plain reference render the relMSE ladders are scored against")
     ap.add_argument("--no-counting", action="store_true",
@@ -183,7 +186,21 @@ def main():
         line["parity_full_size"] = parity
     if not args.no_nrc:
         line["time_to_rmse"] = time_to_rmse(W, H, args.nrc_ref_spp)
+    if not args.no_batch:
+        line["batch"] = batch_jobs()
     print(json.dumps(line), flush=True)
+
+
+def batch_jobs():
+    """32 seeded jobs (256x256 to 1080p, 4-256 spp, adaptive 0.01, scenes of 10k-1.3M triangles):
+    one shared wavefront (autotuned, graph-replayed) against one job at a time in 8-spp chunks."""
+    from verion_amd import batch_eval
+
+    res = batch_eval.evaluate(log=log)
+    keep = ("seconds", "msamples_per_s", "p50_latency_s", "p99_latency_s")
+    return {"jobs": res["jobs"], "pixels_total": res["pixels_total"], "spp_range": res["spp_range"],
+            "adaptive": res["adaptive"], "batched": {k: res["batched"][k] for k in keep + ("tuning",)},
+            "one_at_a_time": {k: res["one_at_a_time"][k] for k in keep}, "films_checked_equal": res["films_checked_equal"]}
 
 
 def time_to_rmse(width, height, ref_spp):
