@@ -52,21 +52,29 @@ def traffic_from_profiles(kernel: str):
 def algorithmic_bytes(c: dict, pixels: int, delta_lights: int, env: bool):
     """Algorithmic bytes of one sample pass, split by kernel, from the traversal counters.
 
-    extend: per live path the ray (24 B read, none at depth 0 where the ray is regenerated) and the
-            hit record (16 B write), plus every LBVH node (128 B) and leaf triangle (48 B) it visits;
-    shade:  per live path the hit (16 B), the state (68 B read beyond depth 0, 68 B write for each
-            survivor), the shading tables of a hit (instance 64 + indices 12 + positions 36 +
-            vertex normals 36 + material 96 = 244 B), 64 B per delta light, 48 B of environment
-            sampling tables, the shadow rays' nodes and triangles, and the FP64 film update
-            (48 B) once per finished path.
+    wf_extend: per live path the ray (24 B read, none at depth 0 where the ray is regenerated) and
+               the hit record (16 B write), plus every LBVH node (128 B) and leaf triangle (48 B)
+               it visits;
+    wf_shade:  per live path the hit (16 B), the state (68 B read beyond depth 0, 68 B write for
+               each survivor), the shading tables of a hit (instance 64 + indices 12 + positions 36
+               + vertex normals 36 + material 96 = 244 B), 64 B per delta light, 48 B of
+               environment sampling tables, one shadow record per shadow ray (40 B + 4 B queue
+               entry), per pending path its count, next slot, pixel and total (24 B), and the FP64
+               film update (48 B) once per finished path;
+    wf_shadow: per shadow ray its queue entry (4 B), origin, direction and tmax (28 B), the
+               visibility word (4 B), and the nodes and triangles it visits; the resolve step
+               (per shadow ray the visibility and contribution, 16 B, plus the pending record) is
+               counted with it.
     """
     rays = c["closest_rays"]
     hits = c["closest_hits"]
+    shadow = c["shadow_rays"]
     later = max(rays - pixels, 0)  # paths that reached depth >= 1
     extend = 24 * later + 16 * rays + NODE_BYTES * c["closest_nodes"] + TRI_BYTES * c["closest_tris"]
     shade = (16 * rays + 4 * STATE_WORDS * later + 4 * STATE_WORDS * later + hits * (244 + 64 * delta_lights + (48 if env else 0))
-             + NODE_BYTES * c["shadow_nodes"] + TRI_BYTES * c["shadow_tris"] + 48 * pixels)
-    return extend, shade
+             + 44 * shadow + 24 * hits + 48 * pixels)
+    shadow_b = (4 + 28 + 4 + 16) * shadow + 24 * hits + NODE_BYTES * c["shadow_nodes"] + TRI_BYTES * c["shadow_tris"]
+    return {"wf_extend": extend, "wf_shade": shade, "wf_shadow": shadow_b}
 
 
 def cpu_baseline(packed, width, height, gpu_film):
@@ -104,6 +112,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-nrc", action="store_true", help="skip the NRC vs plain time-to-relMSE measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the 32-job batch measurement")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (8 blocks per CU, 8x8 tiles)")
+    ap.add_argument("--shape", type=str, default=None, help="fixed launch shape BLOCKS,TILE_W,TILE_H instead of tuning")
     ap.add_argument("--nrc-ref-spp", type=int, default=4096,
                     help="spp of the plain reference render the relMSE ladders are scored against")
     ap.add_argument("--no-counting", action="store_true",
@@ -130,6 +140,20 @@ def main():
     for _ in range(args.warmup):
         r.accumulate(warm, 1)
     warm.close()
+    tuning = None
+    if args.shape:
+        blocks, tw, th = (int(x) for x in args.shape.split(","))
+        r.set_wavefront_blocks(blocks)
+        r.set_wavefront_tiles(tw, th)
+        tuning = {"grid_blocks": blocks, "tile": [tw, th], "fixed": True}
+    elif args.integrator == "wavefront" and not args.no_tune:
+        # launch shape (grid blocks, depth-0 tiles) by the project's autotuner, before timing
+        from verion_amd.autotune import tune_wavefront
+
+        best, table = tune_wavefront(r, W, H)
+        tuning = {"grid_blocks": best[0], "tile": [best[1], best[2]],
+                  "tried_ms_per_pass": [[list(c), round(ms, 3)] for c, ms in table]}
+        log(f"autotuned wavefront: {tuning}")
 
     # timed region: K sample passes from sample 0; accumulate() returns after the GPU finished, so
     # the clock brackets device-complete work on both sides. Stage timing records HIP events around
@@ -141,8 +165,9 @@ def main():
         r.accumulate(acc, 1)
     elapsed = time.perf_counter() - t0
     stage = acc.stage_seconds()
+    kernel_s = acc.kernel_seconds()
     value = args.steps * pixels / elapsed / 1e6
-    log(f"timed {args.steps} steps in {elapsed:.3f} s; stage seconds {stage}")
+    log(f"timed {args.steps} steps in {elapsed:.3f} s; extend, shade, shadow, resolve seconds {kernel_s}")
 
     # traversal counters of one more pass (outside the timed region) for the algorithmic bytes
     if args.no_counting:
@@ -156,18 +181,19 @@ def main():
         r.set_counters(False)
     log(f"counters per pass: {counts}")
     delta_lights = packed.info.light_count
-    extend_b, shade_b = algorithmic_bytes(counts, pixels, delta_lights, packed.info.env_width > 0)
+    algo = algorithmic_bytes(counts, pixels, delta_lights, packed.info.env_width > 0)
     launches = args.steps * 8
-    ext_s, sh_s = stage if stage else (float("nan"), float("nan"))
-    dom = "wf_shade" if sh_s >= ext_s else "wf_extend"
-    dom_s = max(ext_s, sh_s) / launches
-    dom_b = (shade_b if dom == "wf_shade" else extend_b) / 8
+    names = ("wf_extend", "wf_shade", "wf_shadow", "wf_resolve")
+    per = dict(zip(names, kernel_s)) if kernel_s else {k: float("nan") for k in names}
+    dom = max(("wf_extend", "wf_shade", "wf_shadow"), key=lambda k: per[k])
+    dom_s = per[dom] / launches
+    dom_b = algo[dom] / 8
     achieved = dom_b / dom_s / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom),
                 "launch_ms": round(dom_s * 1e3, 4), "algorithmic_bytes_per_launch": int(dom_b),
-                "extend_ms_per_launch": round(ext_s / launches * 1e3, 4),
-                "shade_ms_per_launch": round(sh_s / launches * 1e3, 4)}
+                "ms_per_launch": {k: round(per[k] / launches * 1e3, 4) for k in names},
+                "algorithmic_gb_per_s": {k: round(algo[k] / 8 / (per[k] / launches) / 1e9, 2) for k in algo}}
 
     line = {"metric": "Msamples/s at 1080p + time-to-rMSE 1e-3: NRC vs plain wavefront, 1 MI355X", "value": round(value, 3),
             "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -175,7 +201,7 @@ def main():
             "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "procedural 1,024 icospheres (1,310,720 tris), sun + constant env map",
                        "width": W, "height": H, "spp_per_step": 1, "max_depth": 8, "integrator": args.integrator,
-                       "film": "fp64"},
+                       "film": "fp64", "launch_shape": tuning},
             "roofline": roofline}
     if not args.no_cpu_baseline:
         gpu1 = r.new_accumulation(W, H)

@@ -163,6 +163,10 @@ int vh_set_seed(vh_renderer* r, uint32_t seed);
  * No counterpart in the reference (its dispatch size is the queue length, wf_args.slang:19); set
  * by the autotuner. Images do not depend on it. */
 int vh_set_wavefront_blocks(vh_renderer* r, uint32_t blocks);
+/* Order in which the wavefront takes pixels at depth 0: tiles of tile_width x tile_height pixels
+ * (8 x 8 by default), or 0 x 0 for scanlines. No counterpart in the reference (it dispatches
+ * pixels in 64-wide groups of a row, wavefront.rs:11). Images do not depend on it. */
+int vh_set_wavefront_tiles(vh_renderer* r, uint32_t tile_width, uint32_t tile_height);
 
 /* --- accumulation (new_accumulation / accumulate, lib.rs:47-52; trait Accumulation :74-93) ---- */
 
@@ -177,6 +181,10 @@ uint32_t vh_acc_samples(const vh_accumulation* acc);
 double vh_acc_gpu_seconds(const vh_accumulation* acc);
 /* Extend and shade GPU seconds of the wavefront when stage timing was on; VH_ERR_UNSUPPORTED otherwise. */
 int vh_acc_stage_seconds(const vh_accumulation* acc, double out[2]);
+/* Debug export (no counterpart in the reference): the same GPU seconds split by kernel: extend,
+ * shade, the deferred shadow rays and their resolve (the last two are 0 when shade traces its
+ * shadow rays itself: ReSTIR, the lighting split, more than 7 delta lights). */
+int vh_acc_kernel_seconds(const vh_accumulation* acc, double out[4]);
 uint64_t vh_acc_wavefront_bytes(const vh_accumulation* acc);
 uint64_t vh_acc_restir_bytes(const vh_accumulation* acc);
 /* Pixels the next sample would trace (readback::active_pixels, readback.rs:51-61). */

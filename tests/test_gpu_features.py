@@ -232,6 +232,33 @@ def test_wavefront_memory_is_reported(ctx):
     assert acc.wavefront_bytes() == 0
     r.accumulate(acc, 1)
     n = 64 * 32
-    assert acc.wavefront_bytes() == 2 * 4 * 17 * n + 16 * n + 4 * n + 64
+    # state ping-pong, hits, depth-0 queue, counters, and the deferred shadow rays: per slot K
+    # records of 10 words + visibility + queue entry (K = delta lights + 1), and 6 pending words
+    k = r.packed.info.light_count + 1
+    assert acc.wavefront_bytes() == 2 * 4 * 17 * n + 16 * n + 4 * n + 128 + 4 * 12 * k * n + 4 * 6 * n
+    lit = r.new_accumulation(64, 32, lighting=True)  # the lighting split shades fused
+    r.accumulate(lit, 1)
+    assert lit.wavefront_bytes() == 2 * 4 * 34 * n + 16 * n + 4 * n + 128
     m = ctx.memory()
     assert m.budget > m.allocated > 0 and m.uploaded > 0
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_depth0_order_and_grid_do_not_change_the_image(ctx, adaptive):
+    from verion_amd.backend import Adaptive
+
+    r = Renderer(ctx, scenes.zoo())
+    r.set_integrator(WAVEFRONT)
+    if adaptive:
+        r.set_adaptive(Adaptive(0.05, 2))
+    films = []
+    for blocks, tw, th in [(2048, 0, 0), (2048, 8, 8), (512, 16, 4), (4096, 5, 3), (64, 64, 64)]:
+        r.set_wavefront_blocks(blocks)
+        r.set_wavefront_tiles(tw, th)
+        acc = r.new_accumulation(37, 29)
+        r.accumulate(acc, 3)
+        films.append((acc.film(), acc.stats() if adaptive else None))
+    for f, s in films[1:]:
+        assert np.array_equal(f, films[0][0])
+        if adaptive:
+            assert np.array_equal(s, films[0][1])

@@ -18,7 +18,9 @@ from . import abi
 from .pack import Packed, View, pack
 from .scene import Scene
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libverion_hip.so")
+# the in-tree build; VERION_HIP_LIBRARY names another build of the same sources (e.g. one with
+# different occupancy targets) to load instead
+LIB_PATH = os.environ.get("VERION_HIP_LIBRARY") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "libverion_hip.so")
 
 OK, NO_DEVICE, UNSUPPORTED, OUT_OF_MEMORY, KERNEL, SCENE, INVALID_ARGUMENT = range(7)
 MEGAKERNEL, WAVEFRONT = 0, 1
@@ -91,12 +93,14 @@ def library() -> C.CDLL:
         "vh_set_restir": (i32, [vp, C.POINTER(abi.Restir)]),
         "vh_set_seed": (i32, [vp, u32]),
         "vh_set_wavefront_blocks": (i32, [vp, u32]),
+        "vh_set_wavefront_tiles": (i32, [vp, u32, u32]),
         "vh_accumulation_create": (i32, [vp, u32, u32, i32, C.POINTER(vp)]),
         "vh_accumulation_destroy": (None, [vp]),
         "vh_accumulate": (i32, [vp, vp, u32]),
         "vh_acc_samples": (u32, [vp]),
         "vh_acc_gpu_seconds": (C.c_double, [vp]),
         "vh_acc_stage_seconds": (i32, [vp, f64p]),
+        "vh_acc_kernel_seconds": (i32, [vp, f64p]),
         "vh_acc_wavefront_bytes": (u64, [vp]),
         "vh_acc_restir_bytes": (u64, [vp]),
         "vh_acc_active_pixels": (u64, [vp]),
@@ -228,6 +232,12 @@ class Accumulation(_Owned):
         rc = library().vh_acc_stage_seconds(self.h, _p(out, C.c_double))
         return None if rc != OK else (float(out[0]), float(out[1]))
 
+    def kernel_seconds(self):
+        """(extend, shade, deferred shadow rays, their resolve) GPU seconds, or None."""
+        out = np.zeros(4, np.float64)
+        rc = library().vh_acc_kernel_seconds(self.h, _p(out, C.c_double))
+        return None if rc != OK else tuple(float(x) for x in out)
+
     def wavefront_bytes(self) -> int:
         return library().vh_acc_wavefront_bytes(self.h)
 
@@ -310,6 +320,10 @@ class Renderer(_Owned):
 
     def set_wavefront_blocks(self, blocks: int):
         _check(library().vh_set_wavefront_blocks(self.h, blocks))
+
+    def set_wavefront_tiles(self, tile_width: int, tile_height: int):
+        """Depth-0 pixel order of the wavefront: tiles, or (0, 0) for scanlines."""
+        _check(library().vh_set_wavefront_tiles(self.h, tile_width, tile_height))
 
     def new_accumulation(self, width: int, height: int, lighting: bool = False) -> Accumulation:
         return Accumulation(self, width, height, lighting)

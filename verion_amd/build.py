@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libverion_hip.so")
 SOURCES = ["vh_api.hip", "vh_bvh.hip", "vh_render.hip", "vh_nrc.hip", "vh_aov.hip", "vh_batch.hip"]
-HEADERS = ["vh_internal.h", "vh_math.h", "vh_shade.h", "vh_trace.h"]
+HEADERS = ["vh_internal.h", "vh_math.h", "vh_shade.h", "vh_trace.h", "vh_wavefront.h"]
 
 FLAGS = [
     "--offload-arch=gfx950",

@@ -244,14 +244,29 @@ int ensure_wavefront(vh_accumulation* a)
         return VH_OK;
     uint64_t n = a->pixels();
     uint32_t words = 17 + (a->lighting ? 17 : 0);
-    uint64_t bytes = 2 * 4 * words * n + 16 * n + 4 * n + 64;
+    // deferred shadow rays for the plain integrator: one record per delta light and one for the
+    // emitter or environment sample; scenes with more lights than kMaxShadow - 1 shade fused
+    uint32_t K = a->r->info.light_count + 1;
+    bool defer = !a->lighting && !a->restirOn && K <= kMaxShadow;
+    uint64_t shBytes = defer ? (4ull * (kShadowWords + 2) * K * n + 4ull * 6 * n) : 0;
+    uint64_t bytes = 2 * 4 * words * n + 16 * n + 4 * n + 128 + shBytes;
     TRY(check_fits(bytes));
     HIPTRY(a->state0.alloc(4 * words * n));
     HIPTRY(a->state1.alloc(4 * words * n));
     HIPTRY(a->hits.alloc(16 * n));
     HIPTRY(a->queue0.alloc(4 * n));
-    HIPTRY(a->counts.alloc(64));
-    HIPTRY(hipMemsetAsync(a->counts.p, 0, 64, a->r->ctx->stream));
+    HIPTRY(a->counts.alloc(128));
+    HIPTRY(hipMemsetAsync(a->counts.p, 0, 128, a->r->ctx->stream));
+    if (defer) {
+        HIPTRY(a->shRec.alloc(4ull * kShadowWords * K * n));
+        HIPTRY(a->shVis.alloc(4ull * K * n));
+        HIPTRY(a->shQueue.alloc(4ull * K * n));
+        HIPTRY(a->shNRec.alloc(4 * n));
+        HIPTRY(a->shOut.alloc(4 * n));
+        HIPTRY(a->shPixel.alloc(4 * n));
+        HIPTRY(a->shTotal.alloc(12 * n));
+        a->shK = K;
+    }
     a->wfBytes = bytes;
     a->wfReady = true;
     return VH_OK;
@@ -420,6 +435,17 @@ int vh_set_wavefront_blocks(vh_renderer* r, uint32_t blocks)
     return VH_OK;
 }
 
+int vh_set_wavefront_tiles(vh_renderer* r, uint32_t tile_width, uint32_t tile_height)
+{
+    if (!r)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null renderer");
+    if ((tile_width == 0) != (tile_height == 0) || tile_width > 4096 || tile_height > 4096)
+        return fail(VH_ERR_INVALID_ARGUMENT, "tiles are w x h with both in [1, 4096], or 0 x 0 for scanlines");
+    r->tileW = tile_width;
+    r->tileH = tile_height;
+    return VH_OK;
+}
+
 int vh_accumulation_create(vh_renderer* r, uint32_t width, uint32_t height, int lighting, vh_accumulation** out)
 {
     if (!r || !out)
@@ -503,27 +529,22 @@ int vh_accumulate(vh_renderer* r, vh_accumulation* a, uint32_t samples)
     std::vector<hipEvent_t> stageEv;
     bool timing = wave && r->stageTiming;
     if (timing) {
-        stageEv.resize(4 * kMaxDepth * samples);
+        stageEv.resize(kStageEvents * kMaxDepth * samples);
         for (auto& e : stageEv)
             HIPTRY(hipEventCreate(&e));
     }
     int rc = VH_OK;
     HIPTRY(hipEventRecord(e0, s));
     if (wave) {
-        Wavefront W;
-        W.state[0] = a->state0.as<uint32_t>();
-        W.state[1] = a->state1.as<uint32_t>();
-        W.hits = a->hits.as<uint4>();
-        W.queue0 = a->queue0.as<uint32_t>();
-        W.counts = a->counts.as<uint32_t>();
-        W.n = (uint32_t)n;
-        W.words = 17 + (split ? 17 : 0);
+        Wavefront W = a->wavefront(split);
+        if (restir)
+            W.shK = 0; // ReSTIR shades fused
         LaunchCfg cfg{s, r->wfBlocks ? r->wfBlocks : std::max(256u, r->ctx->cus * 8)};
         for (uint32_t k = 0; k < samples && rc == VH_OK; k++) {
             Params P = base;
             P.w[6] = a->samples + k;
             P.w[31] = (uint32_t)n;
-            rc = launch_wf_sample(S, F, P, W, restir, split, cfg, timing ? &stageEv[4 * kMaxDepth * k] : nullptr);
+            rc = launch_wf_sample(S, F, P, W, restir, split, cfg, timing ? &stageEv[kStageEvents * kMaxDepth * k] : nullptr);
         }
     } else {
         uint32_t k = 0;
@@ -547,12 +568,12 @@ int vh_accumulate(vh_renderer* r, vh_accumulation* a, uint32_t samples)
         if (timing) {
             for (uint32_t k = 0; k < samples; k++)
                 for (uint32_t d = 0; d < kMaxDepth; d++) {
-                    hipEvent_t* ev = &stageEv[4 * kMaxDepth * k + 4 * d];
-                    float ex = 0.0f, sh = 0.0f;
-                    (void)hipEventElapsedTime(&ex, ev[0], ev[1]);
-                    (void)hipEventElapsedTime(&sh, ev[1], ev[2]);
-                    a->stage[0] += (double)ex * 1.0e-3;
-                    a->stage[1] += (double)sh * 1.0e-3;
+                    hipEvent_t* ev = &stageEv[kStageEvents * kMaxDepth * k + kStageEvents * d];
+                    for (uint32_t st = 0; st < 4; st++) {
+                        float ms = 0.0f;
+                        (void)hipEventElapsedTime(&ms, ev[st], ev[st + 1]);
+                        a->stage[st] += (double)ms * 1.0e-3;
+                    }
                 }
             a->stageValid = true;
         }
@@ -574,7 +595,18 @@ int vh_acc_stage_seconds(const vh_accumulation* a, double out[2])
     if (!a->stageValid)
         return fail(VH_ERR_UNSUPPORTED, "stage timing was off");
     out[0] = a->stage[0];
-    out[1] = a->stage[1];
+    out[1] = a->stage[1] + a->stage[2] + a->stage[3];
+    return VH_OK;
+}
+
+int vh_acc_kernel_seconds(const vh_accumulation* a, double out[4])
+{
+    if (!a || !out)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    if (!a->stageValid)
+        return fail(VH_ERR_UNSUPPORTED, "stage timing was off");
+    for (int k = 0; k < 4; k++)
+        out[k] = a->stage[k];
     return VH_OK;
 }
 

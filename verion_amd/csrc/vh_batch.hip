@@ -63,23 +63,6 @@ struct BatchDev {
     uint32_t tileW, tileH;         // depth-0 pixel order; tileW = 0: scanlines
 };
 
-// Local index i (0 .. w h - 1) of a job's depth-0 order -> pixel. Bands of tileH rows, each cut
-// into tiles of tileW columns (the last band and the last tile of a band may be smaller); tiles
-// in row-major order, pixels row-major inside a tile.
-__device__ __forceinline__ uint32_t tilePixel(uint32_t i, uint32_t w, uint32_t h, uint32_t tileW, uint32_t tileH)
-{
-    if (tileW == 0)
-        return i;
-    uint32_t band = i / (tileH * w);
-    uint32_t bh = min(tileH, h - band * tileH);
-    uint32_t j = i - band * tileH * w;
-    uint32_t t = j / (tileW * bh);
-    uint32_t k = j - t * tileW * bh;
-    uint32_t tcw = min(tileW, w - t * tileW);
-    uint32_t r = k / tcw, c = k - r * tcw;
-    return (band * tileH + r) * w + t * tileW + c;
-}
-
 // Exclusive prefix over jobs of the 64-slot waves of work `work` (one value per job, thread t for
 // job t) into s_prefix[0 .. njobs]; returns the total. Call with the whole block.
 __device__ uint32_t buildPrefix(uint32_t work, uint32_t njobs, uint32_t* s_prefix)

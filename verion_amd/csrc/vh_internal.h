@@ -93,7 +93,20 @@ struct Wavefront {
     uint32_t* counts;   // 9
     uint32_t n;
     uint32_t words;
+    uint32_t tileW = 8, tileH = 8; // depth-0 pixel order: tiles, or 0 x 0 for scanlines
+    // deferred shadow rays of the plain integrator (shK = 0: shade traces them itself)
+    uint32_t shK = 0;         // records per slot
+    float* shRec = nullptr;   // kShadowWords x shK x n
+    uint32_t* shVis = nullptr;   // shK x n: 1 = occluded
+    uint32_t* shQueue = nullptr; // shK x n: record ids (k n + i) to trace, per depth
+    uint32_t* shNRec = nullptr;  // n: records of the path in slot i (0: nothing pending)
+    uint32_t* shOut = nullptr;   // n: the path's slot at the next depth, ~0 if it ends
+    uint32_t* shPixel = nullptr; // n
+    float* shTotal = nullptr;    // 3 x n: path total before its shadow contributions
 };
+
+// counts[] layout: live paths per depth at 0..kMaxDepth, shadow records per depth at kShadowCounts + d
+constexpr uint32_t kShadowCounts = 16;
 
 // Kernel launchers (vh_render.hip)
 struct LaunchCfg {
@@ -104,6 +117,7 @@ struct LaunchCfg {
 constexpr uint32_t kMaxDepth = 8;           // pack.rs:13
 constexpr uint32_t kSamplesPerDispatch = 8; // vulkan renderer.rs:29
 constexpr uint32_t kCounterBanks = 64;      // banks of the traversal counters (vh_trace.h)
+constexpr uint32_t kStageEvents = 5;        // per depth: start, after extend, shade, shadow rays, resolve
 
 // blocks of `block` threads for n threads of work, at least 1 and at most `cap`
 inline uint32_t blocks_for(uint64_t n, uint32_t block, uint32_t cap)
@@ -157,6 +171,7 @@ struct vh_renderer {
     vh_restir restir{};
     uint32_t seed = 0;
     uint32_t wfBlocks = 0;
+    uint32_t tileW = 8, tileH = 8; // depth-0 pixel order of the wavefront
     vh::DevBuf counters; // traversal counter banks while counting is on
 
     vh::Scene scene() const
@@ -191,9 +206,11 @@ struct vh_accumulation {
     uint32_t width = 0, height = 0;
     bool lighting = false;
     vh::DevBuf film, stats, light, restirBuf, state0, state1, hits, queue0, counts;
+    vh::DevBuf shRec, shVis, shQueue, shNRec, shOut, shPixel, shTotal; // deferred shadow rays
+    uint32_t shK = 0;
     uint32_t samples = 0;
     double gpuSeconds = 0.0;
-    double stage[2] = {0.0, 0.0};
+    double stage[4] = {0.0, 0.0, 0.0, 0.0}; // extend, shade, deferred shadow rays, resolve
     bool stageValid = false;
     bool adaptiveOn = false;
     vh_adaptive adaptive{};
@@ -212,6 +229,18 @@ struct vh_accumulation {
         W.counts = counts.as<uint32_t>();
         W.n = (uint32_t)pixels();
         W.words = 17 + (split ? 17 : 0);
+        W.tileW = r->tileW;
+        W.tileH = r->tileH;
+        if (!split && shK > 0 && r->info.light_count + 1 <= shK) { // (a scene update may add lights)
+            W.shK = shK;
+            W.shRec = shRec.as<float>();
+            W.shVis = shVis.as<uint32_t>();
+            W.shQueue = shQueue.as<uint32_t>();
+            W.shNRec = shNRec.as<uint32_t>();
+            W.shOut = shOut.as<uint32_t>();
+            W.shPixel = shPixel.as<uint32_t>();
+            W.shTotal = shTotal.as<float>();
+        }
         return W;
     }
 };

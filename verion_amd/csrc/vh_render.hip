@@ -21,6 +21,16 @@ namespace {
 
 constexpr uint32_t kWfBlock = 256;
 
+// Minimum waves per SIMD the compiler must fit the extend and shade kernels into (their register
+// budget). Measured on the 1080p procedural scene (8192 blocks, 8x8 tiles): free / free 167,
+// 5 / 3 177, 6 / free 154 (spills) Msamples/s.
+#ifndef VH_EXTEND_WAVES
+#define VH_EXTEND_WAVES 5
+#endif
+#ifndef VH_SHADE_WAVES
+#define VH_SHADE_WAVES 3
+#endif
+
 template <bool RESTIR, bool SPLIT>
 __global__ __launch_bounds__(64) void k_pathtrace(Scene S, Frame F, Params P, uint32_t samples)
 {
@@ -52,6 +62,8 @@ __global__ void k_wf_reset(uint32_t* counts, uint32_t n, uint32_t adaptive, uint
         counts[0] = adaptive ? 0u : n;
         for (uint32_t k = 1; k <= maxDepth; k++)
             counts[k] = 0;
+        for (uint32_t k = 0; k < maxDepth; k++)
+            counts[kShadowCounts + k] = 0;
     }
 }
 
@@ -80,81 +92,6 @@ __device__ __forceinline__ uint32_t compact(bool keep, uint32_t* counter)
     uint32_t slot = s_base + s_wave[wave] + rank;
     __syncthreads(); // s_wave and s_base are reused by the next call
     return slot;
-}
-
-// wf_select (wf_select.slang:7-17): pixels that still take samples -> depth-0 queue
-__global__ __launch_bounds__(kWfBlock) void k_wf_select(Frame F, Params P, Wavefront W)
-{
-    Scene S{};
-    Ctx c{S, F, P};
-    uint32_t n = W.n;
-    for (uint32_t base = blockIdx.x * kWfBlock; base < n; base += gridDim.x * kWfBlock) {
-        uint32_t p = base + threadIdx.x;
-        bool active = p < n && pixelActive(c, p);
-        uint32_t slot = compact(active, &W.counts[0]);
-        if (active && slot < n) // slots stay below n: counts[0] starts the sample at 0 (k_wf_reset)
-            W.queue0[slot] = p;
-    }
-}
-
-// wf_extend (wf_extend.slang:6-29): closest hit of every live path at depth d
-__global__ __launch_bounds__(kWfBlock) void k_wf_extend(Scene S, Params P, Wavefront W, uint32_t d)
-{
-    uint32_t count = W.counts[d];
-    const uint32_t* st = W.state[d & 1u];
-    for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < count; i += gridDim.x * kWfBlock) {
-        V3 org, dir;
-        if (d == 0) {
-            uint32_t p = P.w[32] != 0 ? W.queue0[i] : i;
-            PathState s;
-            generatePath(P, p % P.w[0], p / P.w[0], P.w[6], s);
-            org = s.org;
-            dir = s.dir;
-        } else {
-            org = get3(st, W.n, 0, i);
-            dir = get3(st, W.n, 3, i);
-        }
-        Hit h = trace<false>(S.bvh, org, dir, 1.0e30f);
-        W.hits[i] = make_uint4(h.hit ? 1u : 0u, h.inst, h.prim, asu(h.t));
-    }
-}
-
-// wf_shade (wf_shade.slang:7-28): shade, then append the survivor to the next depth or finish it
-template <bool RESTIR, bool SPLIT>
-__global__ __launch_bounds__(kWfBlock) void k_wf_shade(Scene S, Frame F, Params P, Wavefront W, uint32_t d)
-{
-    Ctx c{S, F, P};
-    uint32_t count = W.counts[d];
-    const uint32_t* in = W.state[d & 1u];
-    uint32_t* out = W.state[(d + 1) & 1u];
-    for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += gridDim.x * kWfBlock) {
-        uint32_t i = base + threadIdx.x;
-        bool keep = false;
-        PathState st;
-        uint32_t pixel = 0;
-        if (i < count) {
-            if (d == 0) {
-                pixel = P.w[32] != 0 ? W.queue0[i] : i;
-                generatePath(P, pixel % P.w[0], pixel / P.w[0], P.w[6], st);
-            } else {
-                loadPath<SPLIT>(in, W.n, i, st, pixel);
-            }
-            uint4 hw = W.hits[i];
-            Hit h;
-            h.hit = hw.x != 0;
-            h.inst = hw.y;
-            h.prim = hw.z;
-            h.t = asf(hw.w);
-            h.b1 = h.b2 = 0.0f;
-            h.gid = 0;
-            keep = shadePath<RESTIR, SPLIT>(c, st, h, d, pixel) && d + 1 < P.w[7];
-            if (!keep)
-                finishPath<SPLIT>(c, st, pixel);
-        }
-        uint32_t slot = compact(keep, &W.counts[d + 1]);
-        if (keep && slot < W.n) // survivors of depth d never outnumber its count (<= n)
-            storePath<SPLIT>(out, W.n, slot, st, pixel);
-    }
 }
 
 // Block-wide exclusive prefix of per-thread counts k, one atomic per block on `counter`; returns
@@ -187,6 +124,185 @@ __device__ __forceinline__ uint32_t compactN(uint32_t k, uint32_t* counter)
     uint32_t r = s_base + s_wave[wave] + x - k;
     __syncthreads();
     return r;
+}
+
+// wf_select (wf_select.slang:7-17): pixels that still take samples -> depth-0 queue
+__global__ __launch_bounds__(kWfBlock) void k_wf_select(Frame F, Params P, Wavefront W)
+{
+    Scene S{};
+    Ctx c{S, F, P};
+    uint32_t n = W.n;
+    for (uint32_t base = blockIdx.x * kWfBlock; base < n; base += gridDim.x * kWfBlock) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t p = i < n ? tilePixel(i, P.w[0], P.w[1], W.tileW, W.tileH) : 0u;
+        bool active = i < n && pixelActive(c, p);
+        uint32_t slot = compact(active, &W.counts[0]);
+        if (active && slot < n) // slots stay below n: counts[0] starts the sample at 0 (k_wf_reset)
+            W.queue0[slot] = p;
+    }
+}
+
+// wf_extend (wf_extend.slang:6-29): closest hit of every live path at depth d
+__global__ __launch_bounds__(kWfBlock, VH_EXTEND_WAVES) void k_wf_extend(Scene S, Params P, Wavefront W, uint32_t d)
+{
+    uint32_t count = W.counts[d];
+    const uint32_t* st = W.state[d & 1u];
+    for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < count; i += gridDim.x * kWfBlock) {
+        V3 org, dir;
+        if (d == 0) {
+            uint32_t p = P.w[32] != 0 ? W.queue0[i] : tilePixel(i, P.w[0], P.w[1], W.tileW, W.tileH);
+            PathState s;
+            generatePath(P, p % P.w[0], p / P.w[0], P.w[6], s);
+            org = s.org;
+            dir = s.dir;
+        } else {
+            org = get3(st, W.n, 0, i);
+            dir = get3(st, W.n, 3, i);
+        }
+        Hit h = trace<false>(S.bvh, org, dir, 1.0e30f);
+        W.hits[i] = make_uint4(h.hit ? 1u : 0u, h.inst, h.prim, asu(h.t));
+    }
+}
+
+// wf_shade (wf_shade.slang:7-28): shade, then append the survivor to the next depth or finish it
+template <bool RESTIR, bool SPLIT>
+__global__ __launch_bounds__(kWfBlock, VH_SHADE_WAVES) void k_wf_shade(Scene S, Frame F, Params P, Wavefront W, uint32_t d)
+{
+    Ctx c{S, F, P};
+    uint32_t count = W.counts[d];
+    const uint32_t* in = W.state[d & 1u];
+    uint32_t* out = W.state[(d + 1) & 1u];
+    for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += gridDim.x * kWfBlock) {
+        uint32_t i = base + threadIdx.x;
+        bool keep = false;
+        PathState st;
+        uint32_t pixel = 0;
+        if (i < count) {
+            if (d == 0) {
+                pixel = P.w[32] != 0 ? W.queue0[i] : tilePixel(i, P.w[0], P.w[1], W.tileW, W.tileH);
+                generatePath(P, pixel % P.w[0], pixel / P.w[0], P.w[6], st);
+            } else {
+                loadPath<SPLIT>(in, W.n, i, st, pixel);
+            }
+            uint4 hw = W.hits[i];
+            Hit h;
+            h.hit = hw.x != 0;
+            h.inst = hw.y;
+            h.prim = hw.z;
+            h.t = asf(hw.w);
+            h.b1 = h.b2 = 0.0f;
+            h.gid = 0;
+            keep = shadePath<RESTIR, SPLIT>(c, st, h, d, pixel) && d + 1 < P.w[7];
+            if (!keep)
+                finishPath<SPLIT>(c, st, pixel);
+        }
+        uint32_t slot = compact(keep, &W.counts[d + 1]);
+        if (keep && slot < W.n) // survivors of depth d never outnumber its count (<= n)
+            storePath<SPLIT>(out, W.n, slot, st, pixel);
+    }
+}
+
+
+// wf_shade of the plain integrator with deferred shadow rays: shade, record the NEE shadow rays
+// (with the contribution each adds when unoccluded), append survivors to the next depth. A path
+// with records stays pending (its total without them, its next slot or its end) until
+// k_wf_resolve; a path without records is stored or finished here.
+__global__ __launch_bounds__(kWfBlock, VH_SHADE_WAVES) void k_wf_shade_deferred(Scene S, Frame F, Params P, Wavefront W, uint32_t d)
+{
+    Ctx c{S, F, P};
+    uint32_t count = W.counts[d];
+    const uint32_t* in = W.state[d & 1u];
+    uint32_t* out = W.state[(d + 1) & 1u];
+    for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += gridDim.x * kWfBlock) {
+        uint32_t i = base + threadIdx.x;
+        bool keep = false;
+        PathState st;
+        uint32_t pixel = 0, nrec = 0;
+        if (i < count) {
+            if (d == 0) {
+                pixel = P.w[32] != 0 ? W.queue0[i] : tilePixel(i, P.w[0], P.w[1], W.tileW, W.tileH);
+                generatePath(P, pixel % P.w[0], pixel / P.w[0], P.w[6], st);
+            } else {
+                loadPath<false>(in, W.n, i, st, pixel);
+            }
+            uint4 hw = W.hits[i];
+            Hit h;
+            h.hit = hw.x != 0;
+            h.inst = hw.y;
+            h.prim = hw.z;
+            h.t = asf(hw.w);
+            h.b1 = h.b2 = 0.0f;
+            h.gid = 0;
+            ShadowSink sink{W.shRec, W.n, W.shK, i, 0u};
+            keep = shadePath<false, false, 0, true>(c, st, h, d, pixel, nullptr, &sink) && d + 1 < P.w[7];
+            nrec = sink.count;
+            W.shNRec[i] = nrec;
+            if (!keep && nrec == 0)
+                finishPath<false>(c, st, pixel);
+        }
+        uint32_t slot = compact(keep, &W.counts[d + 1]);
+        if (keep && slot < W.n)
+            storePath<false>(out, W.n, slot, st, pixel);
+        uint32_t q = compactN(nrec, &W.counts[kShadowCounts + d]);
+        if (nrec > 0) {
+            W.shOut[i] = keep ? slot : 0xffffffffu;
+            W.shPixel[i] = pixel;
+            put3(reinterpret_cast<uint32_t*>(W.shTotal), W.n, 0, i, st.parts.total);
+            for (uint32_t k = 0; k < nrec; k++)
+                if (q + k < W.shK * W.n)
+                    W.shQueue[q + k] = k * W.n + i;
+        }
+    }
+}
+
+// The shadow rays recorded at depth d (occluded, pt_core.slang:580-613), one per thread.
+__global__ __launch_bounds__(kWfBlock, VH_EXTEND_WAVES) void k_wf_shadow(Scene S, Params P, Wavefront W, uint32_t d)
+{
+    Frame F{};
+    Ctx c{S, F, P};
+    const uint32_t count = W.counts[kShadowCounts + d];
+    const size_t stride = (size_t)W.shK * W.n;
+    for (uint32_t q = blockIdx.x * kWfBlock + threadIdx.x; q < count; q += gridDim.x * kWfBlock) {
+        uint32_t rid = W.shQueue[q];
+        const float* r = W.shRec + rid;
+        V3 o = v3(r[0], r[stride], r[2 * stride]);
+        V3 dir = v3(r[3 * stride], r[4 * stride], r[5 * stride]);
+        float tmax = r[6 * stride];
+        W.shVis[rid] = occluded(c, o, dir, tmax) ? 1u : 0u;
+    }
+}
+
+// Adds the visible shadow contributions of every pending path at depth d in record order, then
+// stores the total in the path's next slot or finishes the path.
+__global__ __launch_bounds__(kWfBlock) void k_wf_resolve(Scene S, Frame F, Params P, Wavefront W, uint32_t d)
+{
+    Ctx c{S, F, P};
+    const uint32_t count = W.counts[d];
+    uint32_t* out = W.state[(d + 1) & 1u];
+    const size_t stride = (size_t)W.shK * W.n;
+    for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < count; i += gridDim.x * kWfBlock) {
+        uint32_t nrec = W.shNRec[i];
+        if (nrec == 0)
+            continue;
+        V3 total = get3(reinterpret_cast<const uint32_t*>(W.shTotal), W.n, 0, i);
+        for (uint32_t k = 0; k < nrec; k++) {
+            uint32_t rid = k * W.n + i;
+            if (W.shVis[rid] == 0) {
+                const float* r = W.shRec + rid;
+                total = total + v3(r[7 * stride], r[8 * stride], r[9 * stride]);
+            }
+        }
+        uint32_t os = W.shOut[i];
+        if (os != 0xffffffffu) {
+            put3(out, W.n, 13, os, total);
+        } else {
+            PathState st;
+            st.parts = Parts{total, v3(0.0f), v3(0.0f), v3(0.0f), v3(0.0f), v3(0.0f)};
+            st.visLit = 0.0f;
+            st.visTotal = 0.0f;
+            finishPath<false>(c, st, W.shPixel[i]);
+        }
+    }
 }
 
 // ---- neural radiance cache: training paths (plain paths of a pixel subset, every vertex recorded)
@@ -381,10 +497,22 @@ int launch_wf_sample(const Scene& S, const Frame& F, const Params& P0, const Wav
     for (uint32_t d = 0; d < maxDepth; d++) {
         P.w[30] = d;
         if (ev)
-            (void)hipEventRecord(ev[4 * d], s);
+            (void)hipEventRecord(ev[kStageEvents * d], s);
         k_wf_extend<<<grid, kWfBlock, 0, s>>>(S, P, W, d);
         if (ev)
-            (void)hipEventRecord(ev[4 * d + 1], s);
+            (void)hipEventRecord(ev[kStageEvents * d + 1], s);
+        if (W.shK > 0 && !restir && !split) {
+            k_wf_shade_deferred<<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+            if (ev)
+                (void)hipEventRecord(ev[kStageEvents * d + 2], s);
+            k_wf_shadow<<<grid, kWfBlock, 0, s>>>(S, P, W, d);
+            if (ev)
+                (void)hipEventRecord(ev[kStageEvents * d + 3], s);
+            k_wf_resolve<<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+            if (ev)
+                (void)hipEventRecord(ev[kStageEvents * d + 4], s);
+            continue;
+        }
         if (restir && split)
             k_wf_shade<true, true><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
         else if (restir)
@@ -394,7 +522,8 @@ int launch_wf_sample(const Scene& S, const Frame& F, const Params& P0, const Wav
         else
             k_wf_shade<false, false><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
         if (ev)
-            (void)hipEventRecord(ev[4 * d + 2], s);
+            for (uint32_t k = 2; k < kStageEvents; k++)
+                (void)hipEventRecord(ev[kStageEvents * d + k], s);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("wavefront launch: ") + hipGetErrorString(e));
@@ -428,8 +557,13 @@ int launch_wf_depth(const Scene& S, const Frame& F, const Params& P0, const Wave
     P.w[30] = d;
     uint32_t grid = cfg.wfGrid ? cfg.wfGrid : 1024;
     k_wf_extend<<<grid, kWfBlock, 0, cfg.stream>>>(S, P, W, d);
-    if (shade)
+    if (shade && W.shK > 0) {
+        k_wf_shade_deferred<<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
+        k_wf_shadow<<<grid, kWfBlock, 0, cfg.stream>>>(S, P, W, d);
+        k_wf_resolve<<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
+    } else if (shade) {
         k_wf_shade<false, false><<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
+    }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("wavefront depth: ") + hipGetErrorString(e));
 }

@@ -732,11 +732,39 @@ struct NrcIo {
 };
 constexpr uint32_t kMaxDepthNrc = 8;
 
+// Deferred shadow rays (plain wavefront). With DEFER, shadePath does not trace its NEE shadow
+// rays: it records each one with the contribution it adds to the path total when unoccluded, in
+// the order the fused code adds them; a later kernel traces the records and adds the visible
+// contributions in that order, so the path total is the same bit for bit.
+// Record k of slot i, word w (origin xyz, direction xyz, tmax, contribution rgb) lives at
+// rec[(w * K + k) * n + i].
+constexpr uint32_t kMaxShadow = 8;  // records per path at most (delta lights + one emitter/env sample)
+constexpr uint32_t kShadowWords = 10;
+struct ShadowSink {
+    float* rec;
+    uint32_t n, K, slot;
+    uint32_t count;
+    __device__ __forceinline__ void push(V3 o, V3 d, float tmax, V3 c)
+    {
+        if (count >= K) // never: K covers every delta light plus one sample (checked on the host)
+            return;
+        const size_t stride = (size_t)K * n;
+        float* r = rec + (size_t)count * n + slot;
+        const float v[kShadowWords] = {o.x, o.y, o.z, d.x, d.y, d.z, tmax, c.x, c.y, c.z};
+#pragma unroll
+        for (uint32_t w = 0; w < kShadowWords; w++)
+            r[w * stride] = v[w];
+        count++;
+    }
+};
+
 // ---------------------------------------------------------------- shadePath (pt_core.slang:992-1543)
 
-template <bool RESTIR, bool SPLIT, int NRC = 0>
-__device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t depth, uint32_t pixel, NrcIo* io = nullptr)
+template <bool RESTIR, bool SPLIT, int NRC = 0, bool DEFER = false>
+__device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t depth, uint32_t pixel, NrcIo* io = nullptr,
+                          ShadowSink* sink = nullptr)
 {
+    static_assert(!DEFER || (!RESTIR && !SPLIT && NRC == 0), "deferred shadow rays are for the plain integrator");
     const Scene& S = c.S;
     const Params& P = c.P;
     V3 sky = v3(P.f(20), P.f(21), P.f(22));
@@ -959,7 +987,11 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
                 }
             }
             float cc = dot(n, wi);
-            if (cc > 0.0f) {
+            if (DEFER && cc > 0.0f) {
+                // plain integrator: virtual lights were skipped above
+                Lobes l = evalLobes(c, surf, woL, v3(dot(wi, T), dot(wi, B), cc));
+                sink->push(p, wi, dist - 1.0e-3f, thr * (l.diffuse + l.specular) * li_);
+            } else if (cc > 0.0f) {
                 bool occ = occluded(c, p, wi, dist - 1.0e-3f);
                 bool track = lightingOn && depth == 0;
                 if (!occ || track) {
@@ -1040,7 +1072,15 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
             V3 wi = toL / dist;
             float cosl = fabsf(dot(normalize(cross(b - a, ce - a)), wi));
             float cosi = dot(n, wi);
-            if (cosi > 0.0f && cosl > 1.0e-5f && (asu(E[14]) & recvMask(c, hq.inst)) != 0) {
+            if (DEFER && cosi > 0.0f && cosl > 1.0e-5f && (asu(E[14]) & recvMask(c, hq.inst)) != 0) {
+                float pl = (1.0f - pEnv) * E[13] * d2 / (E[12] * cosl);
+                V3 wiL = v3(dot(wi, T), dot(wi, B), cosi);
+                float qb = reflectivePdf(surf, woL, wiL);
+                float wl = pl * pl / (pl * pl + qb * qb);
+                V3 lev = v3(E[9], E[10], E[11]);
+                Lobes l = evalLobes(c, surf, woL, wiL);
+                sink->push(p, wi, dist * (1.0f - 1.0e-3f), thr * (l.specular + l.diffuse) * lev / pl * wl);
+            } else if (cosi > 0.0f && cosl > 1.0e-5f && (asu(E[14]) & recvMask(c, hq.inst)) != 0) {
                 bool occ = occluded(c, p, wi, dist * (1.0f - 1.0e-3f));
                 bool track = lightingOn && depth == 0;
                 if (!occ || track) {
@@ -1071,7 +1111,14 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
             V3 leE;
             V3 wi = envSample(c, u1e, u2e, u3e, u4e, pdfE, leE);
             float cosi = dot(n, wi);
-            if (pdfE > 0.0f && cosi > 0.0f) {
+            if (DEFER && pdfE > 0.0f && cosi > 0.0f) {
+                float pl = pEnvHere * pdfE;
+                V3 wiL = v3(dot(wi, T), dot(wi, B), cosi);
+                float qb = reflectivePdf(surf, woL, wiL);
+                float wl = pl * pl / (pl * pl + qb * qb);
+                Lobes l = evalLobes(c, surf, woL, wiL);
+                sink->push(p, wi, 1.0e30f, thr * (l.specular + l.diffuse) * leE / pl * wl);
+            } else if (pdfE > 0.0f && cosi > 0.0f) {
                 bool occ = occluded(c, p, wi, 1.0e30f);
                 bool track = lightingOn && depth == 0;
                 if (!occ || track) {

@@ -11,6 +11,27 @@ namespace vh {
 
 constexpr uint32_t kBaseWords = 17; // 16 words of wf_state.slang:75 + the pixel id; WF_SPLIT (17) more follow
 
+// Depth 0 takes pixels in tile order rather than scanline order, so that a wave's 64 camera rays
+// (and their first bounces) come from one compact patch and share BVH nodes. The image does not
+// depend on the order: a pixel takes one path per sample pass.
+// Local index i (0 .. w h - 1) of the depth-0 order of a w x h image -> pixel. Bands of tileH rows, each cut
+// into tiles of tileW columns (the last band and the last tile of a band may be smaller); tiles
+// in row-major order, pixels row-major inside a tile.
+__device__ __forceinline__ uint32_t tilePixel(uint32_t i, uint32_t w, uint32_t h, uint32_t tileW, uint32_t tileH)
+{
+    if (tileW == 0)
+        return i;
+    uint32_t band = i / (tileH * w);
+    uint32_t bh = min(tileH, h - band * tileH);
+    uint32_t j = i - band * tileH * w;
+    uint32_t t = j / (tileW * bh);
+    uint32_t k = j - t * tileW * bh;
+    uint32_t tcw = min(tileW, w - t * tileW);
+    uint32_t r = k / tcw, c = k - r * tcw;
+    return (band * tileH + r) * w + t * tileW + c;
+}
+
+
 __device__ __forceinline__ void put(uint32_t* s, uint32_t n, uint32_t f, uint32_t i, float v) { s[(size_t)f * n + i] = asu(v); }
 __device__ __forceinline__ float getf(const uint32_t* s, uint32_t n, uint32_t f, uint32_t i) { return asf(s[(size_t)f * n + i]); }
 __device__ __forceinline__ void put3(uint32_t* s, uint32_t n, uint32_t f, uint32_t i, V3 v)
