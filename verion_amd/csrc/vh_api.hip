@@ -249,14 +249,14 @@ int ensure_wavefront(vh_accumulation* a)
     uint32_t K = a->r->info.light_count + 1;
     bool defer = !a->lighting && !a->restirOn && K <= kMaxShadow;
     uint64_t shBytes = defer ? (4ull * (kShadowWords + 2) * K * n + 4ull * 6 * n) : 0;
-    uint64_t bytes = 2 * 4 * words * n + 16 * n + 4 * n + 128 + shBytes;
+    uint64_t bytes = 2 * 4 * words * n + 16 * n + 4 * n + 4 * kCountWords + shBytes;
     TRY(check_fits(bytes));
     HIPTRY(a->state0.alloc(4 * words * n));
     HIPTRY(a->state1.alloc(4 * words * n));
     HIPTRY(a->hits.alloc(16 * n));
     HIPTRY(a->queue0.alloc(4 * n));
-    HIPTRY(a->counts.alloc(128));
-    HIPTRY(hipMemsetAsync(a->counts.p, 0, 128, a->r->ctx->stream));
+    HIPTRY(a->counts.alloc(4 * kCountWords));
+    HIPTRY(hipMemsetAsync(a->counts.p, 0, 4 * kCountWords, a->r->ctx->stream));
     if (defer) {
         HIPTRY(a->shRec.alloc(4ull * kShadowWords * K * n));
         HIPTRY(a->shVis.alloc(4ull * K * n));

@@ -105,8 +105,10 @@ struct Wavefront {
     float* shTotal = nullptr;    // 3 x n: path total before its shadow contributions
 };
 
-// counts[] layout: live paths per depth at 0..kMaxDepth, shadow records per depth at kShadowCounts + d
+// counts[] layout (kCountWords words): live paths per depth at 0..kMaxDepth, shadow records per
+// depth at kShadowCounts + d
 constexpr uint32_t kShadowCounts = 16;
+constexpr uint32_t kCountWords = 32;
 
 // Kernel launchers (vh_render.hip)
 struct LaunchCfg {

@@ -602,7 +602,7 @@ int vh_nrc_create(vh_renderer* r, const vh_nrc_config* cfg, vh_nrc** out)
     HIPTRY(nn->tstate0.alloc(4ull * 17 * tp));
     HIPTRY(nn->tstate1.alloc(4ull * 17 * tp));
     HIPTRY(nn->thits.alloc(16 * tp));
-    HIPTRY(nn->tcounts.alloc(64));
+    HIPTRY(nn->tcounts.alloc(4 * kCountWords));
     HIPTRY(hipStreamSynchronize(s));
     *out = nn.release();
     return VH_OK;

@@ -64,6 +64,7 @@ __global__ void k_wf_reset(uint32_t* counts, uint32_t n, uint32_t adaptive, uint
             counts[k] = 0;
         for (uint32_t k = 0; k < maxDepth; k++)
             counts[kShadowCounts + k] = 0;
+
     }
 }
 
@@ -143,12 +144,12 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_select(Frame F, Params P, Wavef
 }
 
 // wf_extend (wf_extend.slang:6-29): closest hit of every live path at depth d
+// wf_extend (wf_extend.slang:6-29): closest hit of every live path at depth d
 __global__ __launch_bounds__(kWfBlock, VH_EXTEND_WAVES) void k_wf_extend(Scene S, Params P, Wavefront W, uint32_t d)
 {
-    uint32_t count = W.counts[d];
+    const uint32_t count = W.counts[d];
     const uint32_t* st = W.state[d & 1u];
-    for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < count; i += gridDim.x * kWfBlock) {
-        V3 org, dir;
+    auto load = [&](uint32_t i, V3& org, V3& dir, float& tmax) {
         if (d == 0) {
             uint32_t p = P.w[32] != 0 ? W.queue0[i] : tilePixel(i, P.w[0], P.w[1], W.tileW, W.tileH);
             PathState s;
@@ -159,8 +160,14 @@ __global__ __launch_bounds__(kWfBlock, VH_EXTEND_WAVES) void k_wf_extend(Scene S
             org = get3(st, W.n, 0, i);
             dir = get3(st, W.n, 3, i);
         }
-        Hit h = trace<false>(S.bvh, org, dir, 1.0e30f);
-        W.hits[i] = make_uint4(h.hit ? 1u : 0u, h.inst, h.prim, asu(h.t));
+        tmax = 1.0e30f;
+    };
+    auto done = [&](uint32_t i, const Hit& h) { W.hits[i] = make_uint4(h.hit ? 1u : 0u, h.inst, h.prim, asu(h.t)); };
+    for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < count; i += gridDim.x * kWfBlock) {
+        V3 org, dir;
+        float tmax;
+        load(i, org, dir, tmax);
+        done(i, trace<false>(S.bvh, org, dir, tmax));
     }
 }
 
@@ -262,13 +269,17 @@ __global__ __launch_bounds__(kWfBlock, VH_EXTEND_WAVES) void k_wf_shadow(Scene S
     Ctx c{S, F, P};
     const uint32_t count = W.counts[kShadowCounts + d];
     const size_t stride = (size_t)W.shK * W.n;
+    auto load = [&](uint32_t q, V3& o, V3& dir, float& tmax) {
+        const float* r = W.shRec + W.shQueue[q];
+        o = v3(r[0], r[stride], r[2 * stride]);
+        dir = v3(r[3 * stride], r[4 * stride], r[5 * stride]);
+        tmax = r[6 * stride];
+    };
     for (uint32_t q = blockIdx.x * kWfBlock + threadIdx.x; q < count; q += gridDim.x * kWfBlock) {
-        uint32_t rid = W.shQueue[q];
-        const float* r = W.shRec + rid;
-        V3 o = v3(r[0], r[stride], r[2 * stride]);
-        V3 dir = v3(r[3 * stride], r[4 * stride], r[5 * stride]);
-        float tmax = r[6 * stride];
-        W.shVis[rid] = occluded(c, o, dir, tmax) ? 1u : 0u;
+        V3 o, dir;
+        float tmax;
+        load(q, o, dir, tmax);
+        W.shVis[W.shQueue[q]] = occluded(c, o, dir, tmax) ? 1u : 0u;
     }
 }
 

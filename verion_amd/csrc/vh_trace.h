@@ -82,6 +82,14 @@ VH_HD bool intersect(const RayPre& r, V3 o, V3 p0, V3 p1, V3 p2, float tmax, flo
     if (det == 0.0f)
         return false;
     float T = U * (r.sz * Akz) + V * (r.sz * Bkz) + W * (r.sz * Ckz);
+    // cheap rejections before the division, each implying the exact test below would reject:
+    // t = T / det <= 0 when the signs differ or T is 0; t >= tmax when |T| exceeds |det| tmax by
+    // more than the rounding of the two products (only while that product is a normal float)
+    if (det > 0.0f ? T <= 0.0f : T >= 0.0f)
+        return false;
+    float lim = fabsf(det) * (tmax * 1.000001f);
+    if (lim >= 1.0e-30f && fabsf(T) > lim)
+        return false;
     float t = T / det;
     if (!(t > 0.0f && t < tmax))
         return false;
@@ -136,105 +144,152 @@ __device__ __forceinline__ void count_traversal(const Bvh& bvh, uint32_t visited
     }
 }
 
-// Closest hit (any = false) or the first hit found (any = true) over (0, tmax).
+// Traversal of one ray as a resumable state machine: traceBegin sets up the ray, each traceStep
+// visits one node (its box tests, the triangles of its leaf children, the push of its inner
+// children and the pop of the next node) and returns true when the ray is done. trace() runs the
+// steps to the end; the persistent wavefront kernels interleave the steps of many rays per lane.
+struct TraceState {
+    V3 o;
+    float ix, iy, iz, ox, oy, oz;
+    RayPre pre;
+    float tmax, best; // the ray's limit, and the closest accepted t (or tmax)
+    Hit h;
+    uint32_t node;
+    int sp;
+    uint32_t visited, tested;
+};
+
+__device__ __forceinline__ void traceBegin(TraceState& s, V3 o, V3 d, float tmax)
+{
+    s.h.hit = false;
+    s.h.inst = s.h.prim = s.h.gid = 0;
+    s.h.t = 0.0f;
+    s.h.b1 = s.h.b2 = 0.0f;
+    s.o = o;
+    s.pre = ray_pre(d);
+    s.ix = safe_inv(d.x);
+    s.iy = safe_inv(d.y);
+    s.iz = safe_inv(d.z);
+    s.ox = -o.x * s.ix;
+    s.oy = -o.y * s.iy;
+    s.oz = -o.z * s.iz;
+    s.tmax = tmax;
+    s.best = tmax;
+    s.node = 0;
+    s.sp = 0;
+    s.visited = 0;
+    s.tested = 0;
+}
+
+__device__ __forceinline__ uint32_t sel4(uint4 v, uint32_t c) { return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w)); }
+
+// Compare-exchange of the sort of the inner children (nearest first).
+__device__ __forceinline__ void cmpx(float& ka, uint32_t& va, float& kb, uint32_t& vb)
+{
+    bool sw = kb < ka;
+    float k = sw ? kb : ka;
+    uint32_t v = sw ? vb : va;
+    kb = sw ? ka : kb;
+    vb = sw ? va : vb;
+    ka = k;
+    va = v;
+}
+
+// Closest hit (any = false) or the first hit found (any = true) over (0, tmax). One node per step:
+// the box test of its four children; the triangles of the leaf children hit, gathered as bits
+// 4 c + k into one loop (a wave then runs the most any lane has, not one loop per child slot);
+// the inner children hit, sorted by a 5-exchange network and pushed farthest first; then the pop
+// of the next node that can still hold a closer hit. Every triangle that can hold the closest hit
+// (ties up to the next float included) is tested, so the hit does not depend on the visiting order.
+template <bool ANY>
+__device__ __forceinline__ bool traceStep(const Bvh& bvh, TraceState& s, uint32_t* stackN, float* stackT)
+{
+    Hit& h = s.h;
+    const float4* n = bvh.nodes + 8 * (size_t)s.node;
+    s.visited++;
+    float tn[4];
+    uint32_t mask = slab4(n, s.ix, s.iy, s.iz, s.ox, s.oy, s.oz, s.best, tn);
+    uint4 child = *reinterpret_cast<const uint4*>(n + 6);
+    uint4 count = *reinterpret_cast<const uint4*>(n + 7);
+    const float kInf = __builtin_inff();
+    uint32_t tris = 0;
+    float key[4];
+    uint32_t val[4];
+#pragma unroll
+    for (uint32_t c = 0; c < 4; c++) {
+        bool hitc = (mask >> c & 1u) != 0;
+        uint32_t cnt = sel4(count, c);
+        if (hitc && cnt != 0 && !(h.hit && tn[c] > h.t))
+            tris |= ((1u << cnt) - 1u) << (4 * c);
+        key[c] = hitc && cnt == 0 ? tn[c] : kInf;
+        val[c] = sel4(child, c);
+    }
+    while (tris != 0) {
+        uint32_t b = (uint32_t)__builtin_ctz(tris);
+        tris &= tris - 1u;
+        uint32_t c = b >> 2;
+        float tc = c == 0 ? tn[0] : (c == 1 ? tn[1] : (c == 2 ? tn[2] : tn[3]));
+        if (h.hit && tc > h.t) // a closer hit was found in this node meanwhile
+            continue;
+        s.tested++;
+        const float4* tr = bvh.tris + 3 * (size_t)(sel4(child, c) + (b & 3u));
+        float4 a = tr[0], bb = tr[1], cc = tr[2];
+        uint32_t gid = asu(a.w);
+        float t, b1, b2;
+        // ties are kept for the id comparison: test up to the next float above the best t (> 0)
+        float lim = h.hit ? asf(asu(h.t) + 1u) : s.tmax;
+        if (intersect(s.pre, s.o, xyz(a), xyz(bb), xyz(cc), lim, t, b1, b2) && t < s.tmax &&
+            (!h.hit || t < h.t || (t == h.t && gid < h.gid))) {
+            h.hit = true;
+            h.t = t;
+            h.b1 = b1;
+            h.b2 = b2;
+            h.gid = gid;
+            h.inst = asu(bb.w);
+            h.prim = asu(cc.w);
+            s.best = t;
+            if (ANY)
+                return true;
+        }
+    }
+    cmpx(key[0], val[0], key[1], val[1]);
+    cmpx(key[2], val[2], key[3], val[3]);
+    cmpx(key[0], val[0], key[2], val[2]);
+    cmpx(key[1], val[1], key[3], val[3]);
+    cmpx(key[1], val[1], key[2], val[2]);
+    // the inner children hit are now a prefix, nearest first: push farthest first
+#pragma unroll
+    for (int i = 3; i >= 0; i--) {
+        if (key[i] < kInf) {
+            stackN[s.sp] = val[i];
+            stackT[s.sp] = key[i];
+            s.sp++;
+        }
+    }
+    // pop the next node that can still hold a closer hit
+    while (s.sp > 0) {
+        s.sp--;
+        if (h.hit && stackT[s.sp] > h.t)
+            continue;
+        s.node = stackN[s.sp];
+        return false;
+    }
+    return true;
+}
+
 template <bool ANY>
 __device__ Hit trace(const Bvh& bvh, V3 o, V3 d, float tmax)
 {
-    Hit h;
-    h.hit = false;
-    h.inst = h.prim = h.gid = 0;
-    h.t = 0.0f;
-    h.b1 = h.b2 = 0.0f;
+    TraceState s;
+    traceBegin(s, o, d, tmax);
     if (bvh.ntri == 0)
-        return h;
-    RayPre pre = ray_pre(d);
-    float ix = safe_inv(d.x), iy = safe_inv(d.y), iz = safe_inv(d.z);
-    float ox = -o.x * ix, oy = -o.y * iy, oz = -o.z * iz;
-    float best = tmax; // closest accepted t (or tmax)
+        return s.h;
     uint32_t stackN[kStack];
     float stackT[kStack];
-    int sp = 0;
-    uint32_t node = 0;
-    uint32_t visited = 0, tested = 0;
-    while (true) {
-        const float4* n = bvh.nodes + 8 * (size_t)node;
-        visited++;
-        float tn[4];
-        uint32_t mask = slab4(n, ix, iy, iz, ox, oy, oz, best, tn);
-        uint4 child = *reinterpret_cast<const uint4*>(n + 6);
-        uint4 count = *reinterpret_cast<const uint4*>(n + 7);
-        uint32_t ch[4] = {child.x, child.y, child.z, child.w};
-        uint32_t cn[4] = {count.x, count.y, count.z, count.w};
-        // leaves first, nearest first; inner nodes pushed farthest first
-        uint32_t inner[4];
-        float innerT[4];
-        int ni = 0;
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            if (!(mask >> c & 1u))
-                continue;
-            if (cn[c] == 0) {
-                inner[ni] = ch[c];
-                innerT[ni] = tn[c];
-                ni++;
-            } else if (!(h.hit && tn[c] > h.t)) {
-                tested += cn[c];
-                for (uint32_t k = 0; k < cn[c]; k++) {
-                    const float4* tr = bvh.tris + 3 * (size_t)(ch[c] + k);
-                    float4 a = tr[0], b = tr[1], cc = tr[2];
-                    uint32_t gid = asu(a.w);
-                    float t, b1, b2;
-                    // ties are kept for the id comparison: test up to the next float above the best t (> 0)
-                    float lim = h.hit ? asf(asu(h.t) + 1u) : tmax;
-                    if (intersect(pre, o, xyz(a), xyz(b), xyz(cc), lim, t, b1, b2) && t < tmax &&
-                        (!h.hit || t < h.t || (t == h.t && gid < h.gid))) {
-                        h.hit = true;
-                        h.t = t;
-                        h.b1 = b1;
-                        h.b2 = b2;
-                        h.gid = gid;
-                        h.inst = asu(b.w);
-                        h.prim = asu(cc.w);
-                        best = t;
-                        if (ANY) {
-                            count_traversal<ANY>(bvh, visited, tested, true);
-                            return h;
-                        }
-                    }
-                }
-            }
-        }
-        // sort the inner children by entry distance (descending) and push; the nearest is visited next
-        for (int i = 1; i < ni; i++) {
-            for (int j = i; j > 0 && innerT[j - 1] < innerT[j]; j--) {
-                float tt = innerT[j];
-                innerT[j] = innerT[j - 1];
-                innerT[j - 1] = tt;
-                uint32_t nn = inner[j];
-                inner[j] = inner[j - 1];
-                inner[j - 1] = nn;
-            }
-        }
-        for (int i = 0; i < ni; i++) {
-            stackN[sp] = inner[i];
-            stackT[sp] = innerT[i];
-            sp++;
-        }
-        // pop the next node that can still hold a closer hit
-        bool found = false;
-        while (sp > 0) {
-            sp--;
-            if (h.hit && stackT[sp] > h.t)
-                continue;
-            node = stackN[sp];
-            found = true;
-            break;
-        }
-        if (!found)
-            break;
+    while (!traceStep<ANY>(bvh, s, stackN, stackT)) {
     }
-    count_traversal<ANY>(bvh, visited, tested, h.hit);
-    return h;
+    count_traversal<ANY>(bvh, s.visited, s.tested, s.h.hit);
+    return s.h;
 }
 
 } // namespace vh
