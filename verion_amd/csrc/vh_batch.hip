@@ -35,7 +35,8 @@ namespace {
 constexpr uint32_t kBxBlock = 256;   // 4 waves
 constexpr uint32_t kBxWaves = kBxBlock / 64;
 constexpr uint32_t kBxMaxJobs = 255; // one job per thread of the prefix, one more thread for its total
-constexpr uint32_t kCountStride = kMaxDepth + 1;
+constexpr uint32_t kCountStride = 2 * kMaxDepth + 1; // per job: live paths per depth, then shadow records per depth
+constexpr uint32_t kShadowOff = kMaxDepth + 1;
 constexpr uint32_t kNotDone = 0xffffffffu;
 
 struct BatchJob {
@@ -61,6 +62,16 @@ struct BatchDev {
     uint32_t n;                    // slots
     uint32_t njobs;
     uint32_t tileW, tileH;         // depth-0 pixel order; tileW = 0: scanlines
+    // deferred shadow rays (shK = 0: shade traces them itself), as in the single-accumulation
+    // wavefront; job j's shadow queue is the segment [shK slotBase_j, shK (slotBase_j + cap_j))
+    uint32_t shK;
+    float* shRec;
+    uint32_t* shVis;
+    uint32_t* shQueue;
+    uint32_t* shNRec;
+    uint32_t* shOut;
+    uint32_t* shPixel;
+    float* shTotal;
 };
 
 // Exclusive prefix over jobs of the 64-slot waves of work `work` (one value per job, thread t for
@@ -118,6 +129,28 @@ __device__ __forceinline__ uint32_t waveAppend(bool keep, uint32_t* counter)
     base = (uint32_t)__shfl((int)base, 0, 64);
     return base + rank;
 }
+
+// Appends k entries per lane for the whole wave; returns this lane's first index. Whole wave,
+// converged.
+__device__ __forceinline__ uint32_t waveAppendN(uint32_t k, uint32_t* counter)
+{
+    uint32_t lane = threadIdx.x & 63u;
+    uint32_t x = k;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        uint32_t y = (uint32_t)__shfl_up((int)x, off, 64);
+        if (lane >= off)
+            x += y;
+    }
+    uint32_t total = (uint32_t)__shfl((int)x, 63, 64);
+    uint32_t base = 0;
+    if (lane == 63 && total != 0)
+        base = atomicAdd(counter, total);
+    base = (uint32_t)__shfl((int)base, 63, 64);
+    return base + x - k;
+}
+
+__device__ __forceinline__ uint32_t jobCapacity(const BatchJob& J) { return (J.pixels + 63u) & ~63u; }
 
 // Start of a pass: each job's depth-0 queue (all its pixels, none yet for adaptive jobs, none
 // once its samples are taken) and empty deeper queues.
@@ -196,7 +229,9 @@ __global__ __launch_bounds__(kBxBlock) void k_bx_extend(const BatchJob* __restri
 }
 
 // wf_shade (wf_shade.slang:7-28) for every job's live queue at depth d: shade, then append the
-// survivor to the job's next queue or finish it into the job's film
+// survivor to the job's next queue or finish it into the job's film. DEFER: the plain
+// integrator's NEE shadow rays are recorded into the job's shadow queue and resolved later.
+template <bool DEFER>
 __global__ __launch_bounds__(kBxBlock) void k_bx_shade(const BatchJob* __restrict__ jobs, BatchDev B, uint32_t d)
 {
     __shared__ uint32_t s_prefix[kBxMaxJobs + 1];
@@ -211,11 +246,11 @@ __global__ __launch_bounds__(kBxBlock) void k_bx_shade(const BatchJob* __restric
         const Ctx c{J.S, J.F, J.P};
         const uint32_t count = B.counts[j * kCountStride + d];
         const uint32_t i = (gw - s_prefix[j]) * 64 + lane;
+        const uint32_t slot = J.slotBase + i;
         bool keep = false;
         PathState st;
-        uint32_t pixel = 0;
+        uint32_t pixel = 0, nrec = 0;
         if (i < count) {
-            const uint32_t slot = J.slotBase + i;
             if (d == 0) {
                 pixel = J.P.w[32] != 0 ? B.queue0[slot] : tilePixel(i, J.P.w[0], J.P.w[1], B.tileW, B.tileH);
                 generatePath(J.P, pixel % J.P.w[0], pixel / J.P.w[0], J.P.w[6] + pass, st);
@@ -230,13 +265,99 @@ __global__ __launch_bounds__(kBxBlock) void k_bx_shade(const BatchJob* __restric
             h.t = asf(hw.w);
             h.b1 = h.b2 = 0.0f;
             h.gid = 0;
-            keep = shadePath<false, false>(c, st, h, d, pixel) && d + 1 < J.P.w[7];
-            if (!keep)
+            if (DEFER) {
+                ShadowSink sink{B.shRec, B.n, B.shK, slot, 0u};
+                keep = shadePath<false, false, 0, true>(c, st, h, d, pixel, nullptr, &sink) && d + 1 < J.P.w[7];
+                nrec = sink.count;
+                B.shNRec[slot] = nrec;
+            } else {
+                keep = shadePath<false, false>(c, st, h, d, pixel) && d + 1 < J.P.w[7];
+            }
+            if (!keep && nrec == 0)
                 finishPath<false>(c, st, pixel);
         }
         uint32_t k = waveAppend(keep, B.counts + j * kCountStride + d + 1);
         if (keep && k < count) // survivors of depth d never outnumber its count
             storePath<false>(out, B.n, J.slotBase + k, st, pixel);
+        if (DEFER) {
+            uint32_t q = waveAppendN(nrec, B.counts + j * kCountStride + kShadowOff + d);
+            if (nrec > 0) {
+                B.shOut[slot] = keep ? J.slotBase + k : 0xffffffffu;
+                B.shPixel[slot] = pixel;
+                put3(reinterpret_cast<uint32_t*>(B.shTotal), B.n, 0, slot, st.parts.total);
+                const uint32_t cap = B.shK * jobCapacity(J);
+                for (uint32_t r = 0; r < nrec; r++)
+                    if (q + r < cap)
+                        B.shQueue[B.shK * J.slotBase + q + r] = r * B.n + slot;
+            }
+        }
+    }
+}
+
+// The shadow rays every job recorded at depth d (occluded, pt_core.slang:580-613), one per lane.
+__global__ __launch_bounds__(kBxBlock) void k_bx_shadow(const BatchJob* __restrict__ jobs, BatchDev B, uint32_t d)
+{
+    __shared__ uint32_t s_prefix[kBxMaxJobs + 1];
+    const uint32_t total =
+        buildPrefix(threadIdx.x < B.njobs ? B.counts[threadIdx.x * kCountStride + kShadowOff + d] : 0u, B.njobs, s_prefix);
+    const uint32_t lane = threadIdx.x & 63u;
+    const size_t stride = (size_t)B.shK * B.n;
+    const Frame F{};
+    for (uint32_t gw = blockIdx.x * kBxWaves + (threadIdx.x >> 6); gw < total; gw += gridDim.x * kBxWaves) {
+        const uint32_t j = findJob(s_prefix, B.njobs, gw);
+        const BatchJob& J = jobs[j];
+        const uint32_t count = B.counts[j * kCountStride + kShadowOff + d];
+        const uint32_t i = (gw - s_prefix[j]) * 64 + lane;
+        if (i >= count)
+            continue;
+        const Ctx c{J.S, F, J.P};
+        uint32_t rid = B.shQueue[B.shK * J.slotBase + i];
+        const float* r = B.shRec + rid;
+        V3 o = v3(r[0], r[stride], r[2 * stride]);
+        V3 dir = v3(r[3 * stride], r[4 * stride], r[5 * stride]);
+        B.shVis[rid] = occluded(c, o, dir, r[6 * stride]) ? 1u : 0u;
+    }
+}
+
+// Adds each pending path's visible shadow contributions in record order, then stores the total in
+// its next slot or finishes the path into its job's film.
+__global__ __launch_bounds__(kBxBlock) void k_bx_resolve(const BatchJob* __restrict__ jobs, BatchDev B, uint32_t d)
+{
+    __shared__ uint32_t s_prefix[kBxMaxJobs + 1];
+    const uint32_t total = buildPrefix(threadIdx.x < B.njobs ? B.counts[threadIdx.x * kCountStride + d] : 0u, B.njobs, s_prefix);
+    const uint32_t lane = threadIdx.x & 63u;
+    const size_t stride = (size_t)B.shK * B.n;
+    uint32_t* out = B.state[(d + 1) & 1u];
+    for (uint32_t gw = blockIdx.x * kBxWaves + (threadIdx.x >> 6); gw < total; gw += gridDim.x * kBxWaves) {
+        const uint32_t j = findJob(s_prefix, B.njobs, gw);
+        const BatchJob& J = jobs[j];
+        const uint32_t count = B.counts[j * kCountStride + d];
+        const uint32_t i = (gw - s_prefix[j]) * 64 + lane;
+        if (i >= count)
+            continue;
+        const uint32_t slot = J.slotBase + i;
+        const uint32_t nrec = B.shNRec[slot];
+        if (nrec == 0)
+            continue;
+        V3 sum = get3(reinterpret_cast<const uint32_t*>(B.shTotal), B.n, 0, slot);
+        for (uint32_t k = 0; k < nrec; k++) {
+            uint32_t rid = k * B.n + slot;
+            if (B.shVis[rid] == 0) {
+                const float* r = B.shRec + rid;
+                sum = sum + v3(r[7 * stride], r[8 * stride], r[9 * stride]);
+            }
+        }
+        uint32_t os = B.shOut[slot];
+        if (os != 0xffffffffu) {
+            put3(out, B.n, 13, os, sum);
+        } else {
+            const Ctx c{J.S, J.F, J.P};
+            PathState st;
+            st.parts = Parts{sum, v3(0.0f), v3(0.0f), v3(0.0f), v3(0.0f), v3(0.0f)};
+            st.visLit = 0.0f;
+            st.visTotal = 0.0f;
+            finishPath<false>(c, st, B.shPixel[slot]);
+        }
     }
 }
 
@@ -277,6 +398,8 @@ struct vh_batch {
     std::vector<uint32_t> slotBase;
     std::vector<DevBuf> films, stats;
     DevBuf jobTable, state0, state1, hits, queue0, counts, pass, donePass, traced, passPaths;
+    DevBuf shRec, shVis, shQueue, shNRec, shOut, shPixel, shTotal; // deferred shadow rays
+    uint32_t shK = 0;
     uint32_t n = 0;
     uint32_t maxSpp = 0;
     bool anyAdaptive = false;
@@ -309,6 +432,14 @@ struct vh_batch {
         B.njobs = (uint32_t)jobs.size();
         B.tileW = tuning.tile_width;
         B.tileH = tuning.tile_height;
+        B.shK = shK;
+        B.shRec = shRec.as<float>();
+        B.shVis = shVis.as<uint32_t>();
+        B.shQueue = shQueue.as<uint32_t>();
+        B.shNRec = shNRec.as<uint32_t>();
+        B.shOut = shOut.as<uint32_t>();
+        B.shPixel = shPixel.as<uint32_t>();
+        B.shTotal = shTotal.as<float>();
         return B;
     }
     void dropGraph()
@@ -350,7 +481,13 @@ int issue_pass(vh_batch* b)
         k_bx_select<<<grid, kBxBlock, 0, s>>>(jobs, B);
     for (uint32_t d = 0; d < kMaxDepth; d++) {
         k_bx_extend<<<grid, kBxBlock, 0, s>>>(jobs, B, d);
-        k_bx_shade<<<grid, kBxBlock, 0, s>>>(jobs, B, d);
+        if (B.shK > 0) {
+            k_bx_shade<true><<<grid, kBxBlock, 0, s>>>(jobs, B, d);
+            k_bx_shadow<<<grid, kBxBlock, 0, s>>>(jobs, B, d);
+            k_bx_resolve<<<grid, kBxBlock, 0, s>>>(jobs, B, d);
+        } else {
+            k_bx_shade<false><<<grid, kBxBlock, 0, s>>>(jobs, B, d);
+        }
     }
     k_bx_end<<<1, kBxBlock, 0, s>>>(jobs, B);
     hipError_t e = hipGetLastError();
@@ -432,8 +569,24 @@ int vh_batch_create(vh_context* ctx, const vh_batch_job* jobs, uint32_t count, v
         return fail(VH_ERR_INVALID_ARGUMENT, "a batch holds fewer than 2^31 pixels");
     b->n = (uint32_t)slots;
     const uint64_t n = slots;
-    uint64_t bytes = filmBytes + n * (2 * 17 * 4 + 16 + 4) + count * (sizeof(BatchJob) + 64 + 16) + 8ull * b->maxSpp;
+    // deferred shadow rays: one record per delta light plus one sample, for the job with most lights
+    uint32_t K = 0;
+    for (uint32_t j = 0; j < count; j++)
+        K = std::max(K, jobs[j].renderer->info.light_count + 1);
+    const bool defer = K <= kMaxShadow && (uint64_t)K * n < (1ull << 31);
+    uint64_t shBytes = defer ? (4ull * (kShadowWords + 2) * K * n + 4ull * 6 * n) : 0;
+    uint64_t bytes = filmBytes + n * (2 * 17 * 4 + 16 + 4) + count * (sizeof(BatchJob) + 4 * kCountStride + 16) + 8ull * b->maxSpp + shBytes;
     TRY(check_fits(bytes));
+    if (defer) {
+        HIPTRY(b->shRec.alloc(4ull * kShadowWords * K * n));
+        HIPTRY(b->shVis.alloc(4ull * K * n));
+        HIPTRY(b->shQueue.alloc(4ull * K * n));
+        HIPTRY(b->shNRec.alloc(4 * n));
+        HIPTRY(b->shOut.alloc(4 * n));
+        HIPTRY(b->shPixel.alloc(4 * n));
+        HIPTRY(b->shTotal.alloc(12 * n));
+        b->shK = K;
+    }
     HIPTRY(b->state0.alloc(4 * 17 * n));
     HIPTRY(b->state1.alloc(4 * 17 * n));
     HIPTRY(b->hits.alloc(16 * n));
