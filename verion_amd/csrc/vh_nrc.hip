@@ -346,15 +346,34 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcNet net, const float4* sam
     }
 }
 
-// grad[p] = sum over blocks of the partials (in block order: deterministic); loss likewise.
-__global__ void k_nrc_reduce(const float* partials, uint32_t blocks, float* grad, const float* lossPartials, float* loss,
-                             const uint32_t* countPtr, uint32_t countImm, uint32_t cap)
+// The gradient is the sum of the per-block partials in a fixed order (deterministic for a given
+// sample order): first groups of kReduceGroup consecutive blocks, in block order, into one sum per
+// group (enough threads to stream the partials at memory bandwidth), then the group sums in
+// group order.
+constexpr uint32_t kReduceGroup = 16;
+
+__global__ void k_nrc_reduce_groups(const float* partials, uint32_t blocks, float* groupSums)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t g = blockIdx.y;
+    if (p >= (uint32_t)kParams)
+        return;
+    uint32_t b0 = g * kReduceGroup, b1 = min(blocks, b0 + kReduceGroup);
+    float s = 0.0f;
+    for (uint32_t b = b0; b < b1; b++)
+        s += partials[(size_t)b * kParams + p];
+    groupSums[(size_t)g * kParams + p] = s;
+}
+
+// grad[p] = sum of the group sums in group order; loss likewise over the blocks' loss partials.
+__global__ void k_nrc_reduce(const float* groupSums, uint32_t groups, float* grad, const float* lossPartials, uint32_t blocks,
+                             float* loss, const uint32_t* countPtr, uint32_t countImm, uint32_t cap)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < (uint32_t)kParams) {
         float s = 0.0f;
-        for (uint32_t b = 0; b < blocks; b++)
-            s += partials[(size_t)b * kParams + p];
+        for (uint32_t g = 0; g < groups; g++)
+            s += groupSums[(size_t)g * kParams + p];
         grad[p] = s;
     }
     if (p == 0) {
@@ -449,15 +468,22 @@ using namespace vh;
 struct vh_nrc {
     vh_renderer* r = nullptr;
     vh_nrc_config cfg{};
-    DevBuf theta, m, v, grad, wt, wb, partials, lossPart, loss;
+    DevBuf theta, m, v, grad, wt, wb, partials, groupSums, lossPart, loss;
     DevBuf avg, evalTheta, ewt; // moving average of theta and its forward copy (inference)
-    DevBuf trainPix, vertices, tstate0, tstate1, thits, tcounts, queries;
-    DevBuf samples[2], sampleCount[2]; // training samples of alternate training frames
-    // Training paths run on their own stream beside the render; pathsDone[b] / adamDone[b] order
-    // the writes and reads of samples[b] between the two streams.
-    hipStream_t side = nullptr;
-    hipEvent_t pathsDone[2] = {nullptr, nullptr}, adamDone[2] = {nullptr, nullptr};
-    bool adamIssued[2] = {false, false};
+    DevBuf queries;
+    // Training paths run beside the render, two training frames ahead, on two side streams with a
+    // wavefront each (65,536 paths are too few to fill the GPU: a frame of them is bound by
+    // traversal latency, so two run at once). samples[tf % 3] receives training frame tf;
+    // pathsDone[b] / adamDone[b] order its writes and reads between the streams.
+    struct TrainSet {
+        DevBuf trainPix, vertices, state0, state1, hits, counts;
+        hipStream_t stream = nullptr;
+    };
+    static constexpr uint32_t kSets = 2, kBufs = 3;
+    TrainSet sets[kSets];
+    DevBuf samples[kBufs], sampleCount[kBufs];
+    hipEvent_t pathsDone[kBufs] = {}, adamDone[kBufs] = {};
+    bool adamIssued[kBufs] = {};
     uint32_t step = 0;
     uint32_t frames = 0;
     uint32_t trainFrames = 0; // frames of training paths drawn so far (warm-up included)
@@ -485,16 +511,18 @@ struct vh_nrc {
     }
     ~vh_nrc()
     {
-        if (side)
-            (void)hipStreamSynchronize(side);
-        for (int b = 0; b < 2; b++) {
+        for (auto& t : sets)
+            if (t.stream)
+                (void)hipStreamSynchronize(t.stream);
+        for (uint32_t b = 0; b < kBufs; b++) {
             if (pathsDone[b])
                 (void)hipEventDestroy(pathsDone[b]);
             if (adamDone[b])
                 (void)hipEventDestroy(adamDone[b]);
         }
-        if (side)
-            (void)hipStreamDestroy(side);
+        for (auto& t : sets)
+            if (t.stream)
+                (void)hipStreamDestroy(t.stream);
     }
 };
 
@@ -514,8 +542,12 @@ int train_steps(vh_nrc* nn, const float4* samples, const uint32_t* countPtr, uin
     NrcNet net = nn->net(countPtr == nullptr);
     for (uint32_t k = 0; k < steps; k++) {
         k_nrc_train<<<kTrainBlocks, 256, 0, s>>>(net, samples, countPtr, countImm, cap, nn->partials.as<float>(), nn->lossPart.as<float>());
-        k_nrc_reduce<<<(kParams + 255) / 256, 256, 0, s>>>(nn->partials.as<float>(), kTrainBlocks, nn->grad.as<float>(),
-                                                           nn->lossPart.as<float>(), nn->loss.as<float>(), countPtr, countImm, cap);
+        const uint32_t groups = (kTrainBlocks + kReduceGroup - 1) / kReduceGroup;
+        k_nrc_reduce_groups<<<dim3((kParams + 255) / 256, groups), 256, 0, s>>>(nn->partials.as<float>(), kTrainBlocks,
+                                                                                nn->groupSums.as<float>());
+        k_nrc_reduce<<<(kParams + 255) / 256, 256, 0, s>>>(nn->groupSums.as<float>(), groups, nn->grad.as<float>(),
+                                                           nn->lossPart.as<float>(), kTrainBlocks, nn->loss.as<float>(), countPtr,
+                                                           countImm, cap);
         if (adam) {
             nn->step++;
             float c1 = 1.0f - powf(0.9f, (float)nn->step), c2 = 1.0f - powf(0.99f, (float)nn->step);
@@ -571,7 +603,7 @@ int vh_nrc_create(vh_renderer* r, const vh_nrc_config* cfg, vh_nrc** out)
         return fail(VH_ERR_INVALID_ARGUMENT, "learning_rate must be positive");
     hipStream_t s = r->ctx->stream;
     uint64_t tp = nn->cfg.train_paths;
-    uint64_t bytes = 4ull * kParams * 6 + 2ull * (2 * kWt + kWb) + 4ull * kParams * kTrainBlocks + tp * (4 + 16 * 5 * kMaxDepthNrc + 2 * 64 * kMaxDepthNrc + 2 * 4 * 17 + 16);
+    uint64_t bytes = 4ull * kParams * 6 + 2ull * (2 * kWt + kWb) + 4ull * kParams * kTrainBlocks + tp * (2 * (4 + 16 * 5 * kMaxDepthNrc + 2 * 4 * 17 + 16) + 3 * 64 * kMaxDepthNrc);
     TRY(check_fits(bytes));
     HIPTRY(nn->theta.alloc(4ull * kParams));
     HIPTRY(nn->m.alloc(4ull * kParams));
@@ -583,6 +615,7 @@ int vh_nrc_create(vh_renderer* r, const vh_nrc_config* cfg, vh_nrc** out)
     HIPTRY(nn->ewt.alloc(2ull * kWt));
     HIPTRY(nn->wb.alloc(2ull * kWb));
     HIPTRY(nn->partials.alloc(4ull * kParams * kTrainBlocks));
+    HIPTRY(nn->groupSums.alloc(4ull * kParams * ((kTrainBlocks + kReduceGroup - 1) / kReduceGroup)));
     HIPTRY(nn->lossPart.alloc(4ull * kTrainBlocks));
     HIPTRY(nn->loss.alloc(16));
     HIPTRY(hipMemsetAsync(nn->theta.p, 0, 4ull * kParams, s));
@@ -590,19 +623,21 @@ int vh_nrc_create(vh_renderer* r, const vh_nrc_config* cfg, vh_nrc** out)
     HIPTRY(hipMemsetAsync(nn->v.p, 0, 4ull * kParams, s));
     k_zero_bf16<<<(kWb + 255) / 256, 256, 0, s>>>(nn->wb.as<__bf16>(), kWb); // padding columns stay zero
     k_nrc_copies<<<(kParams + 255) / 256, 256, 0, s>>>(nn->theta.as<float>(), nn->wt.as<__bf16>(), nn->wb.as<__bf16>(), nn->average());
-    HIPTRY(nn->trainPix.alloc(4 * tp));
-    HIPTRY(nn->vertices.alloc(16ull * kNrcVertexWords * kMaxDepthNrc * tp));
-    for (int b = 0; b < 2; b++) {
+    for (auto& t : nn->sets) {
+        HIPTRY(t.trainPix.alloc(4 * tp));
+        HIPTRY(t.vertices.alloc(16ull * kNrcVertexWords * kMaxDepthNrc * tp));
+        HIPTRY(t.state0.alloc(4ull * 17 * tp));
+        HIPTRY(t.state1.alloc(4ull * 17 * tp));
+        HIPTRY(t.hits.alloc(16 * tp));
+        HIPTRY(t.counts.alloc(4 * kCountWords));
+        HIPTRY(hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking));
+    }
+    for (uint32_t b = 0; b < vh_nrc::kBufs; b++) {
         HIPTRY(nn->samples[b].alloc(64ull * kMaxDepthNrc * tp));
         HIPTRY(nn->sampleCount[b].alloc(16));
         HIPTRY(hipEventCreateWithFlags(&nn->pathsDone[b], hipEventDisableTiming));
         HIPTRY(hipEventCreateWithFlags(&nn->adamDone[b], hipEventDisableTiming));
     }
-    HIPTRY(hipStreamCreateWithFlags(&nn->side, hipStreamNonBlocking));
-    HIPTRY(nn->tstate0.alloc(4ull * 17 * tp));
-    HIPTRY(nn->tstate1.alloc(4ull * 17 * tp));
-    HIPTRY(nn->thits.alloc(16 * tp));
-    HIPTRY(nn->tcounts.alloc(4 * kCountWords));
     HIPTRY(hipStreamSynchronize(s));
     *out = nn.release();
     return VH_OK;
@@ -709,37 +744,37 @@ int vh_nrc_accumulate(vh_nrc* nn, vh_accumulation* a, uint32_t frames)
     Scene S = r->scene();
     Frame F{a->film.as<double>(), nullptr, nullptr, nullptr};
     Wavefront W = a->wavefront(false);
-    Wavefront T;
-    T.state[0] = nn->tstate0.as<uint32_t>();
-    T.state[1] = nn->tstate1.as<uint32_t>();
-    T.hits = nn->thits.as<uint4>();
-    T.queue0 = nullptr;
-    T.counts = nn->tcounts.as<uint32_t>();
-    T.n = nn->cfg.train_paths;
-    T.words = 17;
     LaunchCfg cfg{s, r->wavefrontBlocks()};
-    LaunchCfg sideCfg{nn->side, r->wavefrontBlocks()};
     Params base = base_params(r, a->width, a->height, false, false, vh_adaptive{}, false, vh_restir{});
     const uint32_t termDepth = nn->cfg.terminate_depth;
     const uint32_t maxDepth = base.w[7];
     const uint32_t sampleCap = nn->cfg.train_paths * kMaxDepthNrc;
-    // Training frame tf: plain paths on a random pixel subset (their own sample stream) on the side
-    // stream into samples[tf & 1], after the Adam steps that last read that buffer; then Adam steps
-    // on the render stream once those paths are done.
+    // Training frame tf: plain paths on a random pixel subset (their own sample stream) on side
+    // stream tf % 2 into samples[tf % 3], after the Adam steps that last read that buffer; then
+    // Adam steps on the render stream once those paths are done.
     auto issuePaths = [&](uint32_t tf) -> int {
-        uint32_t b = tf & 1u;
+        vh_nrc::TrainSet& ts = nn->sets[tf % vh_nrc::kSets];
+        const uint32_t b = tf % vh_nrc::kBufs;
         if (nn->adamIssued[b])
-            HIPTRY(hipStreamWaitEvent(nn->side, nn->adamDone[b], 0));
+            HIPTRY(hipStreamWaitEvent(ts.stream, nn->adamDone[b], 0));
+        Wavefront T;
+        T.state[0] = ts.state0.as<uint32_t>();
+        T.state[1] = ts.state1.as<uint32_t>();
+        T.hits = ts.hits.as<uint4>();
+        T.queue0 = nullptr;
+        T.counts = ts.counts.as<uint32_t>();
+        T.n = nn->cfg.train_paths;
+        T.words = 17;
         Params PT = base;
         PT.w[6] = 0x40000000u + tf;
-        TRY(launch_nrc_train_paths(S, PT, T, nn->trainPix.as<uint32_t>(), nn->cfg.train_paths, (uint32_t)npix, tf,
-                                   nn->vertices.as<float4>(), nn->samples[b].as<float4>(), nn->sampleCount[b].as<uint32_t>(), sampleCap,
-                                   sideCfg));
-        HIPTRY(hipEventRecord(nn->pathsDone[b], nn->side));
+        TRY(launch_nrc_train_paths(S, PT, T, ts.trainPix.as<uint32_t>(), nn->cfg.train_paths, (uint32_t)npix, tf,
+                                   ts.vertices.as<float4>(), nn->samples[b].as<float4>(), nn->sampleCount[b].as<uint32_t>(), sampleCap,
+                                   LaunchCfg{ts.stream, r->wavefrontBlocks()}));
+        HIPTRY(hipEventRecord(nn->pathsDone[b], ts.stream));
         return VH_OK;
     };
     auto adam = [&](uint32_t tf) -> int {
-        uint32_t b = tf & 1u;
+        const uint32_t b = tf % vh_nrc::kBufs;
         HIPTRY(hipStreamWaitEvent(s, nn->pathsDone[b], 0));
         TRY(train_steps(nn, nn->samples[b].as<float4>(), nn->sampleCount[b].as<uint32_t>(), 0, sampleCap, nn->cfg.train_steps, true, s));
         HIPTRY(hipEventRecord(nn->adamDone[b], s));
@@ -769,7 +804,7 @@ int vh_nrc_accumulate(vh_nrc* nn, vh_accumulation* a, uint32_t frames)
         return VH_OK;
     };
     // Training frames of this call (the warm-up on the cache's first call, then one per frame); the
-    // paths of training frame j + 1 run beside the Adam steps and render of frame j. Timing events
+    // paths of training frame j + 2 run beside the Adam steps and render of frame j. Timing events
     // on the render stream: ev[0] at the start, then per training frame (after Adam, after render).
     const uint32_t warm = nn->frames == 0 ? nn->cfg.warmup_frames : 0;
     const uint32_t total = warm + frames;
@@ -777,18 +812,19 @@ int vh_nrc_accumulate(vh_nrc* nn, vh_accumulation* a, uint32_t frames)
     for (auto& e : ev)
         HIPTRY(hipEventCreate(&e));
     int rc = check_hip(hipEventRecord(ev[0], s), "hipEventRecord");
-    // the side stream starts after everything issued before this call (scene uploads, parameters)
-    if (rc == VH_OK)
-        rc = check_hip(hipStreamWaitEvent(nn->side, ev[0], 0), "hipStreamWaitEvent");
+    // the side streams start after everything issued before this call (scene uploads, parameters)
+    for (auto& t : nn->sets)
+        if (rc == VH_OK)
+            rc = check_hip(hipStreamWaitEvent(t.stream, ev[0], 0), "hipStreamWaitEvent");
     const uint32_t tf0 = nn->trainFrames;
-    if (rc == VH_OK)
-        rc = issuePaths(tf0);
+    for (uint32_t j = 0; j < std::min(total, 2u) && rc == VH_OK; j++)
+        rc = issuePaths(tf0 + j);
     for (uint32_t j = 0; j < total && rc == VH_OK; j++) {
         rc = adam(tf0 + j);
         if (rc == VH_OK)
             rc = check_hip(hipEventRecord(ev[1 + 2 * j], s), "hipEventRecord");
-        if (rc == VH_OK && j + 1 < total)
-            rc = issuePaths(tf0 + j + 1);
+        if (rc == VH_OK && j + 2 < total)
+            rc = issuePaths(tf0 + j + 2);
         if (rc == VH_OK && j >= warm)
             rc = render(j - warm);
         if (rc == VH_OK)
@@ -797,7 +833,8 @@ int vh_nrc_accumulate(vh_nrc* nn, vh_accumulation* a, uint32_t frames)
     if (rc == VH_OK && hipEventSynchronize(ev[2 * total]) != hipSuccess)
         rc = fail(VH_ERR_KERNEL, "NRC frames failed on the device");
     if (rc != VH_OK) {
-        (void)hipStreamSynchronize(nn->side);
+        for (auto& t : nn->sets)
+            (void)hipStreamSynchronize(t.stream);
         (void)hipStreamSynchronize(s);
     }
     float trainMs = 0.0f, renderMs = 0.0f;
