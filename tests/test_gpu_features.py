@@ -262,3 +262,20 @@ def test_depth0_order_and_grid_do_not_change_the_image(ctx, adaptive):
         assert np.array_equal(f, films[0][0])
         if adaptive:
             assert np.array_equal(s, films[0][1])
+
+
+def test_many_delta_lights_shade_fused_and_match_the_oracle(ctx):
+    # more than 7 delta lights: the wavefront shades with its shadow rays inline (no deferred records)
+    from verion_amd import scene as sc
+
+    s = scenes.floor_scene(sc.PointLight(position=(0.0, 0.0, 2.0), color=(1.0, 1.0, 1.0), intensity=2.0), occluder=True)
+    for k in range(8):
+        s.add_light(sc.PointLight(position=(0.3 * k - 1.0, 0.5, 1.5), color=(0.5, 0.6, 0.7), intensity=1.0))
+    r = Renderer(ctx, s)
+    assert r.packed.info.light_count >= 9
+    r.set_integrator(WAVEFRONT)
+    acc = r.new_accumulation(24, 24)
+    r.accumulate(acc, 2)
+    assert acc.wavefront_bytes() == 2 * 4 * 17 * 576 + 16 * 576 + 4 * 576 + 128  # no shadow records
+    st = oracle.OracleScene(r.packed).render(params(r.packed, 24, 24), 2)
+    assert np.array_equal(acc.film(), st["film"])

@@ -117,3 +117,27 @@ def test_closed_room_series(ctx):
     got = float(r.render(64, 64, 64)[:, 0].mean())
     want = 1.0 * (1.0 - 0.5 ** 8) / (1.0 - 0.5)
     assert abs(got - want) / want < 0.01
+
+
+@pytest.mark.parametrize("ntri", [1, 2, 3, 4, 5])
+def test_tiny_scenes_trace_and_render(ctx, ntri):
+    # scenes of at most 4 triangles have a single BVH node with empty child slots
+    from verion_amd import scene as sc
+
+    rng = np.random.default_rng(ntri)
+    s = sc.Scene()
+    v = (rng.random((3 * ntri, 3)) * 2.0 - 1.0).astype(np.float32)
+    v[:, 2] = rng.random(3 * ntri).astype(np.float32) * 0.5
+    m = s.add_mesh(sc.Mesh(v, np.arange(3 * ntri, dtype=np.uint32)))
+    s.add_instance(sc.Instance(m, s.add_material(sc.lambert((0.7, 0.6, 0.5), (0.2, 0.2, 0.2)))))
+    s.add_light(sc.DirectionalLight(direction=(0.3, -0.2, -1.0), color=(1.0, 1.0, 1.0), intensity=2.0))
+    s.camera = scenes.camera_at_z(3.0)
+    r = Renderer(ctx, s)
+    p = r.packed
+    o = oracle.OracleScene(p)
+    par = pack.base_params(p.info, p.view, 24, 20, False, True)
+    assert np.array_equal(r.primary_hits(24, 20, 0), o.primary_hits(par, 0, brute=True))
+    r.set_integrator(WAVEFRONT)
+    acc = r.new_accumulation(24, 20)
+    r.accumulate(acc, 2)
+    assert np.array_equal(acc.film(), o.render(par, 2)["film"])

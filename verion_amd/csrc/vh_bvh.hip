@@ -396,12 +396,12 @@ __global__ __launch_bounds__(256) void k_collapse(const float4* __restrict__ tri
         float lo[3][4], hi[3][4];
         uint32_t cref[4], ccount[4];
         for (uint32_t k = 0; k < 4; k++) {
-            if (k >= nref) {
+            if (k >= nref) { // empty slot: the traversal skips the kEmptyChild reference
                 for (int a = 0; a < 3; a++) {
                     lo[a][k] = __builtin_inff();
                     hi[a][k] = -__builtin_inff();
                 }
-                cref[k] = 0;
+                cref[k] = kEmptyChild;
                 ccount[k] = 0;
                 continue;
             }
@@ -455,7 +455,7 @@ __global__ void k_root_leaf(const float4* __restrict__ tris, uint32_t n, float4*
     nodes[3] = make_float4(hi[1] + pad, -inf, -inf, -inf);
     nodes[4] = make_float4(lo[2] - pad, inf, inf, inf);
     nodes[5] = make_float4(hi[2] + pad, -inf, -inf, -inf);
-    nodes[6] = make_float4(asf(0u), asf(0u), asf(0u), asf(0u));
+    nodes[6] = make_float4(asf(0u), asf(kEmptyChild), asf(kEmptyChild), asf(kEmptyChild));
     nodes[7] = make_float4(asf(n), asf(0u), asf(0u), asf(0u));
 }
 

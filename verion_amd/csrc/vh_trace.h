@@ -102,6 +102,10 @@ VH_HD bool intersect(const RayPre& r, V3 o, V3 p0, V3 p1, V3 p2, float tmax, flo
 __device__ __forceinline__ V3 xyz(float4 a) { return v3(a.x, a.y, a.z); }
 
 constexpr int kStack = 96;
+// Child reference of an empty slot of a 4-wide node (nodes with fewer than 4 children: the root of
+// a scene of at most 4 triangles). Its box is inverted (lo = +inf, hi = -inf), which the FMA slab
+// test would take for an infinite box, so the traversal skips it by this reference.
+constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
 
 // Box test of the four children: returns a 4-bit mask of hits and their entry distances.
 __device__ __forceinline__ uint32_t slab4(const float4* n, float ix, float iy, float iz, float ox, float oy, float oz,
@@ -217,7 +221,7 @@ __device__ __forceinline__ bool traceStep(const Bvh& bvh, TraceState& s, uint32_
     uint32_t val[4];
 #pragma unroll
     for (uint32_t c = 0; c < 4; c++) {
-        bool hitc = (mask >> c & 1u) != 0;
+        bool hitc = (mask >> c & 1u) != 0 && sel4(child, c) != kEmptyChild;
         uint32_t cnt = sel4(count, c);
         if (hitc && cnt != 0 && !(h.hit && tn[c] > h.t))
             tris |= ((1u << cnt) - 1u) << (4 * c);
@@ -252,24 +256,27 @@ __device__ __forceinline__ bool traceStep(const Bvh& bvh, TraceState& s, uint32_
                 return true;
         }
     }
-    cmpx(key[0], val[0], key[1], val[1]);
-    cmpx(key[2], val[2], key[3], val[3]);
-    cmpx(key[0], val[0], key[2], val[2]);
-    cmpx(key[1], val[1], key[3], val[3]);
-    cmpx(key[1], val[1], key[2], val[2]);
-    // the inner children hit are now a prefix, nearest first: push farthest first
+    if (!ANY) { // an any-hit ray takes the children in slot order: any occluder ends it
+        cmpx(key[0], val[0], key[1], val[1]);
+        cmpx(key[2], val[2], key[3], val[3]);
+        cmpx(key[0], val[0], key[2], val[2]);
+        cmpx(key[1], val[1], key[3], val[3]);
+        cmpx(key[1], val[1], key[2], val[2]);
+    }
+    // the inner children hit, nearest first (closest hit), are pushed farthest first
 #pragma unroll
     for (int i = 3; i >= 0; i--) {
-        if (key[i] < kInf) {
+        if (key[i] < kInf && s.sp < kStack) { // the build bounds the depth: the stack cannot fill
             stackN[s.sp] = val[i];
-            stackT[s.sp] = key[i];
+            if (!ANY) // an any-hit ray ends at its first hit: entry distances are never compared
+                stackT[s.sp] = key[i];
             s.sp++;
         }
     }
     // pop the next node that can still hold a closer hit
     while (s.sp > 0) {
         s.sp--;
-        if (h.hit && stackT[s.sp] > h.t)
+        if (!ANY && h.hit && stackT[s.sp] > h.t)
             continue;
         s.node = stackN[s.sp];
         return false;
