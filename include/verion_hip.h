@@ -97,7 +97,8 @@ typedef struct vh_adaptive {
 /* Restir (lib.rs:131-149). */
 typedef struct vh_restir {
     uint32_t di;         /* bool */
-    uint32_t gi;         /* bool; ReSTIR GI is not built: a renderer rejects gi = 1 with VH_ERR_UNSUPPORTED */
+    uint32_t gi;         /* bool; ReSTIR GI (pt_core.slang:901-980, 1566-1617): reconnection reuse of the
+                            first bounce's hit; off while the lighting split is on, like the reference */
     uint32_t candidates;
     uint32_t history;
 } vh_restir;
@@ -153,7 +154,7 @@ int vh_integrator(const vh_renderer* r);
 int vh_set_stage_timing(vh_renderer* r, int on);
 /* NULL switches adaptive sampling off. */
 int vh_set_adaptive(vh_renderer* r, const vh_adaptive* adaptive);
-/* NULL switches ReSTIR off. gi = 1 is VH_ERR_UNSUPPORTED. */
+/* NULL switches ReSTIR off (Renderer::set_restir, lib.rs:131-149). */
 int vh_set_restir(vh_renderer* r, const vh_restir* restir);
 /* Random-stream seed for accumulations started afterwards. 0 (the default) is the reference's own
  * stream, pcg(pixel ^ pcg(sample * 0x9E3779B9 + 0x7F4A7C15)) (pt_core.slang:651); any other seed

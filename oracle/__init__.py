@@ -69,6 +69,8 @@ def lib():
         L.or_intersect.argtypes = [f32p, f32p, f32p, C.c_float, f32p]
         L.or_pixel_active.restype = C.c_int
         L.or_pixel_active.argtypes = [f64p, C.c_uint32, C.c_float]
+        L.or_gi_reconnect.restype = C.c_int
+        L.or_gi_reconnect.argtypes = [f32p] * 6
         _lib = L
     return _lib
 
@@ -159,3 +161,11 @@ def mean(state, samples: int) -> np.ndarray:
     else:
         inv = np.full(film.shape[0], 1.0 / max(samples, 1))
     return film * inv[:, None]
+
+
+def gi_reconnect(p, n, pq, x2, n2):
+    """ReSTIR GI's reconnection test (pt_core.slang:938-951): (accepted, Jacobian)."""
+    arrs = [np.ascontiguousarray(v, np.float32) for v in (p, n, pq, x2, n2)]
+    jac = np.zeros(1, np.float32)
+    ok = lib().or_gi_reconnect(*[_p(a, C.c_float) for a in arrs], _p(jac, C.c_float))
+    return bool(ok), float(jac[0])

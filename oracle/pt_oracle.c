@@ -1122,6 +1122,168 @@ static void restir_clear(const ctx_t* c, uint32_t pixel)
     C[27] = asf(0u);
 }
 
+/* ---------------------------------------------------------------- ReSTIR GI (pt_core.slang:901-980, 1566-1617) */
+
+#define RESTIR_MAX_JACOBIAN 10.0f
+
+/* restirScratch (pt_core.slang:713): the per-pixel words after the two frame halves */
+static inline uint32_t restir_scratch(uint32_t pixel) { return pixel * RESTIR_STRIDE + 2u * RESTIR_HALF; }
+
+/* The reconnection test of restirGIPrepare (pt_core.slang:938-951): the direction wi and distance
+   dR from the shading point p (normal n) to a neighbour's second hit x2 (normal n2), found from the
+   neighbour's shading point pq, and the Jacobian of the shift, the ratio of the solid angles that
+   an area element at x2 subtends at p and at pq: (cosR / cosQ) (dQ / dR)^2. Returns whether the
+   sample may be moved: it faces p, is not too close, and the Jacobian is within [1/10, 10]. */
+static int gi_reconnect(f3 p, f3 n, f3 pq, f3 x2, f3 n2, f3* wi, float* cosT, float* dR, float* jac)
+{
+    f3 toX = vsub(x2, p);
+    *dR = len3(toX);
+    *wi = vdivs(toX, fmax_(*dR, 1.0e-6f));
+    float dQ = len3(vsub(x2, pq));
+    *cosT = dot3(n, *wi);
+    float cosR = fabsf(dot3(n2, *wi));
+    float cosQ = fabsf(dot3(n2, normalize3(vsub(pq, x2))));
+    *jac = 0.0f;
+    if (*cosT <= 0.0f || *dR < 1.0e-3f || cosR < 1.0e-3f || cosQ < 1.0e-3f)
+        return 0;
+    *jac = (cosR / cosQ) * (dQ * dQ) / (*dR * *dR);
+    return !(*jac > RESTIR_MAX_JACOBIAN || *jac < 1.0f / RESTIR_MAX_JACOBIAN);
+}
+
+/* gi_reconnect for tests: p, n, pq, x2, n2 are 3 floats each; writes the Jacobian (0 when a cosine
+   or distance test fails) and returns the accept flag */
+int or_gi_reconnect(const float* p, const float* n, const float* pq, const float* x2, const float* n2, float* jac)
+{
+    f3 wi;
+    float cosT, dR;
+    return gi_reconnect(v3(p[0], p[1], p[2]), v3(n[0], n[1], n[2]), v3(pq[0], pq[1], pq[2]), v3(x2[0], x2[1], x2[2]),
+                        v3(n2[0], n2[1], n2[2]), &wi, &cosT, &dR, jac);
+}
+
+/* restirGIPrepare (pt_core.slang:901-972): the previous frame's second-hit reservoirs of this pixel
+   and two random neighbours, reconnected to this shading point with the solid-angle Jacobian
+   (cosR / cosQ) (dQ / dR)^2, kept within [1/10, 10]; the resampled pick goes to the scratch words */
+static void restir_gi_prepare(const ctx_t* c, uint32_t pixel, f3 p, f3 n, float pHitT, const surface_t* surf, f3 T, f3 B, f3 woL, uint32_t* rng)
+{
+    uint32_t w = c->P[0];
+    uint32_t cur = c->P[6] & 1u;
+    float cap = (float)(c->P[38] > 1u ? c->P[38] : 1u);
+    float wsum = 0.0f, mTotal = 0.0f, pSel = 0.0f;
+    f3 cSel = v3(0.0f, 0.0f, 0.0f), xSel = cSel, nSel = cSel, lSel = cSel;
+    if (c->P[6] > 0) {
+        int px = (int)(pixel % w), py = (int)(pixel / w);
+        for (uint32_t k = 0; k < 1 + RESTIR_NEIGHBOURS; k++) {
+            int nx = px, ny = py;
+            if (k > 0) {
+                nx += (int)(rng_next(rng) * (float)(2 * RESTIR_RADIUS + 1)) - RESTIR_RADIUS;
+                ny += (int)(rng_next(rng) * (float)(2 * RESTIR_RADIUS + 1)) - RESTIR_RADIUS;
+                if (nx < 0 || ny < 0 || nx >= (int)w || ny >= (int)c->P[1])
+                    continue;
+            }
+            const float* R = c->restir + restir_base((uint32_t)ny * w + (uint32_t)nx, cur ^ 1u);
+            if (asu(R[14]) == 0 || asu(R[27]) == 0)
+                continue;
+            f3 pq = v3(R[8], R[9], R[10]);
+            f3 nq = v3(R[11], R[12], R[13]);
+            if (dot3(nq, n) < 0.9f || len3(vsub(pq, p)) > 0.05f * fmax_(pHitT, 0.01f))
+                continue;
+            f3 x2 = v3(R[16], R[17], R[18]);
+            f3 n2 = v3(R[19], R[20], R[21]);
+            f3 l2 = v3(R[22], R[23], R[24]);
+            float mq = fmin_(R[26], cap);
+            mTotal += mq;
+            f3 wi;
+            float cosT, dR, jac;
+            if (!gi_reconnect(p, n, pq, x2, n2, &wi, &cosT, &dR, &jac))
+                continue;
+            f3 cc = vmul(eval_reflective(c, surf, woL, v3(dot3(wi, T), dot3(wi, B), cosT)), l2);
+            float phat = lum(cc);
+            if (phat <= 0.0f || occluded(c, p, wi, dR * (1.0f - 1.0e-3f)))
+                continue;
+            float wgt = phat * R[25] * mq * jac;
+            wsum += wgt;
+            if (rng_next(rng) * wsum < wgt) {
+                pSel = phat;
+                cSel = cc;
+                xSel = x2;
+                nSel = n2;
+                lSel = l2;
+            }
+        }
+    }
+    float* S = c->restir + restir_scratch(pixel);
+    S[0] = wsum;
+    S[1] = mTotal;
+    S[2] = pSel;
+    float cv[3] = {cSel.x, cSel.y, cSel.z}, xv[3] = {xSel.x, xSel.y, xSel.z};
+    float nv[3] = {nSel.x, nSel.y, nSel.z}, lv[3] = {lSel.x, lSel.y, lSel.z};
+    for (uint32_t k = 0; k < 3; k++) {
+        S[3 + k] = cv[k];
+        S[6 + k] = xv[k];
+        S[9 + k] = nv[k];
+        S[12 + k] = lv[k];
+    }
+    S[22] = asf(0u);
+}
+
+/* giFinish (pt_core.slang:1566-1617): the path's own candidate (its first-bounce surface, carrying
+   what the rest of the path gathered, divided by the first bounce's throughput) joins the
+   resampled neighbours; the pixel gets what the first hit gathered itself plus the pick's indirect
+   light times its weight W = wsum / (M pSel); the new reservoir goes to this frame's half. The
+   candidate stream hashes the pixel with the sample index (and the render's seed mix, zero for
+   the reference's own seed) */
+static f3 gi_finish(const ctx_t* c, f3 pathTotal, uint32_t pixel)
+{
+    float* C = c->restir + restir_base(pixel, c->P[6] & 1u);
+    float* S = c->restir + restir_scratch(pixel);
+    float wsum = S[0];
+    float m = S[1] + 1.0f;
+    float pSel = S[2];
+    f3 cSel = v3(S[3], S[4], S[5]);
+    f3 xSel = v3(S[6], S[7], S[8]);
+    f3 nSel = v3(S[9], S[10], S[11]);
+    f3 lSel = v3(S[12], S[13], S[14]);
+    f3 thr1 = v3(S[15], S[16], S[17]);
+    f3 d0 = v3(S[18], S[19], S[20]);
+    float qPrev = S[21];
+    int own = asu(S[22]) != 0 && qPrev > 0.0f;
+    f3 giTotal = vsub(pathTotal, d0);
+    f3 result = d0;
+    if (own) {
+        f3 cOwn = vscale(giTotal, qPrev);
+        float pOwn = lum(cOwn);
+        if (pOwn > 0.0f) {
+            float wOwn = pOwn / qPrev;
+            wsum += wOwn;
+            uint32_t r = or_pcg(pixel ^ or_pcg(c->P[6] * 0x85EBCA6Bu + 0x1234u) ^ c->seed_mix);
+            if (rng_next(&r) * wsum < wOwn) {
+                pSel = pOwn;
+                cSel = cOwn;
+                xSel = v3(C[16], C[17], C[18]);
+                nSel = v3(C[19], C[20], C[21]);
+                lSel = v3(thr1.x > 1.0e-6f ? giTotal.x / thr1.x : 0.0f, thr1.y > 1.0e-6f ? giTotal.y / thr1.y : 0.0f,
+                          thr1.z > 1.0e-6f ? giTotal.z / thr1.z : 0.0f);
+            }
+        }
+    } else
+        result = vadd(result, giTotal);
+    float W = pSel > 0.0f ? wsum / (m * pSel) : 0.0f;
+    result = vadd(result, vscale(cSel, W));
+    C[16] = xSel.x;
+    C[17] = xSel.y;
+    C[18] = xSel.z;
+    C[19] = nSel.x;
+    C[20] = nSel.y;
+    C[21] = nSel.z;
+    C[22] = lSel.x;
+    C[23] = lSel.y;
+    C[24] = lSel.z;
+    C[25] = W;
+    C[26] = m;
+    C[27] = asf(pSel > 0.0f ? 1u : 0u);
+    return result;
+}
+
 /* ---------------------------------------------------------------- shadePath (pt_core.slang:992-1543) */
 
 static int shade_path(const ctx_t* c, path_t* st, or_hit hq, uint32_t depth, uint32_t pixel)
@@ -1186,6 +1348,17 @@ static int shade_path(const ctx_t* c, path_t* st, or_hit hq, uint32_t depth, uin
             break;
         }
         f3 p = vadd(hit, vscale(ng, 1.0e-4f));
+        if (depth == 1 && st->gi) {
+            /* the first bounce reached a surface: ReSTIR GI's candidate point and facing normal (pt_core.slang:1072-1082) */
+            float* C = c->restir + restir_base(pixel, c->P[6] & 1u);
+            C[16] = hit.x;
+            C[17] = hit.y;
+            C[18] = hit.z;
+            C[19] = ng.x;
+            C[20] = ng.y;
+            C[21] = ng.z;
+            c->restir[restir_scratch(pixel) + 22] = asf(1u);
+        }
 
         f3 nsObj = vadd(vadd(sscale(1.0f - b1 - b2, vertex_normal(c, i0)), sscale(b1, vertex_normal(c, i1))), sscale(b2, vertex_normal(c, i2)));
         f3 ns = nOut;
@@ -1254,6 +1427,14 @@ static int shade_path(const ctx_t* c, path_t* st, or_hit hq, uint32_t depth, uin
                 wB = st->qPrev * st->qPrev / (st->qPrev * st->qPrev + pl * pl);
             }
             parts_add(&st->parts, depth, cls, vscale(vmul(thr, le), wB));
+            if (depth == 1 && st->gi) {
+                /* emission seen by the first bounce stays with the first hit's own light (pt_core.slang:1162-1168) */
+                float* S = c->restir + restir_scratch(pixel) + 18;
+                f3 a = vscale(vmul(thr, le), wB);
+                S[0] += a.x;
+                S[1] += a.y;
+                S[2] += a.z;
+            }
         }
         float metallic = M[6] * texRMy;
         float rough = M[7] * texRMx;
@@ -1268,6 +1449,9 @@ static int shade_path(const ctx_t* c, path_t* st, or_hit hq, uint32_t depth, uin
             }
             ior = fmax_(ior + (float)(chan - 1) * (ior - 1.0f) * 0.025f * dispersion, 1.0f);
         }
+        /* a glass second hit is no reconnection candidate (pt_core.slang:1187-1190) */
+        if (depth == 1 && st->gi && transmission > 0.0f)
+            c->restir[restir_scratch(pixel) + 22] = asf(0u);
         float alpha = fmax_(rough * rough, 1.0e-3f);
         float eta = outside ? ior : 1.0f / ior;
 
@@ -1352,11 +1536,14 @@ static int shade_path(const ctx_t* c, path_t* st, or_hit hq, uint32_t depth, uin
             int suits = PF(3) == 0.0f && transmission == 0.0f && surf.alpha >= RESTIR_MIN_ALPHA
                 && (surf.coat == 0.0f || surf.coatAlpha >= RESTIR_MIN_ALPHA);
             reDI = suits && (c->P[36] & 1u) != 0 && emitterCount > 0;
+            int reGI = suits && (c->P[36] & 2u) != 0 && !lightingOn;
             st->restir = reDI;
-            st->gi = 0;
-            if (suits)
+            st->gi = reGI;
+            if (suits) {
                 restir_gbuffer(c, pixel, p, n);
-            else
+                if (reGI)
+                    restir_gi_prepare(c, pixel, p, n, pHitT, &surf, T, B, woL, &rng);
+            } else
                 restir_clear(c, pixel);
         }
         if (reDI) {
@@ -1554,6 +1741,17 @@ static int shade_path(const ctx_t* c, path_t* st, or_hit hq, uint32_t depth, uin
     st->thr = thr;
     st->cls = cls;
     st->chan = chan;
+    if (depth == 0 && st->gi) {
+        /* the first hit's own gathered light and the first bounce's throughput and pdf (pt_core.slang:1531-1540) */
+        float* S = c->restir + restir_scratch(pixel);
+        S[15] = thr.x;
+        S[16] = thr.y;
+        S[17] = thr.z;
+        S[18] = st->parts.total.x;
+        S[19] = st->parts.total.y;
+        S[20] = st->parts.total.z;
+        S[21] = st->qPrev;
+    }
     return alive;
 }
 
@@ -1574,6 +1772,12 @@ static void finish_path(const ctx_t* c, const path_t* st, uint32_t pixel)
     parts_t parts = st->parts;
     float visLit = st->visLit, visTotal = st->visTotal;
     f3 radiance = parts.total;
+    if (st->gi) {
+        /* a path that went through dispersive glass carries one tripled channel: not shared (pt_core.slang:1626-1632) */
+        if (st->chan >= 0)
+            c->restir[restir_scratch(pixel) + 22] = asf(0u);
+        radiance = gi_finish(c, radiance, pixel);
+    }
     if (bad3(radiance)) {
         radiance = v3(0.0f, 0.0f, 0.0f);
         memset(&parts, 0, sizeof(parts));
@@ -1635,7 +1839,7 @@ int or_render(const or_scene* s, const uint32_t* params, uint32_t seed, uint32_t
 {
     uint32_t P[43];
     memcpy(P, params, sizeof(P));
-    if ((P[32] && !stats) || (P[28] && !lighting) || (P[36] && !restir) || (P[36] & 2u))
+    if ((P[32] && !stats) || (P[28] && !lighting) || (P[36] && !restir))
         return 1;
     for (uint32_t k = 0; k < samples; k++) {
         P[6] = first_sample + k;

@@ -9,7 +9,7 @@ import pytest
 import oracle
 from helpers import params
 from verion_amd import scenes
-from verion_amd.backend import (INVALID_ARGUMENT, MEGAKERNEL, SCENE, UNSUPPORTED, WAVEFRONT, Adaptive, BackendError,
+from verion_amd.backend import (INVALID_ARGUMENT, MEGAKERNEL, SCENE, WAVEFRONT, Adaptive, BackendError,
                                 Renderer, Restir)
 
 pytestmark = pytest.mark.gpu
@@ -175,13 +175,6 @@ def test_restir_switch_mid_accumulation_is_an_error(many):
         r.accumulate(acc, 1)
     r.set_restir(None)
     assert e.value.status == SCENE
-
-
-def test_restir_gi_is_unsupported(many):
-    r, _ = many
-    with pytest.raises(BackendError) as e:
-        r.set_restir(Restir(gi=True))
-    assert e.value.status == UNSUPPORTED
 
 
 def test_progressive_accumulation_equals_one_shot(ctx):

@@ -1,0 +1,117 @@
+"""ReSTIR GI on the GPU: the film against the oracle (1e-4 relative per value), and the reference's
+own GI tests (crates/verion-gpu-vulkan/tests/restir.rs:233-356) restated on their scenes: the
+closed room of rough walls (verion_amd.scenes.closed_room) and the Cornell box of
+scenes/cornell_box.gltf with its glass set to ior 2.4 and dispersion 20."""
+import dataclasses
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import params
+from verion_amd import gltf, scenes
+from verion_amd.backend import MEGAKERNEL, WAVEFRONT, Renderer, Restir
+
+pytestmark = pytest.mark.gpu
+TOL = 1.0e-4
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+GI = Restir(gi=True)  # restir.rs:233-238: the defaults (DI on, 4 candidates, history 4) plus GI
+
+
+def rel(got, want):
+    got, want = np.asarray(got, np.float64), np.asarray(want, np.float64)
+    return float((np.abs(got - want) / np.maximum(np.abs(want), 1.0e-6)).max())
+
+
+def render(r, spp, restir=None, integrator=MEGAKERNEL):
+    """restir.rs:78-82: a 64 x 64 accumulation of spp samples, its mean."""
+    r.set_integrator(integrator)
+    r.set_restir(restir)
+    acc = r.new_accumulation(64, 64)
+    r.accumulate(acc, spp)
+    r.set_restir(None)
+    r.set_integrator(MEGAKERNEL)
+    return acc.mean()
+
+
+def worst_block(a, b):
+    """restir.rs:273-293: largest relative difference of the 8x8 block sums, per channel."""
+    A = np.asarray(a, np.float64).reshape(8, 8, 8, 8, 3).sum(axis=(1, 3))
+    B = np.asarray(b, np.float64).reshape(8, 8, 8, 8, 3).sum(axis=(1, 3))
+    return float((np.abs(A - B) / np.maximum(B, 1e-3)).max())
+
+
+def worst_chroma(a, b):
+    """restir.rs:306-328: largest difference of the red-blue balance of the 8x8 block sums."""
+    A = np.asarray(a, np.float64).reshape(8, 8, 8, 8, 3).sum(axis=(1, 3))
+    B = np.asarray(b, np.float64).reshape(8, 8, 8, 8, 3).sum(axis=(1, 3))
+    bal = lambda v: (v[..., 0] - v[..., 2]) / np.maximum(v[..., 0] + v[..., 2], 1e-4)
+    return float(np.abs(bal(A) - bal(B)).max())
+
+
+@pytest.fixture(scope="module")
+def room(ctx):
+    r = Renderer(ctx, scenes.closed_room(0.9))
+    return r, oracle.OracleScene(r.packed)
+
+
+@pytest.mark.parametrize("integrator", [MEGAKERNEL, WAVEFRONT])
+@pytest.mark.parametrize("di", [True, False])
+def test_gi_film_matches_oracle(room, integrator, di):
+    r, o = room
+    r.set_integrator(integrator)
+    r.set_restir(Restir(di=di, gi=True, candidates=4, history=4))
+    acc = r.new_accumulation(48, 36)
+    for _ in range(5):  # frames: each reuses the previous frame's second-hit reservoirs
+        r.accumulate(acc, 1)
+    st = o.render(params(r.packed, 48, 36, restir=(di, True, 4, 4)), 5)
+    r.set_restir(None)
+    r.set_integrator(MEGAKERNEL)
+    got = acc.film()
+    assert rel(got, st["film"]) <= TOL, f"exact {np.mean(got == st['film']):.4f}"
+
+
+def test_gi_draws_the_same_image_with_both_integrators(room):
+    # restir.rs:252-261
+    r, _ = room
+    mega = render(r, 16, GI, MEGAKERNEL)
+    wave = render(r, 16, GI, WAVEFRONT)
+    assert np.array_equal(mega.view(np.uint32), wave.view(np.uint32))
+
+
+def test_gi_converges_to_the_path_traced_image(room):
+    # restir.rs:240-250
+    r, _ = room
+    plain = render(r, 1024)[:, 1].mean()
+    resampled = render(r, 1024, GI)[:, 1].mean()
+    err = abs(resampled - plain) / plain
+    assert err < 0.03, f"plain {plain}, gi {resampled}, error {err}"
+
+
+def test_gi_leaves_sharp_surfaces_alone(ctx):
+    # restir.rs:263-271
+    r = Renderer(ctx, scenes.closed_room(0.05))
+    plain = render(r, 8)
+    assert np.array_equal(plain.view(np.uint32), render(r, 8, GI).view(np.uint32))
+
+
+def test_gi_leaves_no_stains_behind_dispersive_glass(ctx):
+    # restir.rs:295-305
+    r = Renderer(ctx, scenes.closed_room(0.9, glass=True))
+    plain = render(r, 2048)
+    resampled = render(r, 2048, GI)
+    worst = worst_block(resampled, plain)
+    assert worst < 0.15, f"worst block differs by {worst}"
+
+
+def test_gi_leaves_no_stains_on_the_cornell_box_walls(ctx):
+    # restir.rs:330-356: the glass sphere at ior 2.4 and dispersion 20
+    s, _ = gltf.load_gltf(os.path.join(GOLDEN, "cornell_box.gltf"))
+    s.materials[5] = dataclasses.replace(s.materials[5], ior=2.4, dispersion=20.0)
+    r = Renderer(ctx, s)
+    reference = render(r, 4096)
+    img = render(r, 256, GI)
+    block, chroma = worst_block(img, reference), worst_chroma(img, reference)
+    assert block < 0.3, f"worst block differs by {block}"
+    assert chroma < 0.15, f"worst color balance differs by {chroma}"

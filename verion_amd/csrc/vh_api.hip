@@ -411,8 +411,6 @@ int vh_set_restir(vh_renderer* r, const vh_restir* restir)
 {
     if (!r)
         return fail(VH_ERR_INVALID_ARGUMENT, "null renderer");
-    if (restir && restir->gi)
-        return fail(VH_ERR_UNSUPPORTED, "ReSTIR GI is not built in this backend");
     r->restirOn = restir != nullptr;
     if (restir)
         r->restir = *restir;

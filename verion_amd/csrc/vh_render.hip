@@ -50,7 +50,7 @@ __global__ __launch_bounds__(64) void k_pathtrace(Scene S, Frame F, Params P, ui
             if (!shadePath<RESTIR, SPLIT>(c, st, h, depth, pixel))
                 break;
         }
-        finishPath<SPLIT>(c, st, pixel);
+        finishPath<SPLIT, RESTIR>(c, st, pixel);
     }
 }
 
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(kWfBlock, VH_SHADE_WAVES) void k_wf_shade(Scene S, 
             h.gid = 0;
             keep = shadePath<RESTIR, SPLIT>(c, st, h, d, pixel) && d + 1 < P.w[7];
             if (!keep)
-                finishPath<SPLIT>(c, st, pixel);
+                finishPath<SPLIT, RESTIR>(c, st, pixel);
         }
         uint32_t slot = compact(keep, &W.counts[d + 1]);
         if (keep && slot < W.n) // survivors of depth d never outnumber its count (<= n)

@@ -1,13 +1,13 @@
-// vh_shade.h - one path bounce on the GPU: camera rays, BSDF, light sampling, ReSTIR DI, film.
+// vh_shade.h - one path bounce on the GPU: camera rays, BSDF, light sampling, ReSTIR DI and GI, film.
 //
 // The device form of shaders/include/pt_core.slang: generatePathFor (:645-682), the BSDF
-// (:115-557), occluded (:580-613), ReSTIR DI (:702-995), shadePath (:992-1543), pixelActive
-// (:1551-1559), finishPath (:1620-1667). Expressions keep the shader's evaluation order (the
+// (:115-557), occluded (:580-613), ReSTIR DI (:702-900), ReSTIR GI (:901-980, 1566-1617),
+// shadePath (:992-1543), pixelActive (:1551-1559), finishPath (:1620-1667). Expressions keep the shader's evaluation order (the
 // library is built with contraction off), so the megakernel and the wavefront kernels, which both
 // call these functions, give identical bits, and the CPU oracle reproduces them.
 // Deviations shared with the oracle: FP64 film and adaptive statistics; closest hits with ties to
 // the smallest (instance, primitive); closest-hit shadow rays when a material is translucent;
-// ReSTIR GI is not built (its flag is rejected on the host).
+// ReSTIR GI's own-candidate draw hashes in the render's seed mix (zero for the reference's stream).
 #pragma once
 
 #include "vh_trace.h"
@@ -27,6 +27,7 @@ constexpr int kRestirRadius = 6;
 constexpr uint32_t kRestirNeighbours = 2;
 constexpr uint32_t kRestirHalf = 28;
 constexpr uint32_t kRestirStride = 80;
+constexpr float kRestirMaxJacobian = 10.0f;
 
 // Scene tables in device memory (the bindings of pt_core.slang:12-49, minus the RT structure).
 struct Scene {
@@ -712,6 +713,156 @@ __device__ __forceinline__ void restirClear(const Ctx& c, uint32_t pixel)
     C[27] = asf(0u);
 }
 
+// ---------------------------------------------------------------- ReSTIR GI (pt_core.slang:901-980, 1566-1617)
+//
+// Per pixel, each frame half holds after the DI words a second-hit reservoir: 16..18 the point x2,
+// 19..21 its facing normal, 22..24 the radiance it sends back (per unit throughput), 25 W, 26 M,
+// 27 valid. The 24 scratch words after the two halves carry the frame's resampled pick from the
+// first hit to the end of the path: 0 wsum, 1 M, 2 pSel, 3..5 its contribution, 6..8 x2,
+// 9..11 n2, 12..14 radiance, 15..17 the first bounce's throughput, 18..20 what the first hit
+// gathered itself, 21 the first bounce's pdf, 22 "the first bounce reached a reusable surface".
+
+__device__ __forceinline__ uint32_t restirScratch(uint32_t pixel) { return pixel * kRestirStride + 2 * kRestirHalf; }
+
+// restirGIPrepare (pt_core.slang:901-972): the previous frame's reservoirs of this pixel and two
+// random neighbours, reconnected to this shading point; a neighbour's sample is weighed by the
+// change of solid angle (cosR / cosQ) (dQ / dR)^2, which must stay within [1/10, 10], and must be
+// visible from the point.
+__device__ void restirGIPrepare(const Ctx& c, uint32_t pixel, V3 p, V3 n, float pHitT, const Surface& surf, V3 T, V3 B, V3 woL, uint32_t& rng)
+{
+    const Params& P = c.P;
+    uint32_t w = P.w[0];
+    uint32_t cur = P.w[6] & 1u;
+    float cap = (float)max(P.w[38], 1u);
+    float wsum = 0.0f, mTotal = 0.0f, pSel = 0.0f;
+    V3 cSel = v3(0.0f), xSel = v3(0.0f), nSel = v3(0.0f), lSel = v3(0.0f);
+    if (P.w[6] > 0) {
+        int px = (int)(pixel % w), py = (int)(pixel / w);
+        for (uint32_t k = 0; k < 1 + kRestirNeighbours; k++) {
+            int nx = px, ny = py;
+            if (k > 0) {
+                nx += (int)(rnext(rng) * (float)(2 * kRestirRadius + 1)) - kRestirRadius;
+                ny += (int)(rnext(rng) * (float)(2 * kRestirRadius + 1)) - kRestirRadius;
+                if (nx < 0 || ny < 0 || nx >= (int)w || ny >= (int)P.w[1])
+                    continue;
+            }
+            const float* R = c.F.restir + restirBase((uint32_t)ny * w + (uint32_t)nx, cur ^ 1u);
+            if (asu(R[14]) == 0 || asu(R[27]) == 0)
+                continue;
+            V3 pq = v3(R[8], R[9], R[10]);
+            V3 nq = v3(R[11], R[12], R[13]);
+            if (dot(nq, n) < 0.9f || length(pq - p) > 0.05f * fmaxf(pHitT, 0.01f))
+                continue;
+            V3 x2 = v3(R[16], R[17], R[18]);
+            V3 n2 = v3(R[19], R[20], R[21]);
+            V3 l2 = v3(R[22], R[23], R[24]);
+            float mq = fminf(R[26], cap);
+            mTotal += mq;
+            V3 toX = x2 - p;
+            float dR = length(toX);
+            V3 wi = toX / fmaxf(dR, 1.0e-6f);
+            float dQ = length(x2 - pq);
+            float cosT = dot(n, wi);
+            float cosR = fabsf(dot(n2, wi));
+            float cosQ = fabsf(dot(n2, normalize(pq - x2)));
+            if (cosT <= 0.0f || dR < 1.0e-3f || cosR < 1.0e-3f || cosQ < 1.0e-3f)
+                continue;
+            float jac = (cosR / cosQ) * (dQ * dQ) / (dR * dR);
+            if (jac > kRestirMaxJacobian || jac < 1.0f / kRestirMaxJacobian)
+                continue;
+            V3 cc = evalReflective(c, surf, woL, v3(dot(wi, T), dot(wi, B), cosT)) * l2;
+            float phat = lum(cc);
+            if (phat <= 0.0f || occluded(c, p, wi, dR * (1.0f - 1.0e-3f)))
+                continue;
+            float wgt = phat * R[25] * mq * jac;
+            wsum += wgt;
+            if (rnext(rng) * wsum < wgt) {
+                pSel = phat;
+                cSel = cc;
+                xSel = x2;
+                nSel = n2;
+                lSel = l2;
+            }
+        }
+    }
+    float* Sc = c.F.restir + restirScratch(pixel);
+    Sc[0] = wsum;
+    Sc[1] = mTotal;
+    Sc[2] = pSel;
+    Sc[3] = cSel.x;
+    Sc[4] = cSel.y;
+    Sc[5] = cSel.z;
+    Sc[6] = xSel.x;
+    Sc[7] = xSel.y;
+    Sc[8] = xSel.z;
+    Sc[9] = nSel.x;
+    Sc[10] = nSel.y;
+    Sc[11] = nSel.z;
+    Sc[12] = lSel.x;
+    Sc[13] = lSel.y;
+    Sc[14] = lSel.z;
+    Sc[22] = asf(0u);
+}
+
+// giFinish (pt_core.slang:1566-1617): the path's own candidate (the surface its first bounce
+// reached, with what the rest of the path gathered divided by the first bounce's throughput) joins
+// the resampled ones; the pixel gets what the first hit gathered itself plus the pick's indirect
+// light times W = wsum / (M pSel), and the new reservoir goes to this frame's half. The
+// candidate's random number hashes the pixel with the sample index and the render's seed mix
+// (zero for the reference's own stream).
+__device__ V3 giFinish(const Ctx& c, V3 pathTotal, uint32_t pixel)
+{
+    float* C = c.F.restir + restirBase(pixel, c.P.w[6] & 1u);
+    const float* Sc = c.F.restir + restirScratch(pixel);
+    float wsum = Sc[0];
+    float m = Sc[1] + 1.0f;
+    float pSel = Sc[2];
+    V3 cSel = v3(Sc[3], Sc[4], Sc[5]);
+    V3 xSel = v3(Sc[6], Sc[7], Sc[8]);
+    V3 nSel = v3(Sc[9], Sc[10], Sc[11]);
+    V3 lSel = v3(Sc[12], Sc[13], Sc[14]);
+    V3 thr1 = v3(Sc[15], Sc[16], Sc[17]);
+    V3 d0 = v3(Sc[18], Sc[19], Sc[20]);
+    float qPrev = Sc[21];
+    bool own = asu(Sc[22]) != 0 && qPrev > 0.0f;
+    V3 giTotal = pathTotal - d0;
+    V3 result = d0;
+    if (own) {
+        V3 cOwn = giTotal * qPrev;
+        float pOwn = lum(cOwn);
+        if (pOwn > 0.0f) {
+            float wOwn = pOwn / qPrev;
+            wsum += wOwn;
+            uint32_t r = pcg(pixel ^ pcg(c.P.w[6] * 0x85EBCA6Bu + 0x1234u) ^ c.P.seedMix);
+            if (rnext(r) * wsum < wOwn) {
+                pSel = pOwn;
+                cSel = cOwn;
+                xSel = v3(C[16], C[17], C[18]);
+                nSel = v3(C[19], C[20], C[21]);
+                lSel = v3(thr1.x > 1.0e-6f ? giTotal.x / thr1.x : 0.0f, thr1.y > 1.0e-6f ? giTotal.y / thr1.y : 0.0f,
+                          thr1.z > 1.0e-6f ? giTotal.z / thr1.z : 0.0f);
+            }
+        }
+    } else {
+        result = result + giTotal;
+    }
+    float W = pSel > 0.0f ? wsum / (m * pSel) : 0.0f;
+    result = result + cSel * W;
+    C[16] = xSel.x;
+    C[17] = xSel.y;
+    C[18] = xSel.z;
+    C[19] = nSel.x;
+    C[20] = nSel.y;
+    C[21] = nSel.z;
+    C[22] = lSel.x;
+    C[23] = lSel.y;
+    C[24] = lSel.z;
+    C[25] = W;
+    C[26] = m;
+    C[27] = asf(pSel > 0.0f ? 1u : 0u);
+    return result;
+}
+
 // ---------------------------------------------------------------- neural radiance cache hook
 
 // What shadePath exchanges with the radiance cache. NRC = 1 (training paths): every opaque vertex
@@ -825,6 +976,17 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
             break;
         }
         V3 p = hit + ng * 1.0e-4f;
+        if (RESTIR && depth == 1 && st.gi) {
+            // the first bounce reached a surface: ReSTIR GI's candidate point and facing normal (pt_core.slang:1072-1082)
+            float* C = c.F.restir + restirBase(pixel, P.w[6] & 1u);
+            C[16] = hit.x;
+            C[17] = hit.y;
+            C[18] = hit.z;
+            C[19] = ng.x;
+            C[20] = ng.y;
+            C[21] = ng.z;
+            c.F.restir[restirScratch(pixel) + 22] = asf(1u);
+        }
 
         V3 nsObj = (1.0f - b1 - b2) * vtx(S.vnormals, i0) + b1 * vtx(S.vnormals, i1) + b2 * vtx(S.vnormals, i2);
         V3 ns = nOut;
@@ -891,6 +1053,14 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
                 wB = st.qPrev * st.qPrev / (st.qPrev * st.qPrev + pl * pl);
             }
             partsAdd(st.parts, depth, cls, thr * le * wB, SPLIT);
+            if (RESTIR && depth == 1 && st.gi) {
+                // emission the first bounce sees stays with the first hit's own light (pt_core.slang:1162-1168)
+                float* Sd = c.F.restir + restirScratch(pixel) + 18;
+                V3 a = thr * le * wB;
+                Sd[0] += a.x;
+                Sd[1] += a.y;
+                Sd[2] += a.z;
+            }
         }
         if constexpr (NRC == 1) {
             if (depth < kMaxDepthNrc) {
@@ -928,6 +1098,9 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
             }
             ior = fmaxf(ior + (float)(chan - 1) * (ior - 1.0f) * 0.025f * dispersion, 1.0f);
         }
+        // a glass second hit is no reconnection candidate (pt_core.slang:1187-1190)
+        if (RESTIR && depth == 1 && st.gi && transmission > 0.0f)
+            c.F.restir[restirScratch(pixel) + 22] = asf(0u);
         float alpha = fmaxf(rough * rough, 1.0e-3f);
         float eta = outside ? ior : 1.0f / ior;
 
@@ -1016,11 +1189,15 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
             bool suits = P.f(3) == 0.0f && transmission == 0.0f && surf.alpha >= kRestirMinAlpha &&
                          (surf.coat == 0.0f || surf.coatAlpha >= kRestirMinAlpha);
             reDI = suits && (P.w[36] & 1u) != 0 && emitterCount > 0;
+            // ReSTIR GI replaces the path's own indirect light, which the lighting split cannot follow
+            bool reGI = suits && (P.w[36] & 2u) != 0 && !lightingOn;
             st.restir = reDI;
-            st.gi = false;
-            if (suits)
+            st.gi = reGI;
+            if (suits) {
                 restirGbuffer(c, pixel, p, n);
-            else
+                if (reGI)
+                    restirGIPrepare(c, pixel, p, n, pHitT, surf, T, B, woL, rng);
+            } else
                 restirClear(c, pixel);
         }
         if (RESTIR && reDI) {
@@ -1228,6 +1405,17 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
     st.thr = thr;
     st.cls = cls;
     st.chan = chan;
+    if (RESTIR && depth == 0 && st.gi) {
+        // the first hit's own gathered light, the first bounce's throughput and pdf (pt_core.slang:1531-1540)
+        float* Sc = c.F.restir + restirScratch(pixel);
+        Sc[15] = thr.x;
+        Sc[16] = thr.y;
+        Sc[17] = thr.z;
+        Sc[18] = st.parts.total.x;
+        Sc[19] = st.parts.total.y;
+        Sc[20] = st.parts.total.z;
+        Sc[21] = st.qPrev;
+    }
     return alive;
 }
 
@@ -1244,12 +1432,18 @@ __device__ __forceinline__ bool pixelActive(const Ctx& c, uint32_t p)
 }
 
 // finishPath (pt_core.slang:1620-1667), FP64 film; only this path's thread touches the pixel
-template <bool SPLIT>
+template <bool SPLIT, bool RESTIR = false>
 __device__ void finishPath(const Ctx& c, const PathState& st, uint32_t pixel)
 {
     Parts parts = st.parts;
     float visLit = st.visLit, visTotal = st.visTotal;
     V3 radiance = parts.total;
+    if (RESTIR && !SPLIT && st.gi) {
+        // a path through dispersive glass carries one tripled channel: not shared (pt_core.slang:1626-1632)
+        if (st.chan >= 0)
+            c.F.restir[restirScratch(pixel) + 22] = asf(0u);
+        radiance = giFinish(c, radiance, pixel);
+    }
     if (bad(radiance)) {
         radiance = v3(0.0f);
         parts = Parts{v3(0.0f), v3(0.0f), v3(0.0f), v3(0.0f), v3(0.0f), v3(0.0f)};

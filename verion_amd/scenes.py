@@ -9,7 +9,8 @@ is measured on:
              metallic Bernoulli(0.3), one sun and a constant environment map (importance sampled);
   many_lights(): ground and box clutter with emissive triangles of lognormal emission.
 The closed-form scenes restate crates/verion-gpu-metal/tests/pathtrace.rs: `cube` (:38-58),
-`plane` (:61-70), `furnace_room` (:87-98), the white furnace (:118-137).
+`plane` (:61-70), `furnace_room` (:87-98), the white furnace (:118-137); `closed_room` restates the
+ReSTIR GI test room of crates/verion-gpu-vulkan/tests/restir.rs:152-231.
 """
 from __future__ import annotations
 
@@ -82,6 +83,35 @@ def sky_scene(mesh: sc.Mesh, material: sc.BaseBsdf, camera_z: float = 3.0, sky: 
     s.add_instance(sc.Instance(m, s.add_material(material)))
     s.add_light(sc.Environment((sky, sky, sky)))
     s.camera = camera_at_z(camera_z)
+    return s
+
+
+def closed_room(roughness: float, glass: bool = False, dispersion: float = 8.0) -> sc.Scene:
+    """A closed room of rough walls (a 3 m cube) lit by 16 small ceiling panels of radiance 30, with
+    a rough 0.8 m block in the middle, seen from inside at z = 1.4: most of what the camera sees is
+    indirect light (crates/verion-gpu-vulkan/tests/restir.rs:152-231). `glass` adds a strongly
+    dispersive glass block (ior 1.6, dispersion 8 unless given) of half size 0.35 at (-0.7, -0.9, 0)."""
+    s = sc.Scene()
+    room = s.add_mesh(cube_mesh(1.5))
+    ball = s.add_mesh(cube_mesh(0.4))
+    pos, idx = [], []
+    for k in range(16):
+        x, z, h = -0.6 + (k % 4) * 0.4, -0.6 + (k // 4) * 0.4, 0.06
+        b = len(pos)
+        pos += [[x - h, 1.45, z - h], [x + h, 1.45, z - h], [x + h, 1.45, z + h], [x - h, 1.45, z + h]]
+        idx += [b, b + 1, b + 2, b, b + 2, b + 3]
+    lamps = s.add_mesh(sc.Mesh(np.array(pos, np.float32), np.array(idx, np.uint32)))
+    walls = s.add_material(sc.BaseBsdf(base_color=(0.7, 0.65, 0.6), roughness=roughness, ior=1.0))
+    glow = s.add_material(sc.BaseBsdf(base_color=(0.0, 0.0, 0.0), emission=(30.0, 30.0, 30.0), roughness=roughness))
+    s.add_instance(sc.Instance(room, walls))
+    s.add_instance(sc.Instance(ball, walls))
+    s.add_instance(sc.Instance(lamps, glow))
+    if glass:
+        block = s.add_mesh(cube_mesh(0.35))
+        dispersive = s.add_material(sc.BaseBsdf(base_color=(1.0, 1.0, 1.0), transmission=1.0, roughness=0.0,
+                                                ior=1.6, dispersion=dispersion))
+        s.add_instance(sc.Instance(block, dispersive, ((1.0, 0.0, 0.0, -0.7), (0.0, 1.0, 0.0, -0.9), (0.0, 0.0, 1.0, 0.0))))
+    s.camera = camera_at_z(1.4)
     return s
 
 
