@@ -6,7 +6,8 @@
  *
  * It restates, function by function, shaders/include/pt_core.slang (bindings :12-49, pcg/Rng
  * :69-84, BSDF :115-557, occluded :580-613, generatePathFor :645-682, traceClosest :684-700,
- * ReSTIR DI :702-995, shadePath :992-1543, pixelActive :1551-1559, finishPath :1620-1667), the
+ * ReSTIR DI :702-900, ReSTIR GI :901-980 and :1566-1617, shadePath :992-1543, pixelActive
+ * :1551-1559, finishPath :1620-1667), the
  * megakernel loop of shaders/pathtrace.slang:9-25 and the primary-hit AOVs of shaders/aov.slang:84-188
  * and shaders/normals.slang:39-110, with these stated deviations (shared with the
  * GPU library):
@@ -17,12 +18,14 @@
  *     translucent material take the closest hit per layer (the reference's first-found hit is
  *     implementation-defined);
  *   - transcendental functions come from or_math.h (Cephes), division and sqrt are IEEE;
- *   - ReSTIR GI (pt_core.slang:897-980, 1566-1617) is not restated; its flags stay off.
+ *   - ReSTIR GI's own-candidate draw hashes in the render's seed mix (zero for seed 0).
  * Every float literal carries an f suffix: float arithmetic as in the shader, compiled with
  * -ffp-contract=off and -Werror=double-promotion.
  *
  * Parity pins: tests/test_oracle_*.py (closed forms of the reference's own GPU tests, the albedo
- * tables, brute force vs BVH, golden PCG vectors).
+ * tables, brute force vs BVH, golden PCG vectors, ReSTIR GI's solid-angle Jacobian, first-frame
+ * identity and convergence). Parity unpinned: ReSTIR GI's split of first-bounce emission into the
+ * first hit's own light (pt_core.slang:1162-1168) changes no tested image beyond its noise.
  */
 #include <math.h>
 #include <stdint.h>
