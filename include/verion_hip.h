@@ -110,12 +110,29 @@ typedef struct vh_memory_usage {
     uint64_t uploaded;
 } vh_memory_usage;
 
-/* UpdateReport (lib.rs:65-71). */
+/* UpdateReport (lib.rs:65-71). bvh_work has no counterpart in the reference: the LBVH is one tree
+ * over world-space triangles, and a TLAS or BLAS rebuild of the plan becomes a refit of it (or a
+ * rebuild when the refit's surface-area cost grows past 1.5 times the built tree's). */
 typedef struct vh_update_report {
-    uint64_t blases_built;
+    uint64_t blases_built;   /* meshes whose triangles were redone (every mesh for a full update) */
     uint32_t tlas_built;
     uint64_t bytes_uploaded;
+    uint32_t bvh_work;       /* 0: LBVH untouched, 1: refit in place, 2: built anew */
 } vh_update_report;
+
+/* pack::UpdatePlan (pack.rs:758-821): what a batch of scene changes makes the backend redo;
+ * UpdatePlan::new(&Changes) derives it (pack.rs:785-815). */
+typedef struct vh_update_plan {
+    uint32_t full;       /* meshes or instances added or re-pointed: everything, as at creation */
+    uint32_t materials;  /* material table; texels and texture info when they differ */
+    uint32_t lights;     /* light table (albedo tables, link masks, IES), sky, environment map when it differs */
+    uint32_t emitters;   /* emitter table and CDF */
+    uint32_t instances;  /* instance table */
+    uint32_t vertices;   /* positions, uvs and vertex normals rewritten at the same sizes */
+    uint32_t tlas;       /* instance transforms changed */
+    uint32_t camera;     /* the view */
+    const uint32_t* blases; uint64_t blas_count; /* meshes whose vertices changed */
+} vh_update_plan;
 
 typedef struct vh_context vh_context;
 typedef struct vh_renderer vh_renderer;
@@ -144,8 +161,12 @@ int vh_memory(const vh_context* ctx, vh_memory_usage* out);
  * VH_ERR_SCENE when the scene has no instances (pack.rs:727-729) or a table is inconsistent. */
 int vh_renderer_create(vh_context* ctx, const vh_scene_tables* tables, vh_renderer** out);
 void vh_renderer_destroy(vh_renderer* r);
-/* RayTracingBackend::update (lib.rs:27): replaces every table and rebuilds the LBVH. */
-int vh_renderer_update(vh_renderer* r, const vh_scene_tables* tables, vh_update_report* report);
+/* RayTracingBackend::update (lib.rs:27, vulkan renderer.rs:404-502): `tables` is the whole scene
+ * after the edit; the plan says what changed and only that is uploaded and redone. NULL (or
+ * plan->full) replaces every table and builds the LBVH anew. A partial plan must keep every table
+ * size but the material, light, emitter and texture tables (VH_ERR_INVALID_ARGUMENT otherwise).
+ * The call waits for the device first: no accumulation may be running. */
+int vh_renderer_update(vh_renderer* r, const vh_scene_tables* tables, const vh_update_plan* plan, vh_update_report* report);
 /* set_camera (lib.rs:30): the view of pack::View::new. */
 int vh_set_camera(vh_renderer* r, const vh_view* view);
 int vh_set_multiscatter(vh_renderer* r, int on);

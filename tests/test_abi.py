@@ -49,7 +49,7 @@ def test_struct_layouts_match_the_header(tmp_path):
     pairs = {"vh_scene_info": abi.SceneInfo, "vh_view": abi.ViewS, "vh_scene_tables": abi.SceneTables,
              "vh_adaptive": abi.Adaptive, "vh_restir": abi.Restir, "vh_memory_usage": abi.MemoryUsage,
              "vh_update_report": abi.UpdateReport, "vh_nrc_config": abi.NrcConfig, "vh_batch_job": abi.BatchJob,
-             "vh_batch_tuning": abi.BatchTuning}
+             "vh_batch_tuning": abi.BatchTuning, "vh_update_plan": abi.UpdatePlanS}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for cname, py in pairs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

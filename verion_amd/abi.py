@@ -60,7 +60,14 @@ class MemoryUsage(C.Structure):
 
 
 class UpdateReport(C.Structure):
-    _fields_ = [("blases_built", C.c_uint64), ("tlas_built", C.c_uint32), ("bytes_uploaded", C.c_uint64)]
+    _fields_ = [("blases_built", C.c_uint64), ("tlas_built", C.c_uint32), ("bytes_uploaded", C.c_uint64),
+                ("bvh_work", C.c_uint32)]
+
+
+class UpdatePlanS(C.Structure):
+    _fields_ = [("full", C.c_uint32), ("materials", C.c_uint32), ("lights", C.c_uint32), ("emitters", C.c_uint32),
+                ("instances", C.c_uint32), ("vertices", C.c_uint32), ("tlas", C.c_uint32), ("camera", C.c_uint32),
+                ("blases", C.POINTER(C.c_uint32)), ("blas_count", C.c_uint64)]
 
 
 def ptr(a: np.ndarray, ctype):

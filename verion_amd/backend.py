@@ -15,7 +15,7 @@ from typing import Optional
 import numpy as np
 
 from . import abi
-from .pack import Packed, View, pack
+from .pack import Packed, UpdatePlan, View, pack
 from .scene import Scene
 
 # the in-tree build; VERION_HIP_LIBRARY names another build of the same sources (e.g. one with
@@ -83,7 +83,7 @@ def library() -> C.CDLL:
         "vh_memory": (i32, [vp, C.POINTER(abi.MemoryUsage)]),
         "vh_renderer_create": (i32, [vp, C.POINTER(abi.SceneTables), C.POINTER(vp)]),
         "vh_renderer_destroy": (None, [vp]),
-        "vh_renderer_update": (i32, [vp, C.POINTER(abi.SceneTables), C.POINTER(abi.UpdateReport)]),
+        "vh_renderer_update": (i32, [vp, C.POINTER(abi.SceneTables), C.POINTER(abi.UpdatePlanS), C.POINTER(abi.UpdateReport)]),
         "vh_set_camera": (i32, [vp, C.POINTER(abi.ViewS)]),
         "vh_set_multiscatter": (i32, [vp, i32]),
         "vh_set_integrator": (i32, [vp, i32]),
@@ -284,11 +284,21 @@ class Renderer(_Owned):
         _check(library().vh_renderer_create(ctx.h, C.byref(self._tables), C.byref(h)))
         self._handle = _Handle(h, library().vh_renderer_destroy, ctx._handle)
 
-    def update(self, scene) -> abi.UpdateReport:
+    def update(self, scene, changes=None) -> abi.UpdateReport:
+        """RayTracingBackend::update (lib.rs:27): `changes` (a scene.Changes, from
+        Scene.take_changes) selects what is re-uploaded and redone through pack.UpdatePlan; None
+        replaces everything."""
         self.packed = scene if isinstance(scene, Packed) else pack(scene)
         self._tables, self._keep = abi.tables_struct(self.packed)
+        plan, blases = None, None
+        if changes is not None:
+            up = UpdatePlan.new(changes)
+            blases = np.array(up.blases, np.uint32)
+            plan = abi.UpdatePlanS(int(up.full), int(up.materials), int(up.lights), int(up.emitters), int(up.instances),
+                                   int(up.vertices), int(up.tlas), int(up.camera),
+                                   abi.ptr(blases, C.c_uint32) if blases.size else None, blases.size)
         rep = abi.UpdateReport()
-        _check(library().vh_renderer_update(self.h, C.byref(self._tables), C.byref(rep)))
+        _check(library().vh_renderer_update(self.h, C.byref(self._tables), C.byref(plan) if plan else None, C.byref(rep)))
         return rep
 
     def set_camera(self, view: View):

@@ -81,6 +81,43 @@ int upload(vh_context* ctx, DevBuf& b, const T* src, uint64_t n)
     return VH_OK;
 }
 
+// Rewrites a table in place when its size is unchanged (a partial update), else reallocates.
+template <class T>
+int reupload(vh_context* ctx, DevBuf& b, const T* src, uint64_t n)
+{
+    if (b.p && b.bytes == sizeof(T) * n && n) {
+        HIPTRY(hipMemcpyAsync(b.p, src, sizeof(T) * n, hipMemcpyHostToDevice, ctx->stream));
+        ctx->uploaded += sizeof(T) * n;
+        return VH_OK;
+    }
+    return upload(ctx, b, src, n);
+}
+
+// FNV-1a over 32-bit words (with the length), to tell whether a table changed
+uint64_t hash_words(uint64_t h, const void* p, uint64_t words)
+{
+    const uint32_t* w = static_cast<const uint32_t*>(p);
+    h = (h ^ words) * 0x100000001B3ull;
+    for (uint64_t i = 0; i < words; i++)
+        h = (h ^ w[i]) * 0x100000001B3ull;
+    return h;
+}
+
+uint64_t texture_hash(const vh_scene_tables* t)
+{
+    uint64_t h = hash_words(0xCBF29CE484222325ull, t->texels, t->texels_len);
+    return hash_words(h, t->tex_info, t->tex_info_len);
+}
+
+uint64_t env_hash(const vh_scene_tables* t)
+{
+    uint32_t dims[3] = {t->info.env_width, t->info.env_height, asu(t->info.env_pdf_scale)};
+    uint64_t h = hash_words(0xCBF29CE484222325ull, dims, 3);
+    h = hash_words(h, t->env_pixels, t->env_pixels_len);
+    h = hash_words(h, t->env_marginal, t->env_marginal_len);
+    return hash_words(h, t->env_cond, t->env_cond_len);
+}
+
 // Consistency of the tables against the layouts of pt_core.slang:12-49 (the kernels trust them).
 int validate(const vh_scene_tables* t, std::vector<uint32_t>& instStart, uint64_t& ntri, uint32_t& translucent)
 {
@@ -230,6 +267,10 @@ int upload_tables(vh_renderer* r, const vh_scene_tables* t)
     r->view = t->view;
     r->instanceCount = (uint32_t)t->instance_count;
     r->translucent = translucent;
+    uint64_t lens[6] = {t->positions_len, t->indices_len, t->uvs_len, t->vnormals_len, t->instances_len, t->mesh_count};
+    std::memcpy(r->geomLens, lens, sizeof(lens));
+    r->texHash = texture_hash(t);
+    r->envHash = env_hash(t);
     DeviceTables dt{r->positions.as<float>(), r->indices.as<uint32_t>(), r->instances.as<uint32_t>(), r->instStart.as<uint32_t>(),
                     r->instanceCount, (uint32_t)ntri};
     TRY(build_bvh(r->bvh, dt, c->stream));
@@ -349,17 +390,103 @@ void vh_renderer_destroy(vh_renderer* r)
     delete r;
 }
 
-int vh_renderer_update(vh_renderer* r, const vh_scene_tables* tables, vh_update_report* report)
+// Renderer::update (vulkan renderer.rs:404-502) for a partial plan: each flag re-uploads its tables
+// in place; moved instances and deformed meshes refit the LBVH (or rebuild it, see kRefitLimit).
+static int update_tables(vh_renderer* r, const vh_scene_tables* t, const vh_update_plan& plan, vh_update_report& rep)
 {
-    if (!r)
-        return fail(VH_ERR_INVALID_ARGUMENT, "null renderer");
-    uint64_t before = r->ctx->uploaded;
-    TRY(upload_tables(r, tables));
-    if (report) {
-        report->blases_built = tables->mesh_count;
-        report->tlas_built = 1;
-        report->bytes_uploaded = r->ctx->uploaded - before;
+    std::vector<uint32_t> instStart;
+    uint64_t ntri = 0;
+    uint32_t translucent = 0;
+    TRY(validate(t, instStart, ntri, translucent));
+    uint64_t lens[6] = {t->positions_len, t->indices_len, t->uvs_len, t->vnormals_len, t->instances_len, t->mesh_count};
+    if (std::memcmp(lens, r->geomLens, sizeof(lens)) != 0 || t->instance_count != r->instanceCount || ntri != r->bvh.ntri)
+        return fail(VH_ERR_INVALID_ARGUMENT, "a partial update must keep the geometry table sizes (use a full plan)");
+    if (plan.blas_count && !plan.blases)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null mesh list");
+    for (uint64_t k = 0; k < plan.blas_count; k++)
+        if (plan.blases[k] >= t->mesh_count)
+            return fail(VH_ERR_INVALID_ARGUMENT, "plan names a missing mesh");
+    vh_context* c = r->ctx;
+    HIPTRY(hipDeviceSynchronize()); // NRC side streams and batches read these tables too
+    if (plan.camera)
+        r->view = t->view;
+    if (plan.vertices) {
+        TRY(reupload(c, r->positions, t->positions, t->positions_len));
+        TRY(reupload(c, r->uvs, t->uvs, t->uvs_len));
+        TRY(reupload(c, r->vnormals, t->vnormals, t->vnormals_len));
     }
+    if (plan.materials) {
+        TRY(reupload(c, r->materials, t->materials, t->materials_len));
+        uint64_t h = texture_hash(t);
+        if (h != r->texHash) {
+            TRY(reupload(c, r->texels, t->texels, t->texels_len));
+            TRY(reupload(c, r->texInfo, t->tex_info, t->tex_info_len));
+            r->texHash = h;
+        }
+    }
+    if (plan.lights) {
+        TRY(reupload(c, r->lights, t->lights, t->lights_len));
+        r->info.light_count = t->info.light_count;
+        std::memcpy(r->info.sky, t->info.sky, sizeof(r->info.sky));
+        r->info.virtual_light = t->info.virtual_light;
+        r->info.transparent_background = t->info.transparent_background;
+        uint64_t h = env_hash(t);
+        if (h != r->envHash) {
+            TRY(reupload(c, r->envPixels, t->env_pixels, t->env_pixels_len));
+            TRY(reupload(c, r->envMarginal, t->env_marginal, t->env_marginal_len));
+            TRY(reupload(c, r->envCond, t->env_cond, t->env_cond_len));
+            r->info.env_width = t->info.env_width;
+            r->info.env_height = t->info.env_height;
+            r->info.env_pdf_scale = t->info.env_pdf_scale;
+            r->envHash = h;
+        }
+    }
+    if (plan.emitters) {
+        TRY(reupload(c, r->emitters, t->emitters, t->emitters_len));
+        TRY(reupload(c, r->emitCdf, t->emit_cdf, t->emit_cdf_len));
+        r->info.emitter_count = t->info.emitter_count;
+    }
+    if (plan.instances)
+        TRY(reupload(c, r->instances, t->instances, t->instances_len));
+    r->translucent = translucent;
+    HIPTRY(hipStreamSynchronize(c->stream));
+    if (plan.vertices || plan.tlas) {
+        DeviceTables dt{r->positions.as<float>(), r->indices.as<uint32_t>(), r->instances.as<uint32_t>(), r->instStart.as<uint32_t>(),
+                        r->instanceCount, (uint32_t)ntri};
+        double ratio = 1.0;
+        TRY(refit_bvh(r->bvh, dt, c->stream, ratio));
+        rep.bvh_work = 1;
+        if (ratio > kRefitLimit) {
+            TRY(build_bvh(r->bvh, dt, c->stream));
+            rep.bvh_work = 2;
+            if (3 * (r->bvh.depth + 1) + 4 > (uint32_t)kStack)
+                return fail(VH_ERR_SCENE, "LBVH deeper than the traversal stack holds (depth " + std::to_string(r->bvh.depth) + ")");
+        }
+        HIPTRY(hipStreamSynchronize(c->stream));
+    }
+    rep.blases_built = plan.vertices ? plan.blas_count : 0;
+    rep.tlas_built = plan.instances && plan.tlas ? 1u : 0u;
+    return VH_OK;
+}
+
+int vh_renderer_update(vh_renderer* r, const vh_scene_tables* tables, const vh_update_plan* plan, vh_update_report* report)
+{
+    if (!r || !tables)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    uint64_t before = r->ctx->uploaded;
+    vh_update_report rep{};
+    if (!plan || plan->full) {
+        HIPTRY(hipDeviceSynchronize());
+        TRY(upload_tables(r, tables));
+        rep.blases_built = tables->mesh_count;
+        rep.tlas_built = 1;
+        rep.bvh_work = 2;
+    } else {
+        TRY(update_tables(r, tables, *plan, rep));
+    }
+    rep.bytes_uploaded = r->ctx->uploaded - before;
+    if (report)
+        *report = rep;
     return VH_OK;
 }
 

@@ -15,6 +15,12 @@
 //   7. collapse to 4-wide nodes of 128 bytes, level by level: each wide node opens its largest
 //      binary children until it holds four, and a binary subtree of at most kLeafMax triangles
 //      becomes a leaf (the triangles of an LBVH subtree are contiguous in key order).
+// The build keeps the leaf order (leaf slot -> global triangle id), the id range of every level
+// and the tree's surface-area cost, so that a scene edit that keeps the triangle count (moved
+// instances, deformed meshes: UpdatePlan's TLAS and BLAS rebuilds, pack.rs:758-821) refits the
+// boxes bottom-up in place instead; a refit whose cost grows past kRefitLimit times the built
+// tree's is rebuilt. Closest hits do not depend on the tree (ties go to the smallest id), so
+// both give the same image.
 #include "vh_internal.h"
 
 namespace vh {
@@ -33,33 +39,50 @@ __device__ __forceinline__ uint32_t ord(float f)
 }
 __host__ __device__ __forceinline__ float unord(uint32_t u) { return asf((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u); }
 
-// 1. world triangles, in global id order: 3 float4 (p0, gid) (p1, instance) (p2, primitive)
+// world-space triangle g: 3 float4 (p0, gid) (p1, instance) (p2, primitive)
+__device__ __forceinline__ void worldTriangle(const float* __restrict__ positions, const uint32_t* __restrict__ indices,
+                                              const uint32_t* __restrict__ instances, const uint32_t* __restrict__ instStart,
+                                              uint32_t ninst, uint32_t g, float4* __restrict__ out)
+{
+    uint32_t lo = 0, hi = ninst - 1; // last instance whose start <= g
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) / 2;
+        if (instStart[mid] <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    uint32_t inst = lo;
+    uint32_t prim = g - instStart[inst];
+    uint32_t base = inst * 16;
+    uint32_t tri = instances[base + 1] + 3 * prim;
+    uint32_t vo = instances[base];
+    V3 p[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        p[k] = transformPoint(instances, base + 4, vtx(positions, indices[tri + k] + vo));
+    out[0] = make_float4(p[0].x, p[0].y, p[0].z, asf(g));
+    out[1] = make_float4(p[1].x, p[1].y, p[1].z, asf(inst));
+    out[2] = make_float4(p[2].x, p[2].y, p[2].z, asf(prim));
+}
+
+// 1. world triangles, in global id order
 __global__ __launch_bounds__(256) void k_world(const float* __restrict__ positions, const uint32_t* __restrict__ indices,
                                                const uint32_t* __restrict__ instances, const uint32_t* __restrict__ instStart,
                                                uint32_t ninst, uint32_t ntri, float4* __restrict__ out)
 {
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ntri; g += gridDim.x * blockDim.x) {
-        uint32_t lo = 0, hi = ninst - 1; // last instance whose start <= g
-        while (lo < hi) {
-            uint32_t mid = (lo + hi + 1) / 2;
-            if (instStart[mid] <= g)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        uint32_t inst = lo;
-        uint32_t prim = g - instStart[inst];
-        uint32_t base = inst * 16;
-        uint32_t tri = instances[base + 1] + 3 * prim;
-        uint32_t vo = instances[base];
-        V3 p[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++)
-            p[k] = transformPoint(instances, base + 4, vtx(positions, indices[tri + k] + vo));
-        out[3 * (size_t)g] = make_float4(p[0].x, p[0].y, p[0].z, asf(g));
-        out[3 * (size_t)g + 1] = make_float4(p[1].x, p[1].y, p[1].z, asf(inst));
-        out[3 * (size_t)g + 2] = make_float4(p[2].x, p[2].y, p[2].z, asf(prim));
-    }
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ntri; g += gridDim.x * blockDim.x)
+        worldTriangle(positions, indices, instances, instStart, ninst, g, out + 3 * (size_t)g);
+}
+
+// refit: world triangles straight into leaf order, leaf slot k holding triangle perm[k]
+__global__ __launch_bounds__(256) void k_world_leaf(const float* __restrict__ positions, const uint32_t* __restrict__ indices,
+                                                    const uint32_t* __restrict__ instances, const uint32_t* __restrict__ instStart,
+                                                    uint32_t ninst, const uint32_t* __restrict__ perm, uint32_t ntri,
+                                                    float4* __restrict__ out)
+{
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ntri; k += gridDim.x * blockDim.x)
+        worldTriangle(positions, indices, instances, instStart, ninst, perm[k], out + 3 * (size_t)k);
 }
 
 // 2. bounds of all vertices: bnd[0..2] min (ordered), bnd[3..5] max (ordered), bnd[6] max |coord| (ordered)
@@ -459,6 +482,88 @@ __global__ void k_root_leaf(const float4* __restrict__ tris, uint32_t n, float4*
     nodes[7] = make_float4(asf(n), asf(0u), asf(0u), asf(0u));
 }
 
+// refit of one level of wide nodes (ids first .. first + count - 1), deepest level first: a leaf
+// slot's box is its triangles' box widened by pad; an inner slot's is the union of its child
+// node's slot boxes, already refit and widened (the same boxes the build gives)
+__global__ __launch_bounds__(256) void k_refit(const float4* __restrict__ tris, float4* __restrict__ nodes, uint32_t first,
+                                               uint32_t count, float pad)
+{
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+        float4* nd = nodes + 8 * (size_t)(first + q);
+        float4 r6 = nd[6], r7 = nd[7];
+        uint32_t ref[4] = {asu(r6.x), asu(r6.y), asu(r6.z), asu(r6.w)};
+        uint32_t cnt[4] = {asu(r7.x), asu(r7.y), asu(r7.z), asu(r7.w)};
+        float lo[3][4], hi[3][4];
+        for (int k = 0; k < 4; k++) {
+            float l[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+            float h[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+            if (ref[k] != kEmptyChild) {
+                if (cnt[k] > 0) {
+                    for (uint32_t t = 0; t < cnt[k]; t++) {
+                        float tl[3], th[3];
+                        triBox(tris, ref[k] + t, tl, th);
+                        for (int a = 0; a < 3; a++) {
+                            l[a] = fminf(l[a], tl[a]);
+                            h[a] = fmaxf(h[a], th[a]);
+                        }
+                    }
+                    for (int a = 0; a < 3; a++) {
+                        l[a] -= pad;
+                        h[a] += pad;
+                    }
+                } else {
+                    const float4* cd = nodes + 8 * (size_t)ref[k];
+                    for (int a = 0; a < 3; a++) {
+                        float4 cl = cd[2 * a], ch = cd[2 * a + 1];
+                        l[a] = fminf(fminf(cl.x, cl.y), fminf(cl.z, cl.w));
+                        h[a] = fmaxf(fmaxf(ch.x, ch.y), fmaxf(ch.z, ch.w));
+                    }
+                }
+            }
+            for (int a = 0; a < 3; a++) {
+                lo[a][k] = l[a];
+                hi[a][k] = h[a];
+            }
+        }
+        for (int a = 0; a < 3; a++) {
+            nd[2 * a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+            nd[2 * a + 1] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+        }
+    }
+}
+
+// surface-area cost of the tree: per block, the sum over the slots of its nodes of the slot box
+// area, times the triangle count for a leaf slot (in FP64; the host adds the blocks in order)
+__global__ __launch_bounds__(256) void k_sah(const float4* __restrict__ nodes, uint32_t n, double* __restrict__ partial)
+{
+    double acc = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4* nd = nodes + 8 * (size_t)i;
+        float4 r6 = nd[6], r7 = nd[7];
+        uint32_t ref[4] = {asu(r6.x), asu(r6.y), asu(r6.z), asu(r6.w)};
+        uint32_t cnt[4] = {asu(r7.x), asu(r7.y), asu(r7.z), asu(r7.w)};
+        float lx[4] = {nd[0].x, nd[0].y, nd[0].z, nd[0].w}, hx[4] = {nd[1].x, nd[1].y, nd[1].z, nd[1].w};
+        float ly[4] = {nd[2].x, nd[2].y, nd[2].z, nd[2].w}, hy[4] = {nd[3].x, nd[3].y, nd[3].z, nd[3].w};
+        float lz[4] = {nd[4].x, nd[4].y, nd[4].z, nd[4].w}, hz[4] = {nd[5].x, nd[5].y, nd[5].z, nd[5].w};
+        for (int k = 0; k < 4; k++) {
+            if (ref[k] == kEmptyChild)
+                continue;
+            double dx = (double)hx[k] - (double)lx[k], dy = (double)hy[k] - (double)ly[k], dz = (double)hz[k] - (double)lz[k];
+            acc += (dx * dy + dy * dz + dz * dx) * (cnt[k] > 0 ? (double)cnt[k] : 1.0);
+        }
+    }
+    __shared__ double s[256];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            s[threadIdx.x] += s[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = s[0];
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -479,6 +584,88 @@ uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 8192)
         if (e_ != hipSuccess)                                                                                               \
             return fail(VH_ERR_KERNEL, std::string(#x) + ": " + hipGetErrorString(e_));                                     \
     } while (0)
+#define TRY(x)                    \
+    do {                          \
+        int rc_ = (x);            \
+        if (rc_ != VH_OK)         \
+            return rc_;           \
+    } while (0)
+
+// The tree's surface-area cost: the sum over every slot of its box area (times the triangle count
+// of a leaf slot), over the root box's area.
+static int tree_cost(const BvhBuild& b, hipStream_t s, double& cost)
+{
+    uint32_t blocks = std::min<uint32_t>(grid_for(b.nodes, 256), 1024u);
+    DevBuf partial;
+    VH_CHECK(partial.alloc(sizeof(double) * blocks));
+    k_sah<<<blocks, 256, 0, s>>>(b.nodes_buf.as<float4>(), b.nodes, partial.as<double>());
+    VH_CHECK(hipGetLastError());
+    std::vector<double> host(blocks);
+    float4 root[6];
+    VH_CHECK(hipMemcpyAsync(host.data(), partial.p, sizeof(double) * blocks, hipMemcpyDeviceToHost, s));
+    VH_CHECK(hipMemcpyAsync(root, b.nodes_buf.p, sizeof(root), hipMemcpyDeviceToHost, s));
+    VH_CHECK(hipStreamSynchronize(s));
+    double sum = 0.0;
+    for (double v : host)
+        sum += v;
+    double ext[3];
+    for (int a = 0; a < 3; a++) {
+        float4 l = root[2 * a], h = root[2 * a + 1];
+        ext[a] = (double)std::max(std::max(h.x, h.y), std::max(h.z, h.w)) - (double)std::min(std::min(l.x, l.y), std::min(l.z, l.w));
+    }
+    double area = ext[0] * ext[1] + ext[1] * ext[2] + ext[2] * ext[0];
+    cost = area > 0.0 ? sum / area : sum;
+    return VH_OK;
+}
+
+// scene bounds of the (world-space) triangles -> b.lo, b.hi and the box padding
+static int scene_bounds(BvhBuild& b, const float4* tris, uint32_t n, hipStream_t s, DevBuf& bnd, uint32_t host_bnd[8])
+{
+    VH_CHECK(bnd.alloc(8 * sizeof(uint32_t)));
+    uint32_t init[8] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u, 0u, 0u};
+    VH_CHECK(hipMemcpyAsync(bnd.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+    k_bounds<<<grid_for(n, 256, 1024), 256, 0, s>>>(tris, n, bnd.as<uint32_t>());
+    VH_CHECK(hipGetLastError());
+    VH_CHECK(hipMemcpyAsync(host_bnd, bnd.p, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    return VH_OK;
+}
+
+static void apply_bounds(BvhBuild& b, const uint32_t host_bnd[8])
+{
+    float scale = unord(host_bnd[6]);
+    b.pad = scale * 1.0e-5f + 1.0e-6f;
+    for (int k = 0; k < 3; k++) {
+        b.lo[k] = unord(host_bnd[k]);
+        b.hi[k] = unord(host_bnd[3 + k]);
+    }
+}
+
+int refit_bvh(BvhBuild& b, const DeviceTables& T, hipStream_t s, double& ratio)
+{
+    ratio = 1.0;
+    if (T.triangleCount != b.ntri || (b.ntri > 0 && (b.levels.size() < 2 || !b.perm.p)))
+        return fail(VH_ERR_INVALID_ARGUMENT, "refit needs the triangle count of the built tree");
+    uint32_t n = b.ntri;
+    if (n == 0)
+        return VH_OK;
+    k_world_leaf<<<grid_for(n, 256), 256, 0, s>>>(T.positions, T.indices, T.instances, T.instStart, T.instanceCount,
+                                                  b.perm.as<uint32_t>(), n, b.tris.as<float4>());
+    VH_CHECK(hipGetLastError());
+    DevBuf bnd;
+    uint32_t host_bnd[8];
+    TRY(scene_bounds(b, b.tris.as<float4>(), n, s, bnd, host_bnd));
+    VH_CHECK(hipStreamSynchronize(s));
+    apply_bounds(b, host_bnd);
+    for (size_t j = b.levels.size() - 1; j-- > 0;) {
+        uint32_t first = b.levels[j], count = b.levels[j + 1] - b.levels[j];
+        k_refit<<<grid_for(count, 256), 256, 0, s>>>(b.tris.as<float4>(), b.nodes_buf.as<float4>(), first, count, b.pad);
+        VH_CHECK(hipGetLastError());
+    }
+    double cost = 0.0;
+    TRY(tree_cost(b, s, cost));
+    ratio = b.sah > 0.0 ? cost / b.sah : 1.0;
+    return VH_OK;
+}
 
 int build_bvh(BvhBuild& out, const DeviceTables& T, hipStream_t s)
 {
@@ -488,6 +675,9 @@ int build_bvh(BvhBuild& out, const DeviceTables& T, hipStream_t s)
     out.nodes = 0;
     out.depth = 0;
     out.pad = 0.0f;
+    out.levels.clear();
+    out.perm.release();
+    out.sah = 0.0;
     if (n == 0)
         return VH_OK;
     DevBuf world, keysA, valsA, keysB, valsB, hist, bnd, child, range, parent, flags, box, qa, qb, qc, wide;
@@ -495,19 +685,14 @@ int build_bvh(BvhBuild& out, const DeviceTables& T, hipStream_t s)
     VH_CHECK(out.tris.alloc(sizeof(float4) * 3 * (size_t)n));
     k_world<<<grid_for(n, 256), 256, 0, s>>>(T.positions, T.indices, T.instances, T.instStart, ninst, n, world.as<float4>());
     VH_CHECK(hipGetLastError());
-    VH_CHECK(bnd.alloc(8 * sizeof(uint32_t)));
-    uint32_t init[8] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u, 0u, 0u};
-    VH_CHECK(hipMemcpyAsync(bnd.p, init, sizeof(init), hipMemcpyHostToDevice, s));
-    k_bounds<<<grid_for(n, 256, 1024), 256, 0, s>>>(world.as<float4>(), n, bnd.as<uint32_t>());
-    VH_CHECK(hipGetLastError());
+    uint32_t host_bnd[8];
+    TRY(scene_bounds(out, world.as<float4>(), n, s, bnd, host_bnd));
     VH_CHECK(keysA.alloc(4 * (size_t)n));
     VH_CHECK(valsA.alloc(4 * (size_t)n));
     VH_CHECK(keysB.alloc(4 * (size_t)n));
     VH_CHECK(valsB.alloc(4 * (size_t)n));
     k_morton<<<grid_for(n, 256), 256, 0, s>>>(world.as<float4>(), n, bnd.as<uint32_t>(), keysA.as<uint32_t>(), valsA.as<uint32_t>());
     VH_CHECK(hipGetLastError());
-    uint32_t host_bnd[8];
-    VH_CHECK(hipMemcpyAsync(host_bnd, bnd.p, sizeof(host_bnd), hipMemcpyDeviceToHost, s));
 
     uint32_t tiles = (n + kRsTile - 1) / kRsTile;
     VH_CHECK(hist.alloc(4 * 256 * (size_t)tiles));
@@ -524,12 +709,10 @@ int build_bvh(BvhBuild& out, const DeviceTables& T, hipStream_t s)
     k_gather<<<grid_for(n, 256), 256, 0, s>>>(world.as<float4>(), vin, n, out.tris.as<float4>());
     VH_CHECK(hipGetLastError());
     VH_CHECK(hipStreamSynchronize(s));
-    float scale = unord(host_bnd[6]);
-    out.pad = scale * 1.0e-5f + 1.0e-6f;
-    for (int k = 0; k < 3; k++) {
-        out.lo[k] = unord(host_bnd[k]);
-        out.hi[k] = unord(host_bnd[3 + k]);
-    }
+    apply_bounds(out, host_bnd);
+    // after four passes the sorted pairs are back in keysA/valsA: the leaf order stays for refits
+    if (vin != valsA.as<uint32_t>())
+        return fail(VH_ERR_KERNEL, "radix sort left its values in the wrong buffer");
 
     if (n <= kLeafMax) {
         VH_CHECK(out.nodes_buf.alloc(sizeof(float4) * 8));
@@ -539,7 +722,9 @@ int build_bvh(BvhBuild& out, const DeviceTables& T, hipStream_t s)
         out.nodes = 1;
         out.depth = 1;
         out.leaves = 1;
-        return VH_OK;
+        out.levels = {0u, 1u};
+        out.perm = std::move(valsA);
+        return tree_cost(out, s, out.sah);
     }
     VH_CHECK(child.alloc(sizeof(uint2) * (n - 1)));
     VH_CHECK(range.alloc(sizeof(uint2) * (n - 1)));
@@ -566,6 +751,8 @@ int build_bvh(BvhBuild& out, const DeviceTables& T, hipStream_t s)
     uint32_t* cnt = qc.as<uint32_t>();
     uint32_t level = 0;
     uint32_t items = 1;
+    // the wide nodes of one level are the ids the level above allocated, in one contiguous run
+    out.levels = {0u, 1u};
     while (items > 0) {
         level++;
         k_fill_u32<<<1, 1, 0, s>>>(cnt + 1, 1, 0u);
@@ -580,6 +767,8 @@ int build_bvh(BvhBuild& out, const DeviceTables& T, hipStream_t s)
         VH_CHECK(hipMemcpyAsync(cnt, cnt + 1, sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
         std::swap(qin, qout);
         out.nodes = hc[2];
+        if (hc[2] > out.levels.back())
+            out.levels.push_back(hc[2]);
         if (level > 200)
             return fail(VH_ERR_SCENE, "BVH collapse did not terminate");
     }
@@ -588,7 +777,8 @@ int build_bvh(BvhBuild& out, const DeviceTables& T, hipStream_t s)
     VH_CHECK(out.nodes_buf.alloc(sizeof(float4) * 8 * (size_t)out.nodes));
     VH_CHECK(hipMemcpyAsync(out.nodes_buf.p, wide.p, sizeof(float4) * 8 * (size_t)out.nodes, hipMemcpyDeviceToDevice, s));
     VH_CHECK(hipStreamSynchronize(s));
-    return VH_OK;
+    out.perm = std::move(valsA);
+    return tree_cost(out, s, out.sah);
 }
 
 } // namespace vh

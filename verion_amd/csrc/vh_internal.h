@@ -5,6 +5,7 @@
 
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "../../include/verion_hip.h"
 #include "vh_shade.h"
@@ -74,6 +75,9 @@ struct DeviceTables {
 struct BvhBuild {
     DevBuf nodes_buf;
     DevBuf tris;
+    DevBuf perm;                  // leaf slot -> global triangle id
+    std::vector<uint32_t> levels; // wide node ids of level j: levels[j] .. levels[j + 1] - 1
+    double sah = 0.0;             // surface-area cost of the tree as built
     uint32_t nodes = 0;
     uint32_t leaves = 0;
     uint32_t depth = 0;
@@ -82,7 +86,13 @@ struct BvhBuild {
     float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {1.0f, 1.0f, 1.0f}; // scene box of all vertices
 };
 
+// A refit whose surface-area cost exceeds the built tree's by this factor is rebuilt instead.
+constexpr double kRefitLimit = 1.5;
+
 int build_bvh(BvhBuild& out, const DeviceTables& T, hipStream_t s);
+// New world-space triangles in the built leaf order and new boxes, bottom-up; ratio = the
+// refit tree's surface-area cost over the built one's. The triangle count must not change.
+int refit_bvh(BvhBuild& b, const DeviceTables& T, hipStream_t s, double& ratio);
 
 // Wavefront buffers of one accumulation (slot-indexed structure of arrays, two copies for the
 // ping-pong between depths).
@@ -175,6 +185,10 @@ struct vh_renderer {
     uint32_t wfBlocks = 0;
     uint32_t tileW = 8, tileH = 8; // depth-0 pixel order of the wavefront
     vh::DevBuf counters; // traversal counter banks while counting is on
+    // what a partial update must keep: the table lengths of positions, indices, uvs, vertex
+    // normals, instances and meshes; hashes of the texture and environment tables
+    uint64_t geomLens[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t texHash = 0, envHash = 0;
 
     vh::Scene scene() const
     {

@@ -415,6 +415,57 @@ def pack(s: sc.Scene) -> Packed:
                   tex_info, meshes, inst_mesh, info, View.of(s.camera))
 
 
+@dataclass
+class UpdatePlan:
+    """What a batch of scene changes makes a backend redo (pack.rs:758-825): each flag is a table to
+    re-upload or a structure to redo; everything else is reused.
+
+    | change | reused | redone |
+    |---|---|---|
+    | camera | everything | the view |
+    | material | LBVH, vertex tables | material and texture tables, emitters, instance table |
+    | light | LBVH, tables | light table, sky, environment map |
+    | instance transform | vertex tables | instance table, LBVH refit, emitters |
+    | mesh deformation | other tables | vertex tables, LBVH refit, emitters |
+    | geometry | nothing | everything |
+    """
+
+    full: bool = False
+    materials: bool = False
+    lights: bool = False
+    emitters: bool = False
+    instances: bool = False
+    vertices: bool = False
+    blases: List[int] = field(default_factory=list)
+    tlas: bool = False
+    camera: bool = False
+
+    @staticmethod
+    def new(changes: sc.Changes) -> "UpdatePlan":
+        """UpdatePlan::new (pack.rs:785-815)."""
+        p = UpdatePlan(camera=changes.camera, lights=changes.lights)
+        if changes.geometry:
+            return UpdatePlan(full=True, materials=True, lights=True, emitters=True, instances=True, vertices=True,
+                              tlas=True, camera=changes.camera)
+        if changes.materials:
+            p.materials = True
+            p.emitters = True
+        if changes.instances:
+            p.instances = True
+            p.tlas = True
+            p.emitters = True
+        if changes.deformed:
+            p.vertices = True
+            p.blases = list(changes.deformed)
+            p.tlas = True
+            p.emitters = True
+        p.instances |= p.emitters  # emitters carry the instance table's emitter bases
+        return p
+
+    def is_empty(self) -> bool:
+        return self == UpdatePlan()
+
+
 def base_params(info: SceneInfo, view: View, width: int, height: int, lighting: bool, multiscatter: bool,
                 adaptive=None, restir=None) -> np.ndarray:
     """pack.rs:828-874: the parameter block with the per-dispatch words (6, 30, 31, 35) zero.

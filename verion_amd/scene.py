@@ -142,6 +142,28 @@ class Camera:
 
 
 @dataclass
+class Changes:
+    """What changed since the last `Scene.take_changes` (scene.rs:109-129): a renderer's update
+    redoes only that."""
+
+    geometry: bool = False  # meshes or instances added or re-pointed, or a mesh changed shape
+    deformed: List[int] = field(default_factory=list)  # meshes with new vertices, same triangles (sorted)
+    instances: List[int] = field(default_factory=list)  # instances with a new transform or material (sorted)
+    materials: List[int] = field(default_factory=list)  # materials with new parameters (sorted)
+    lights: bool = False
+    camera: bool = False
+
+    def is_empty(self) -> bool:
+        return self == Changes()
+
+
+def _mark(ids: List[int], i: int) -> None:
+    if i not in ids:
+        ids.append(i)
+        ids.sort()
+
+
+@dataclass
 class Scene:
     meshes: List[Mesh] = field(default_factory=list)
     materials: List[BaseBsdf] = field(default_factory=list)
@@ -153,24 +175,92 @@ class Scene:
     instance_light_masks: List[int] = field(default_factory=list)
     instance_emit_masks: List[int] = field(default_factory=list)
     virtual_light: bool = True  # StylizationWorld::virtual_light default
+    changes: Changes = field(default_factory=Changes, repr=False, compare=False)
+
+    # The setters record what they change the way scene.rs:268-448 does; assigning the fields
+    # directly records nothing.
+
+    def take_changes(self) -> Changes:
+        """scene.rs:223-226: the changes since the previous call, which are then forgotten."""
+        c, self.changes = self.changes, Changes()
+        return c
 
     def add_mesh(self, mesh: Mesh) -> int:
+        self.changes.geometry = True
         self.meshes.append(mesh)
         return len(self.meshes) - 1
 
+    def set_mesh(self, i: int, mesh: Mesh) -> None:
+        """scene.rs:275-284: the same triangles over the same vertex count is a deformation."""
+        old = self.meshes[i]
+        if np.array_equal(old.indices, mesh.indices) and len(old.positions) == len(mesh.positions):
+            _mark(self.changes.deformed, i)
+        else:
+            self.changes.geometry = True
+        self.meshes[i] = mesh
+
     def add_material(self, m: BaseBsdf) -> int:
+        self.changes.geometry = True
         self.materials.append(m)
         return len(self.materials) - 1
+
+    def set_material(self, i: int, m: BaseBsdf) -> None:
+        if self.materials[i] != m:
+            _mark(self.changes.materials, i)
+        self.materials[i] = m
 
     def add_instance(self, inst: Instance) -> int:
         if not (0 <= inst.mesh < len(self.meshes)) or not (0 <= inst.material < len(self.materials)):
             raise ValueError("instance refers to a missing mesh or material")
+        self.changes.geometry = True
         self.instances.append(inst)
         return len(self.instances) - 1
 
+    def set_instance(self, i: int, inst: Instance) -> None:
+        """scene.rs:331-343: another mesh is a geometry change, another transform or material is not."""
+        if not (0 <= inst.mesh < len(self.meshes)) or not (0 <= inst.material < len(self.materials)):
+            raise ValueError("instance refers to a missing mesh or material")
+        old = self.instances[i]
+        if old.mesh != inst.mesh:
+            self.changes.geometry = True
+        elif old != inst:
+            _mark(self.changes.instances, i)
+        self.instances[i] = inst
+
     def add_light(self, light: Light) -> int:
+        self.changes.lights = True
         self.lights.append(light)
         return len(self.lights) - 1
+
+    def set_light(self, i: int, light: Light) -> None:
+        self.changes.lights = True
+        self.lights[i] = light
+
+    def set_camera(self, camera: Camera) -> None:
+        self.changes.camera = True
+        self.camera = camera
+
+    def set_transparent_background(self, on: bool) -> None:
+        self.changes.lights = True
+        self.transparent_background = on
+
+    def set_instance_light_mask(self, i: int, mask: int) -> None:
+        """scene.rs:393-405: link masks live in the light table."""
+        if self.instance_light_mask(i) == mask:
+            return
+        while len(self.instance_light_masks) <= i:
+            self.instance_light_masks.append(0xFFFFFFFF)
+        self.changes.lights = True
+        self.instance_light_masks[i] = mask
+
+    def set_instance_emit_mask(self, i: int, mask: int) -> None:
+        """scene.rs:411-425: emitter records carry the mask, so a change rebuilds."""
+        if self.instance_emit_mask(i) == mask:
+            return
+        while len(self.instance_emit_masks) <= i:
+            self.instance_emit_masks.append(0xFFFFFFFF)
+        self.changes.geometry = True
+        self.instance_emit_masks[i] = mask
 
     def add_image(self, image: Image) -> int:
         self.images.append(image)
