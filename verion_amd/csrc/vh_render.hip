@@ -10,7 +10,10 @@
 //     one global atomic per block (the reference appends with one atomic per path, wf_shade.slang:24);
 //   - instead of an indirect dispatch sized by a one-thread kernel (wf_args.slang), the extend and
 //     shade kernels are grid-stride loops over the live count of their depth, read on the device,
-//     so the host never waits between bounces and no launch depends on a count.
+//     so the host never waits between bounces and no launch depends on a count;
+//   - the extend and shadow kernels trace with the wave-cooperative traversal (traceWave in
+//     vh_trace.h): a wave's leaf-triangle tests are shared by its 64 lanes through LDS
+//     (1080p procedural scene: wf_extend 0.366 -> 0.281 ms, wf_shadow 0.315 -> 0.204 ms per launch).
 // Both integrators call the same device functions (vh_shade.h), so their images are identical.
 #include "vh_internal.h"
 #include "vh_wavefront.h"
@@ -30,6 +33,7 @@ constexpr uint32_t kWfBlock = 256;
 #ifndef VH_SHADE_WAVES
 #define VH_SHADE_WAVES 3
 #endif
+
 
 template <bool RESTIR, bool SPLIT>
 __global__ __launch_bounds__(64) void k_pathtrace(Scene S, Frame F, Params P, uint32_t samples)
@@ -163,11 +167,19 @@ __global__ __launch_bounds__(kWfBlock, VH_EXTEND_WAVES) void k_wf_extend(Scene S
         tmax = 1.0e30f;
     };
     auto done = [&](uint32_t i, const Hit& h) { W.hits[i] = make_uint4(h.hit ? 1u : 0u, h.inst, h.prim, asu(h.t)); };
-    for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < count; i += gridDim.x * kWfBlock) {
-        V3 org, dir;
-        float tmax;
-        load(i, org, dir, tmax);
-        done(i, trace<false>(S.bvh, org, dir, tmax));
+    // wave-uniform grid stride: the 64 lanes of a wave trace together (traceWave)
+    __shared__ WaveTris waveTris[kWfBlock / 64];
+    WaveTris& sh = waveTris[threadIdx.x >> 6];
+    for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += gridDim.x * kWfBlock) {
+        uint32_t i = base + threadIdx.x;
+        bool act = i < count;
+        V3 org = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
+        float tmax = 1.0e30f;
+        if (act)
+            load(i, org, dir, tmax);
+        Hit h = traceWave<false>(S.bvh, org, dir, tmax, act, sh);
+        if (act)
+            done(i, h);
     }
 }
 
@@ -275,6 +287,23 @@ __global__ __launch_bounds__(kWfBlock, VH_EXTEND_WAVES) void k_wf_shadow(Scene S
         dir = v3(r[3 * stride], r[4 * stride], r[5 * stride]);
         tmax = r[6 * stride];
     };
+    if (!S.translucent) { // any hit ends an opaque shadow ray: the wave-cooperative traversal
+        __shared__ WaveTris waveTris[kWfBlock / 64];
+        WaveTris& sh = waveTris[threadIdx.x >> 6];
+        for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += gridDim.x * kWfBlock) {
+            uint32_t q = base + threadIdx.x;
+            bool act = q < count;
+            V3 o = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
+            float tmax = 1.0e30f;
+            if (act)
+                load(q, o, dir, tmax);
+            Hit h = traceWave<true>(S.bvh, o, dir, tmax, act, sh);
+            if (act)
+                W.shVis[W.shQueue[q]] = h.hit ? 1u : 0u;
+        }
+        return;
+    }
+    // translucent scenes: closest hits per layer (occluded)
     for (uint32_t q = blockIdx.x * kWfBlock + threadIdx.x; q < count; q += gridDim.x * kWfBlock) {
         V3 o, dir;
         float tmax;

@@ -284,6 +284,189 @@ __device__ __forceinline__ bool traceStep(const Bvh& bvh, TraceState& s, uint32_
     return true;
 }
 
+// ---------------------------------------------------------------- wave-cooperative traversal
+//
+// In traceStep a wave runs its triangle loop as many times as its busiest lane has leaf triangles
+// in the node (up to 16) while most lanes wait: ~0.45 tests per lane and node step against a
+// maximum of 4 to 8. traceWave has the wave's 64 lanes step their rays together and share the
+// triangle tests of each step through LDS: the lanes list their pending (ray, triangle) tests, every
+// lane takes the next 64 of them (usually all), and each ray keeps the smallest (t, id) found,
+// through a 64-bit LDS atomic minimum. A ray's tests within one step use the limit of the step's
+// start instead of one tightened after each test, which only adds tests whose hits lose: the hit,
+// the smallest t with ties to the smallest id, is the same. Lanes whose ray is done keep testing
+// the other lanes' triangles until the wave's last ray ends. All 64 lanes of a wave must call it
+// together (the wave-uniform loops of the wavefront kernels).
+
+constexpr uint32_t kWaveTests = 1024; // 64 lanes x 16 leaf triangles per node at most
+
+struct WaveTris {
+    float4 ray0[64];  // origin, and the largest t a triangle can have and still matter this step
+    float4 ray1[64];  // shear constants sx, sy, sz of the watertight test, tmax
+    uint4 base[64];   // first triangle (leaf order) of each child slot of the node
+    uint32_t axes[64]; // kx | ky << 2 | kz << 4
+    unsigned long long key[64]; // (t bits << 32) | global id: the best hit of the step
+    float4 win[64];   // b1, b2, instance, primitive of that hit
+    uint16_t list[kWaveTests]; // pending tests: owner lane << 4 | child slot << 2 | triangle
+};
+
+__device__ __forceinline__ void waveSync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// number of set bits of m in the lanes below this one
+__device__ __forceinline__ uint32_t lanesBelow(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+template <bool ANY>
+__device__ Hit traceWave(const Bvh& bvh, V3 o, V3 d, float tmax, bool active, WaveTris& sh)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    TraceState s;
+    traceBegin(s, o, d, tmax);
+    Hit& h = s.h;
+    bool live = active && bvh.ntri != 0;
+    sh.ray0[lane] = make_float4(o.x, o.y, o.z, tmax);
+    sh.ray1[lane] = make_float4(s.pre.sx, s.pre.sy, s.pre.sz, tmax);
+    sh.axes[lane] = (uint32_t)s.pre.kx | ((uint32_t)s.pre.ky << 2) | ((uint32_t)s.pre.kz << 4);
+    uint32_t stackN[kStack];
+    float stackT[kStack];
+    const float kInf = __builtin_inff();
+    while (__ballot(live) != 0) {
+        uint32_t tris = 0;
+        float key[4] = {kInf, kInf, kInf, kInf};
+        uint32_t val[4] = {0u, 0u, 0u, 0u};
+        if (live) {
+            const float4* n = bvh.nodes + 8 * (size_t)s.node;
+            s.visited++;
+            float tn[4];
+            uint32_t mask = slab4(n, s.ix, s.iy, s.iz, s.ox, s.oy, s.oz, s.best, tn);
+            uint4 child = *reinterpret_cast<const uint4*>(n + 6);
+            uint4 count = *reinterpret_cast<const uint4*>(n + 7);
+#pragma unroll
+            for (uint32_t c = 0; c < 4; c++) {
+                bool hitc = (mask >> c & 1u) != 0 && sel4(child, c) != kEmptyChild;
+                uint32_t cnt = sel4(count, c);
+                if (hitc && cnt != 0 && !(h.hit && tn[c] > h.t))
+                    tris |= ((1u << cnt) - 1u) << (4 * c);
+                key[c] = hitc && cnt == 0 ? tn[c] : kInf;
+                val[c] = sel4(child, c);
+            }
+            if (tris != 0) {
+                sh.base[lane] = child;
+                sh.key[lane] = ~0ull;
+                sh.ray0[lane].w = h.hit ? asf(asu(h.t) + 1u) : s.tmax; // ties are kept for the id comparison
+            }
+        }
+        // exclusive prefix of the per-lane test counts (at most 16: five bits) over the wave
+        uint32_t cnt = (uint32_t)__builtin_popcount(tris);
+        uint32_t pre = 0, total = 0;
+#pragma unroll
+        for (uint32_t b = 0; b < 5; b++) {
+            uint64_t m = __ballot((cnt >> b) & 1u);
+            pre += lanesBelow(m) << b;
+            total += (uint32_t)__builtin_popcountll(m) << b;
+        }
+        if (total != 0) {
+            for (uint32_t m = tris, j = pre; m != 0; m &= m - 1u, j++)
+                sh.list[j] = (uint16_t)((lane << 4) | (uint32_t)__builtin_ctz(m));
+            s.tested += cnt;
+            waveSync();
+            for (uint32_t r = 0; r < total; r += 64) {
+                uint32_t q = r + lane;
+                bool found = false;
+                uint32_t owner = 0;
+                unsigned long long mine = 0;
+                float4 w = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (q < total) {
+                    uint32_t e = sh.list[q];
+                    owner = e >> 4;
+                    uint32_t c = (e >> 2) & 3u, k = e & 3u;
+                    float4 r0 = sh.ray0[owner], r1 = sh.ray1[owner];
+                    uint32_t ax = sh.axes[owner];
+                    RayPre rp;
+                    rp.kx = (int)(ax & 3u);
+                    rp.ky = (int)((ax >> 2) & 3u);
+                    rp.kz = (int)((ax >> 4) & 3u);
+                    rp.sx = r1.x;
+                    rp.sy = r1.y;
+                    rp.sz = r1.z;
+                    const float4* tr = bvh.tris + 3 * (size_t)(sel4(sh.base[owner], c) + k);
+                    float4 a = tr[0], bb = tr[1], cc = tr[2];
+                    float t, b1, b2;
+                    if (intersect(rp, v3(r0.x, r0.y, r0.z), xyz(a), xyz(bb), xyz(cc), r0.w, t, b1, b2) && t < r1.w) {
+                        found = true;
+                        mine = ((unsigned long long)asu(t) << 32) | asu(a.w); // t > 0: its bits order like the value
+                        w = make_float4(b1, b2, bb.w, cc.w);
+                        atomicMin(&sh.key[owner], mine);
+                    }
+                }
+                waveSync();
+                if (found && sh.key[owner] == mine)
+                    sh.win[owner] = w;
+                waveSync();
+            }
+            if (tris != 0) {
+                unsigned long long kk = sh.key[lane];
+                if (kk != ~0ull) {
+                    float t = asf((uint32_t)(kk >> 32));
+                    uint32_t gid = (uint32_t)kk;
+                    if (!h.hit || t < h.t || (t == h.t && gid < h.gid)) {
+                        float4 w = sh.win[lane];
+                        h.hit = true;
+                        h.t = t;
+                        h.b1 = w.x;
+                        h.b2 = w.y;
+                        h.gid = gid;
+                        h.inst = asu(w.z);
+                        h.prim = asu(w.w);
+                        s.best = t;
+                    }
+                }
+            }
+            waveSync();
+        }
+        if (live) {
+            if (ANY && h.hit) {
+                live = false;
+            } else {
+                if (!ANY) {
+                    cmpx(key[0], val[0], key[1], val[1]);
+                    cmpx(key[2], val[2], key[3], val[3]);
+                    cmpx(key[0], val[0], key[2], val[2]);
+                    cmpx(key[1], val[1], key[3], val[3]);
+                    cmpx(key[1], val[1], key[2], val[2]);
+                }
+#pragma unroll
+                for (int i = 3; i >= 0; i--) {
+                    if (key[i] < kInf && s.sp < kStack) {
+                        stackN[s.sp] = val[i];
+                        if (!ANY)
+                            stackT[s.sp] = key[i];
+                        s.sp++;
+                    }
+                }
+                live = false;
+                while (s.sp > 0) {
+                    s.sp--;
+                    if (!ANY && h.hit && stackT[s.sp] > h.t)
+                        continue;
+                    s.node = stackN[s.sp];
+                    live = true;
+                    break;
+                }
+            }
+        }
+    }
+    if (active)
+        count_traversal<ANY>(bvh, s.visited, s.tested, h.hit);
+    return h;
+}
+
 template <bool ANY>
 __device__ Hit trace(const Bvh& bvh, V3 o, V3 d, float tmax)
 {
