@@ -204,27 +204,29 @@ __global__ __launch_bounds__(kBxBlock) void k_bx_extend(const BatchJob* __restri
     const uint32_t total = buildPrefix(threadIdx.x < B.njobs ? B.counts[threadIdx.x * kCountStride + d] : 0u, B.njobs, s_prefix);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t* st = B.state[d & 1u];
+    __shared__ WaveTris waveTris[kBxWaves];
+    WaveTris& sh = waveTris[threadIdx.x >> 6];
     for (uint32_t gw = blockIdx.x * kBxWaves + (threadIdx.x >> 6); gw < total; gw += gridDim.x * kBxWaves) {
         const uint32_t j = findJob(s_prefix, B.njobs, gw);
         const BatchJob& J = jobs[j];
         const uint32_t count = B.counts[j * kCountStride + d];
         const uint32_t i = (gw - s_prefix[j]) * 64 + lane;
-        if (i >= count)
-            continue;
+        const bool act = i < count; // the wave's lanes trace together (traceWave); idle lanes help
         const uint32_t slot = J.slotBase + i;
-        V3 org, dir;
-        if (d == 0) {
+        V3 org = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
+        if (act && d == 0) {
             uint32_t p = J.P.w[32] != 0 ? B.queue0[slot] : tilePixel(i, J.P.w[0], J.P.w[1], B.tileW, B.tileH);
             PathState s;
             generatePath(J.P, p % J.P.w[0], p / J.P.w[0], J.P.w[6] + pass, s);
             org = s.org;
             dir = s.dir;
-        } else {
+        } else if (act) {
             org = get3(st, B.n, 0, slot);
             dir = get3(st, B.n, 3, slot);
         }
-        Hit h = trace<false>(J.S.bvh, org, dir, 1.0e30f);
-        B.hits[slot] = make_uint4(h.hit ? 1u : 0u, h.inst, h.prim, asu(h.t));
+        Hit h = traceWave<false>(J.S.bvh, org, dir, 1.0e30f, act, sh);
+        if (act)
+            B.hits[slot] = make_uint4(h.hit ? 1u : 0u, h.inst, h.prim, asu(h.t));
     }
 }
 
@@ -303,19 +305,31 @@ __global__ __launch_bounds__(kBxBlock) void k_bx_shadow(const BatchJob* __restri
     const uint32_t lane = threadIdx.x & 63u;
     const size_t stride = (size_t)B.shK * B.n;
     const Frame F{};
+    __shared__ WaveTris waveTris[kBxWaves];
+    WaveTris& sh = waveTris[threadIdx.x >> 6];
     for (uint32_t gw = blockIdx.x * kBxWaves + (threadIdx.x >> 6); gw < total; gw += gridDim.x * kBxWaves) {
         const uint32_t j = findJob(s_prefix, B.njobs, gw);
         const BatchJob& J = jobs[j];
         const uint32_t count = B.counts[j * kCountStride + kShadowOff + d];
         const uint32_t i = (gw - s_prefix[j]) * 64 + lane;
-        if (i >= count)
-            continue;
+        const bool act = i < count;
         const Ctx c{J.S, F, J.P};
-        uint32_t rid = B.shQueue[B.shK * J.slotBase + i];
+        uint32_t rid = act ? B.shQueue[B.shK * J.slotBase + i] : 0u;
         const float* r = B.shRec + rid;
-        V3 o = v3(r[0], r[stride], r[2 * stride]);
-        V3 dir = v3(r[3 * stride], r[4 * stride], r[5 * stride]);
-        B.shVis[rid] = occluded(c, o, dir, r[6 * stride]) ? 1u : 0u;
+        V3 o = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
+        float tmax = 1.0e30f;
+        if (act) {
+            o = v3(r[0], r[stride], r[2 * stride]);
+            dir = v3(r[3 * stride], r[4 * stride], r[5 * stride]);
+            tmax = r[6 * stride];
+        }
+        if (!J.S.translucent) { // wave-uniform: the job is the wave's
+            Hit h = traceWave<true>(J.S.bvh, o, dir, tmax, act, sh);
+            if (act)
+                B.shVis[rid] = h.hit ? 1u : 0u;
+        } else if (act) {
+            B.shVis[rid] = occluded(c, o, dir, tmax) ? 1u : 0u;
+        }
     }
 }
 

@@ -24,6 +24,7 @@ namespace {
 
 constexpr uint32_t kWfBlock = 256;
 
+
 // Minimum waves per SIMD the compiler must fit the extend and shade kernels into (their register
 // budget). Measured on the 1080p procedural scene (8192 blocks, 8x8 tiles): free / free 167,
 // 5 / 3 177, 6 / free 154 (spills) Msamples/s.
@@ -167,9 +168,9 @@ __global__ __launch_bounds__(kWfBlock, VH_EXTEND_WAVES) void k_wf_extend(Scene S
         tmax = 1.0e30f;
     };
     auto done = [&](uint32_t i, const Hit& h) { W.hits[i] = make_uint4(h.hit ? 1u : 0u, h.inst, h.prim, asu(h.t)); };
-    // wave-uniform grid stride: the 64 lanes of a wave trace together (traceWave)
     __shared__ WaveTris waveTris[kWfBlock / 64];
     WaveTris& sh = waveTris[threadIdx.x >> 6];
+    // wave-uniform grid stride: the 64 lanes of a wave trace together (traceWave)
     for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += gridDim.x * kWfBlock) {
         uint32_t i = base + threadIdx.x;
         bool act = i < count;
@@ -364,20 +365,25 @@ __global__ __launch_bounds__(kWfBlock) void k_nrc_t_extend(Scene S, Params P, Wa
 {
     uint32_t count = W.counts[d];
     const uint32_t* st = W.state[d & 1u];
-    for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < count; i += gridDim.x * kWfBlock) {
-        V3 org, dir;
-        if (d == 0) {
+    __shared__ WaveTris waveTris[kWfBlock / 64];
+    WaveTris& sh = waveTris[threadIdx.x >> 6];
+    for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += gridDim.x * kWfBlock) {
+        uint32_t i = base + threadIdx.x;
+        bool act = i < count;
+        V3 org = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
+        if (act && d == 0) {
             uint32_t p = trainPix[i];
             PathState s;
             generatePath(P, p % P.w[0], p / P.w[0], P.w[6], s);
             org = s.org;
             dir = s.dir;
-        } else {
+        } else if (act) {
             org = get3(st, W.n, 0, i);
             dir = get3(st, W.n, 3, i);
         }
-        Hit h = trace<false>(S.bvh, org, dir, 1.0e30f);
-        W.hits[i] = make_uint4(h.hit ? 1u : 0u, h.inst, h.prim, asu(h.t));
+        Hit h = traceWave<false>(S.bvh, org, dir, 1.0e30f, act, sh);
+        if (act)
+            W.hits[i] = make_uint4(h.hit ? 1u : 0u, h.inst, h.prim, asu(h.t));
     }
 }
 

@@ -322,149 +322,165 @@ __device__ __forceinline__ uint32_t lanesBelow(uint64_t m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-template <bool ANY>
-__device__ Hit traceWave(const Bvh& bvh, V3 o, V3 d, float tmax, bool active, WaveTris& sh)
+// The ray of this lane, for the lanes that test its triangles.
+__device__ __forceinline__ void wavePublish(const TraceState& s, WaveTris& sh)
 {
     const uint32_t lane = threadIdx.x & 63u;
-    TraceState s;
-    traceBegin(s, o, d, tmax);
-    Hit& h = s.h;
-    bool live = active && bvh.ntri != 0;
-    sh.ray0[lane] = make_float4(o.x, o.y, o.z, tmax);
-    sh.ray1[lane] = make_float4(s.pre.sx, s.pre.sy, s.pre.sz, tmax);
+    sh.ray0[lane] = make_float4(s.o.x, s.o.y, s.o.z, s.tmax);
+    sh.ray1[lane] = make_float4(s.pre.sx, s.pre.sy, s.pre.sz, s.tmax);
     sh.axes[lane] = (uint32_t)s.pre.kx | ((uint32_t)s.pre.ky << 2) | ((uint32_t)s.pre.kz << 4);
-    uint32_t stackN[kStack];
-    float stackT[kStack];
+}
+
+// One traversal step of the wave's rays (every lane calls it; `live` lanes have a ray): each live
+// ray visits one node, the wave tests all their leaf triangles together, then each ray pushes its
+// inner children and pops the next node; `live` turns false when the ray is done.
+template <bool ANY>
+__device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& live, WaveTris& sh, uint32_t* stackN, float* stackT)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    Hit& h = s.h;
     const float kInf = __builtin_inff();
-    while (__ballot(live) != 0) {
-        uint32_t tris = 0;
-        float key[4] = {kInf, kInf, kInf, kInf};
-        uint32_t val[4] = {0u, 0u, 0u, 0u};
-        if (live) {
-            const float4* n = bvh.nodes + 8 * (size_t)s.node;
-            s.visited++;
-            float tn[4];
-            uint32_t mask = slab4(n, s.ix, s.iy, s.iz, s.ox, s.oy, s.oz, s.best, tn);
-            uint4 child = *reinterpret_cast<const uint4*>(n + 6);
-            uint4 count = *reinterpret_cast<const uint4*>(n + 7);
+    uint32_t tris = 0;
+    float key[4] = {kInf, kInf, kInf, kInf};
+    uint32_t val[4] = {0u, 0u, 0u, 0u};
+    if (live) {
+        const float4* n = bvh.nodes + 8 * (size_t)s.node;
+        s.visited++;
+        float tn[4];
+        uint32_t mask = slab4(n, s.ix, s.iy, s.iz, s.ox, s.oy, s.oz, s.best, tn);
+        uint4 child = *reinterpret_cast<const uint4*>(n + 6);
+        uint4 count = *reinterpret_cast<const uint4*>(n + 7);
 #pragma unroll
-            for (uint32_t c = 0; c < 4; c++) {
-                bool hitc = (mask >> c & 1u) != 0 && sel4(child, c) != kEmptyChild;
-                uint32_t cnt = sel4(count, c);
-                if (hitc && cnt != 0 && !(h.hit && tn[c] > h.t))
-                    tris |= ((1u << cnt) - 1u) << (4 * c);
-                key[c] = hitc && cnt == 0 ? tn[c] : kInf;
-                val[c] = sel4(child, c);
-            }
-            if (tris != 0) {
-                sh.base[lane] = child;
-                sh.key[lane] = ~0ull;
-                sh.ray0[lane].w = h.hit ? asf(asu(h.t) + 1u) : s.tmax; // ties are kept for the id comparison
-            }
+        for (uint32_t c = 0; c < 4; c++) {
+            bool hitc = (mask >> c & 1u) != 0 && sel4(child, c) != kEmptyChild;
+            uint32_t cnt = sel4(count, c);
+            if (hitc && cnt != 0 && !(h.hit && tn[c] > h.t))
+                tris |= ((1u << cnt) - 1u) << (4 * c);
+            key[c] = hitc && cnt == 0 ? tn[c] : kInf;
+            val[c] = sel4(child, c);
         }
-        // exclusive prefix of the per-lane test counts (at most 16: five bits) over the wave
-        uint32_t cnt = (uint32_t)__builtin_popcount(tris);
-        uint32_t pre = 0, total = 0;
+        if (tris != 0) {
+            sh.base[lane] = child;
+            sh.key[lane] = ~0ull;
+            sh.ray0[lane].w = h.hit ? asf(asu(h.t) + 1u) : s.tmax; // ties are kept for the id comparison
+        }
+    }
+    // exclusive prefix of the per-lane test counts (at most 16: five bits) over the wave
+    uint32_t cnt = (uint32_t)__builtin_popcount(tris);
+    uint32_t pre = 0, total = 0;
 #pragma unroll
-        for (uint32_t b = 0; b < 5; b++) {
-            uint64_t m = __ballot((cnt >> b) & 1u);
-            pre += lanesBelow(m) << b;
-            total += (uint32_t)__builtin_popcountll(m) << b;
-        }
-        if (total != 0) {
-            for (uint32_t m = tris, j = pre; m != 0; m &= m - 1u, j++)
-                sh.list[j] = (uint16_t)((lane << 4) | (uint32_t)__builtin_ctz(m));
-            s.tested += cnt;
-            waveSync();
-            for (uint32_t r = 0; r < total; r += 64) {
-                uint32_t q = r + lane;
-                bool found = false;
-                uint32_t owner = 0;
-                unsigned long long mine = 0;
-                float4 w = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (q < total) {
-                    uint32_t e = sh.list[q];
-                    owner = e >> 4;
-                    uint32_t c = (e >> 2) & 3u, k = e & 3u;
-                    float4 r0 = sh.ray0[owner], r1 = sh.ray1[owner];
-                    uint32_t ax = sh.axes[owner];
-                    RayPre rp;
-                    rp.kx = (int)(ax & 3u);
-                    rp.ky = (int)((ax >> 2) & 3u);
-                    rp.kz = (int)((ax >> 4) & 3u);
-                    rp.sx = r1.x;
-                    rp.sy = r1.y;
-                    rp.sz = r1.z;
-                    const float4* tr = bvh.tris + 3 * (size_t)(sel4(sh.base[owner], c) + k);
-                    float4 a = tr[0], bb = tr[1], cc = tr[2];
-                    float t, b1, b2;
-                    if (intersect(rp, v3(r0.x, r0.y, r0.z), xyz(a), xyz(bb), xyz(cc), r0.w, t, b1, b2) && t < r1.w) {
-                        found = true;
-                        mine = ((unsigned long long)asu(t) << 32) | asu(a.w); // t > 0: its bits order like the value
-                        w = make_float4(b1, b2, bb.w, cc.w);
-                        atomicMin(&sh.key[owner], mine);
-                    }
-                }
-                waveSync();
-                if (found && sh.key[owner] == mine)
-                    sh.win[owner] = w;
-                waveSync();
-            }
-            if (tris != 0) {
-                unsigned long long kk = sh.key[lane];
-                if (kk != ~0ull) {
-                    float t = asf((uint32_t)(kk >> 32));
-                    uint32_t gid = (uint32_t)kk;
-                    if (!h.hit || t < h.t || (t == h.t && gid < h.gid)) {
-                        float4 w = sh.win[lane];
-                        h.hit = true;
-                        h.t = t;
-                        h.b1 = w.x;
-                        h.b2 = w.y;
-                        h.gid = gid;
-                        h.inst = asu(w.z);
-                        h.prim = asu(w.w);
-                        s.best = t;
-                    }
+    for (uint32_t b = 0; b < 5; b++) {
+        uint64_t m = __ballot((cnt >> b) & 1u);
+        pre += lanesBelow(m) << b;
+        total += (uint32_t)__builtin_popcountll(m) << b;
+    }
+    if (total != 0) {
+        for (uint32_t m = tris, j = pre; m != 0; m &= m - 1u, j++)
+            sh.list[j] = (uint16_t)((lane << 4) | (uint32_t)__builtin_ctz(m));
+        s.tested += cnt;
+        waveSync();
+        for (uint32_t r = 0; r < total; r += 64) {
+            uint32_t q = r + lane;
+            bool found = false;
+            uint32_t owner = 0;
+            unsigned long long mine = 0;
+            float4 w = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (q < total) {
+                uint32_t e = sh.list[q];
+                owner = e >> 4;
+                uint32_t c = (e >> 2) & 3u, k = e & 3u;
+                float4 r0 = sh.ray0[owner], r1 = sh.ray1[owner];
+                uint32_t ax = sh.axes[owner];
+                RayPre rp;
+                rp.kx = (int)(ax & 3u);
+                rp.ky = (int)((ax >> 2) & 3u);
+                rp.kz = (int)((ax >> 4) & 3u);
+                rp.sx = r1.x;
+                rp.sy = r1.y;
+                rp.sz = r1.z;
+                const float4* tr = bvh.tris + 3 * (size_t)(sel4(sh.base[owner], c) + k);
+                float4 a = tr[0], bb = tr[1], cc = tr[2];
+                float t, b1, b2;
+                if (intersect(rp, v3(r0.x, r0.y, r0.z), xyz(a), xyz(bb), xyz(cc), r0.w, t, b1, b2) && t < r1.w) {
+                    found = true;
+                    mine = ((unsigned long long)asu(t) << 32) | asu(a.w); // t > 0: its bits order like the value
+                    w = make_float4(b1, b2, bb.w, cc.w);
+                    atomicMin(&sh.key[owner], mine);
                 }
             }
             waveSync();
+            if (found && sh.key[owner] == mine)
+                sh.win[owner] = w;
+            waveSync();
         }
-        if (live) {
-            if (ANY && h.hit) {
-                live = false;
-            } else {
-                if (!ANY) {
-                    cmpx(key[0], val[0], key[1], val[1]);
-                    cmpx(key[2], val[2], key[3], val[3]);
-                    cmpx(key[0], val[0], key[2], val[2]);
-                    cmpx(key[1], val[1], key[3], val[3]);
-                    cmpx(key[1], val[1], key[2], val[2]);
+        if (tris != 0) {
+            unsigned long long kk = sh.key[lane];
+            if (kk != ~0ull) {
+                float t = asf((uint32_t)(kk >> 32));
+                uint32_t gid = (uint32_t)kk;
+                if (!h.hit || t < h.t || (t == h.t && gid < h.gid)) {
+                    float4 w = sh.win[lane];
+                    h.hit = true;
+                    h.t = t;
+                    h.b1 = w.x;
+                    h.b2 = w.y;
+                    h.gid = gid;
+                    h.inst = asu(w.z);
+                    h.prim = asu(w.w);
+                    s.best = t;
                 }
+            }
+        }
+        waveSync();
+    }
+    if (live) {
+        if (ANY && h.hit) {
+            live = false;
+        } else {
+            if (!ANY) {
+                cmpx(key[0], val[0], key[1], val[1]);
+                cmpx(key[2], val[2], key[3], val[3]);
+                cmpx(key[0], val[0], key[2], val[2]);
+                cmpx(key[1], val[1], key[3], val[3]);
+                cmpx(key[1], val[1], key[2], val[2]);
+            }
 #pragma unroll
-                for (int i = 3; i >= 0; i--) {
-                    if (key[i] < kInf && s.sp < kStack) {
-                        stackN[s.sp] = val[i];
-                        if (!ANY)
-                            stackT[s.sp] = key[i];
-                        s.sp++;
-                    }
+            for (int i = 3; i >= 0; i--) {
+                if (key[i] < kInf && s.sp < kStack) {
+                    stackN[s.sp] = val[i];
+                    if (!ANY)
+                        stackT[s.sp] = key[i];
+                    s.sp++;
                 }
-                live = false;
-                while (s.sp > 0) {
-                    s.sp--;
-                    if (!ANY && h.hit && stackT[s.sp] > h.t)
-                        continue;
-                    s.node = stackN[s.sp];
-                    live = true;
-                    break;
-                }
+            }
+            live = false;
+            while (s.sp > 0) {
+                s.sp--;
+                if (!ANY && h.hit && stackT[s.sp] > h.t)
+                    continue;
+                s.node = stackN[s.sp];
+                live = true;
+                break;
             }
         }
     }
+}
+
+// One ray per lane, traced by the whole wave together.
+template <bool ANY>
+__device__ Hit traceWave(const Bvh& bvh, V3 o, V3 d, float tmax, bool active, WaveTris& sh)
+{
+    TraceState s;
+    traceBegin(s, o, d, tmax);
+    bool live = active && bvh.ntri != 0;
+    wavePublish(s, sh);
+    uint32_t stackN[kStack];
+    float stackT[kStack];
+    while (__ballot(live) != 0)
+        waveStep<ANY>(bvh, s, live, sh, stackN, stackT);
     if (active)
-        count_traversal<ANY>(bvh, s.visited, s.tested, h.hit);
-    return h;
+        count_traversal<ANY>(bvh, s.visited, s.tested, s.h.hit);
+    return s.h;
 }
 
 template <bool ANY>
