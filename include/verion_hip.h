@@ -268,10 +268,12 @@ typedef struct vh_nrc_config {
                                  parameters over Adam steps with this decay; 0 = the parameters themselves */
     uint32_t warmup_frames;   /* training-only frames (paths + Adam steps, no film) before the first
                                  rendered frame of the cache; their time counts as the render's */
+    uint32_t minibatch;       /* bool: Adam step k of a frame takes slice k of train_steps equal slices of
+                                 the frame's samples (each sample once per frame) instead of all of them */
 } vh_nrc_config;
 typedef struct vh_nrc vh_nrc;
 
-/* NULL config = {65536, 8, 1e-2, 1, 0.95, 16}. The cache starts with zero parameters: set them first. */
+/* NULL config = {65536, 8, 1e-2, 1, 0.95, 16, 0}. The cache starts with zero parameters: set them first. */
 int vh_nrc_create(vh_renderer* r, const vh_nrc_config* config, vh_nrc** out);
 void vh_nrc_destroy(vh_nrc* nrc);
 /* Replaces the FP32 master parameters (n = 23888), their moving average, and resets Adam. */

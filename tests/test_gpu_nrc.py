@@ -146,3 +146,25 @@ def test_inference_uses_the_bias_corrected_moving_average(ctx):
     assert err < 0.02 * np.abs(want).max() + 1e-3, err
     # and it is the average, not the last parameters
     assert err < 0.5 * np.abs(got - last).max(), (err, np.abs(got - last).max())
+
+
+def test_minibatch_steps_take_disjoint_slices(ctx):
+    # one step per frame: the one slice is every sample, so both modes draw the same film (up to
+    # the order the training samples arrive in, which is not fixed); with four steps the minibatch
+    # mode trains on quarter slices and still renders finite images
+    r = Renderer(ctx, scenes.cornell())
+    r.set_integrator(WAVEFRONT)
+    films = []
+    for minibatch in (False, True):
+        c = Nrc(r, train_paths=4096, train_steps=1, warmup_frames=2, minibatch=minibatch)
+        c.set_params(nrc.init(10))
+        acc = r.new_accumulation(64, 64)
+        c.accumulate(acc, 4)
+        films.append(acc.film())
+    diff = np.abs(films[0] - films[1]) / np.maximum(np.abs(films[1]), 1e-3)
+    assert diff.mean() < 1e-3, diff.mean()
+    c = Nrc(r, train_paths=4096, train_steps=4, warmup_frames=2, minibatch=True)
+    c.set_params(nrc.init(10))
+    acc = r.new_accumulation(64, 64)
+    c.accumulate(acc, 4)
+    assert np.all(np.isfinite(acc.mean())) and c.stats()["samples"] > 0

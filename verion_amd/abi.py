@@ -43,7 +43,8 @@ class Restir(C.Structure):
 
 class NrcConfig(C.Structure):
     _fields_ = [("train_paths", C.c_uint32), ("train_steps", C.c_uint32), ("learning_rate", C.c_float),
-                ("terminate_depth", C.c_uint32), ("ema_decay", C.c_float), ("warmup_frames", C.c_uint32)]
+                ("terminate_depth", C.c_uint32), ("ema_decay", C.c_float), ("warmup_frames", C.c_uint32),
+                ("minibatch", C.c_uint32)]
 
 
 class BatchJob(C.Structure):

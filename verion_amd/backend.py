@@ -398,9 +398,9 @@ class Nrc(_Owned):
     PARAMS = 23888
 
     def __init__(self, renderer: Renderer, train_paths: int = 65536, train_steps: int = 8, learning_rate: float = 1e-2,
-                 terminate_depth: int = 1, ema_decay: float = 0.95, warmup_frames: int = 16):
+                 terminate_depth: int = 1, ema_decay: float = 0.95, warmup_frames: int = 16, minibatch: bool = False):
         self.renderer = renderer
-        cfg = abi.NrcConfig(train_paths, train_steps, learning_rate, terminate_depth, ema_decay, warmup_frames)
+        cfg = abi.NrcConfig(train_paths, train_steps, learning_rate, terminate_depth, ema_decay, warmup_frames, int(minibatch))
         h = C.c_void_p()
         _check(library().vh_nrc_create(renderer.h, C.byref(cfg), C.byref(h)))
         self._handle = _Handle(h, library().vh_nrc_destroy, renderer._handle)

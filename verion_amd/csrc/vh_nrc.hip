@@ -236,10 +236,18 @@ __device__ __forceinline__ void storeLayer(float* part, const f32x4_t (&acc)[NT]
         part[bOff(L) + threadIdx.x] = db;
 }
 
+// The samples lo .. hi - 1 of slice `slice` of `slices` equal parts of n samples.
+__host__ __device__ __forceinline__ void sliceRange(uint32_t n, uint32_t slice, uint32_t slices, uint32_t& lo, uint32_t& hi)
+{
+    lo = (uint32_t)((uint64_t)n * slice / slices);
+    hi = (uint32_t)((uint64_t)n * (slice + 1) / slices);
+}
+
 // Fused forward + backward over 64-row tiles of training samples (4 float4 each: 3 of inputs, the
-// RGB target). Writes this block's gradient partial sums (kParams floats) and loss sum.
+// RGB target), over slice `slice` of `slices` of them. Writes this block's gradient partial sums
+// (kParams floats) and loss sum.
 __global__ __launch_bounds__(256) void k_nrc_train(NrcNet net, const float4* samples, const uint32_t* countPtr, uint32_t countImm,
-                                                   uint32_t cap, float* partials, float* lossPartials)
+                                                   uint32_t cap, uint32_t slice, uint32_t slices, float* partials, float* lossPartials)
 {
     __shared__ __attribute__((aligned(16))) __bf16 s_x[kRows * kLdX];
     __shared__ __attribute__((aligned(16))) __bf16 s_h[5][kRows * kLdH];
@@ -249,10 +257,12 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcNet net, const float4* sam
     __shared__ __attribute__((aligned(16))) __bf16 s_xt[kIn * kLdH];
     __shared__ __attribute__((aligned(16))) __bf16 s_dzt[kHid * kLdH];
     __shared__ float s_loss[256];
-    uint32_t n = countPtr ? *countPtr : countImm;
-    if (n > cap)
-        n = cap;
-    const float invB = 1.0f / (float)(n > 0 ? n : 1);
+    uint32_t total = countPtr ? *countPtr : countImm;
+    if (total > cap)
+        total = cap;
+    uint32_t first, n;
+    sliceRange(total, slice, slices, first, n); // n: one past the slice's last sample
+    const float invB = 1.0f / (float)(n > first ? n - first : 1);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
     f32x4_t a0[6], a1[4], a2[4], a3[4], a4[4], a5[1];
     float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f, d3 = 0.0f, d4 = 0.0f, d5 = 0.0f;
@@ -263,7 +273,7 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcNet net, const float4* sam
     for (auto& v : a4) v = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
     a5[0] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
     float lossAcc = 0.0f;
-    for (uint32_t tile = blockIdx.x * kRows; tile < n; tile += gridDim.x * kRows) {
+    for (uint32_t tile = first + blockIdx.x * kRows; tile < n; tile += gridDim.x * kRows) {
         // encode
         uint32_t q = tile + wave * 16 + r;
         __bf16* xr = s_x + (wave * 16 + r) * kLdX + g * 24;
@@ -367,7 +377,7 @@ __global__ void k_nrc_reduce_groups(const float* partials, uint32_t blocks, floa
 
 // grad[p] = sum of the group sums in group order; loss likewise over the blocks' loss partials.
 __global__ void k_nrc_reduce(const float* groupSums, uint32_t groups, float* grad, const float* lossPartials, uint32_t blocks,
-                             float* loss, const uint32_t* countPtr, uint32_t countImm, uint32_t cap)
+                             float* loss, const uint32_t* countPtr, uint32_t countImm, uint32_t cap, uint32_t slice, uint32_t slices)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < (uint32_t)kParams) {
@@ -377,9 +387,12 @@ __global__ void k_nrc_reduce(const float* groupSums, uint32_t groups, float* gra
         grad[p] = s;
     }
     if (p == 0) {
-        uint32_t n = countPtr ? *countPtr : countImm;
-        if (n > cap)
-            n = cap;
+        uint32_t total = countPtr ? *countPtr : countImm;
+        if (total > cap)
+            total = cap;
+        uint32_t lo, hi;
+        sliceRange(total, slice, slices, lo, hi);
+        uint32_t n = hi - lo;
         float s = 0.0f;
         for (uint32_t b = 0; b < blocks; b++)
             s += lossPartials[b];
@@ -536,18 +549,22 @@ namespace {
     } while (0)
 #define HIPTRY(x) TRY(check_hip((x), #x))
 
+// `steps` Adam steps (or one gradient when !adam) on the samples; with `slices` > 1 step k takes
+// slice k % slices of them (minibatches), else every step takes all of them.
 int train_steps(vh_nrc* nn, const float4* samples, const uint32_t* countPtr, uint32_t countImm, uint32_t cap, uint32_t steps,
-                bool adam, hipStream_t s)
+                bool adam, hipStream_t s, uint32_t slices = 1)
 {
     NrcNet net = nn->net(countPtr == nullptr);
     for (uint32_t k = 0; k < steps; k++) {
-        k_nrc_train<<<kTrainBlocks, 256, 0, s>>>(net, samples, countPtr, countImm, cap, nn->partials.as<float>(), nn->lossPart.as<float>());
+        const uint32_t slice = slices > 1 ? k % slices : 0u, nsl = slices > 1 ? slices : 1u;
+        k_nrc_train<<<kTrainBlocks, 256, 0, s>>>(net, samples, countPtr, countImm, cap, slice, nsl, nn->partials.as<float>(),
+                                                 nn->lossPart.as<float>());
         const uint32_t groups = (kTrainBlocks + kReduceGroup - 1) / kReduceGroup;
         k_nrc_reduce_groups<<<dim3((kParams + 255) / 256, groups), 256, 0, s>>>(nn->partials.as<float>(), kTrainBlocks,
                                                                                 nn->groupSums.as<float>());
         k_nrc_reduce<<<(kParams + 255) / 256, 256, 0, s>>>(nn->groupSums.as<float>(), groups, nn->grad.as<float>(),
                                                            nn->lossPart.as<float>(), kTrainBlocks, nn->loss.as<float>(), countPtr,
-                                                           countImm, cap);
+                                                           countImm, cap, slice, nsl);
         if (adam) {
             nn->step++;
             float c1 = 1.0f - powf(0.9f, (float)nn->step), c2 = 1.0f - powf(0.99f, (float)nn->step);
@@ -590,7 +607,7 @@ int vh_nrc_create(vh_renderer* r, const vh_nrc_config* cfg, vh_nrc** out)
     *out = nullptr;
     std::unique_ptr<vh_nrc> nn(new vh_nrc());
     nn->r = r;
-    nn->cfg = cfg ? *cfg : vh_nrc_config{65536, 8, 1.0e-2f, 1, 0.95f, 16};
+    nn->cfg = cfg ? *cfg : vh_nrc_config{65536, 8, 1.0e-2f, 1, 0.95f, 16, 0};
     if (nn->cfg.train_paths == 0 || nn->cfg.train_steps == 0)
         return fail(VH_ERR_INVALID_ARGUMENT, "train_paths and train_steps must be positive");
     if (nn->cfg.train_paths > (1u << 24))
@@ -776,7 +793,8 @@ int vh_nrc_accumulate(vh_nrc* nn, vh_accumulation* a, uint32_t frames)
     auto adam = [&](uint32_t tf) -> int {
         const uint32_t b = tf % vh_nrc::kBufs;
         HIPTRY(hipStreamWaitEvent(s, nn->pathsDone[b], 0));
-        TRY(train_steps(nn, nn->samples[b].as<float4>(), nn->sampleCount[b].as<uint32_t>(), 0, sampleCap, nn->cfg.train_steps, true, s));
+        TRY(train_steps(nn, nn->samples[b].as<float4>(), nn->sampleCount[b].as<uint32_t>(), 0, sampleCap, nn->cfg.train_steps, true, s,
+                        nn->cfg.minibatch ? nn->cfg.train_steps : 1u));
         HIPTRY(hipEventRecord(nn->adamDone[b], s));
         nn->adamIssued[b] = true;
         return VH_OK;
