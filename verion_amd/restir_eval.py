@@ -1,0 +1,97 @@
+"""ReSTIR on the many-lights workload: per-frame cost and error against the plain wavefront tracer.
+
+Scene: scenes.many_lights() (ground + 41,000 boxes, 502,252 triangles, 10,000 emissive triangles of
+lognormal(0, 1) emission). Every method renders 1 spp per frame into an FP64 film at 1920x1080;
+the reference is a plain render with `ref_spp` samples and an independent seed. Methods: plain
+(deferred shadow rays), ReSTIR DI (`candidates` initial candidates, the reservoirs of the pixel
+and two random neighbours within 6 px of the previous frame, M capped at `history`:
+pt_core.slang:702-900), and DI + GI (pt_core.slang:901-980, 1566-1617). Each advances along a
+doubling frame ladder to `frames` and is scored by relMSE after every rung; times are the GPU
+seconds the library reports. Reported per method: ms per frame, relMSE at `frames` frames, and
+the plain tracer's time to reach each method's error at `frames` frames (log-log interpolation),
+as a ratio to the method's own time (above 1: the method gets there sooner).
+
+    python -m verion_amd.restir_eval [--frames 64] [--ref-spp 1024] [--candidates 32]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import quality, scenes
+from .backend import WAVEFRONT, Context, Renderer, Restir
+
+
+def evaluate(width: int = 1920, height: int = 1080, frames: int = 64, ref_spp: int = 1024, candidates: int = 32,
+             history: int = 4, seed: int = 1, log=lambda *a: None) -> dict:
+    ctx = Context()
+    r = Renderer(ctx, scenes.many_lights())
+    r.set_integrator(WAVEFRONT)
+    t0 = time.perf_counter()
+    r.set_seed(0x5EED0000 + seed)
+    ref_acc = r.new_accumulation(width, height)
+    done = 0
+    while done < ref_spp:
+        k = min(128, ref_spp - done)
+        r.accumulate(ref_acc, k)
+        done += k
+    ref = ref_acc.mean().astype(np.float64)
+    ref_s = ref_acc.gpu_seconds()
+    ref_acc.close()
+    log(f"reference {ref_spp} spp: {ref_s:.1f} s GPU ({time.perf_counter() - t0:.1f} s wall)")
+    r.set_seed(seed)
+
+    def run(restir, max_frames):
+        r.set_restir(restir)
+        acc = r.new_accumulation(width, height)
+
+        def step(k):
+            before = acc.gpu_seconds()
+            r.accumulate(acc, k)
+            return acc.gpu_seconds() - before
+
+        pts = quality.ladder(step, acc.mean, ref, max_frames)
+        acc.close()
+        r.set_restir(None)
+        return pts
+
+    plain = run(None, frames * 16)
+    log("plain", [(f, round(t, 4), e) for f, t, e in plain])
+    at = {f: (t, e) for f, t, e in plain}
+    out = {"plain": {"ms_per_frame": 1e3 * at[frames][0] / frames, "relmse": at[frames][1],
+                     "ladder": [[f, round(t, 5), e] for f, t, e in plain]}}
+    curve = [(t, e) for _, t, e in plain]
+    for name, restir in (("restir_di", Restir(di=True, gi=False, candidates=candidates, history=history)),
+                         ("restir_di_gi", Restir(di=True, gi=True, candidates=candidates, history=history))):
+        pts = run(restir, frames)
+        log(name, [(f, round(t, 4), e) for f, t, e in pts])
+        t, e = pts[-1][1], pts[-1][2]
+        tp = quality.time_to(curve, e)
+        out[name] = {"ms_per_frame": 1e3 * t / pts[-1][0], "relmse": e, "plain_seconds_to_this_error": tp,
+                     "speedup_to_equal_error": (tp / t) if tp else None,
+                     "ladder": [[f, round(tt, 5), ee] for f, tt, ee in pts]}
+    return {"scene": "many_lights (502,252 triangles, 10,000 emitters)", "width": width, "height": height,
+            "frames": frames, "reference_spp": ref_spp, "candidates": candidates, "history": history,
+            "relmse": "mean over pixels and channels of (x-r)^2/(r^2+0.01)", **out}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--frames", type=int, default=64)
+    ap.add_argument("--ref-spp", type=int, default=1024)
+    ap.add_argument("--candidates", type=int, default=32)
+    ap.add_argument("--history", type=int, default=4)
+    a = ap.parse_args()
+    res = evaluate(a.width, a.height, a.frames, a.ref_spp, a.candidates, a.history,
+                   log=lambda *x: print(*x, file=sys.stderr, flush=True))
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()
