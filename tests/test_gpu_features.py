@@ -152,6 +152,24 @@ def test_restir_di_matches_oracle(many, integrator):
     assert rel(acc.film(), st["film"]) <= TOL, f"exact {np.mean(acc.film() == st['film']):.4f}"
 
 
+@pytest.mark.parametrize("integrator", [MEGAKERNEL, WAVEFRONT])
+def test_restir_di_with_the_lighting_split_matches_oracle(many, integrator):
+    # the lighting split under ReSTIR DI (pt_core.slang:1306-1332: the reservoir's light is tracked
+    # in the first-hit slots); GI stays off with the split (:1296), even when asked for
+    r, o = many
+    r.set_integrator(integrator)
+    r.set_restir(Restir(di=True, gi=True, candidates=8, history=4))
+    acc = r.new_accumulation(64, 48, lighting=True)
+    for _ in range(3):
+        r.accumulate(acc, 1)
+    st = o.render(params(r.packed, 64, 48, lighting=True, restir=(True, True, 8, 4)), 3)
+    r.set_restir(None)
+    r.set_integrator(MEGAKERNEL)
+    assert rel(acc.film(), st["film"]) <= TOL, f"exact {np.mean(acc.film() == st['film']):.4f}"
+    got = acc.lighting_sums()
+    assert np.abs(got - st["lighting"]).max() <= TOL * np.maximum(np.abs(st["lighting"]), 1.0).max()
+
+
 def test_restir_integrators_agree(many):
     r, _ = many
     r.set_restir(Restir(candidates=8))
