@@ -27,7 +27,6 @@ namespace vh {
 
 namespace {
 
-constexpr uint32_t kLeafMax = 4;
 constexpr uint32_t kRsBlock = 256;
 constexpr uint32_t kRsItems = 8;
 constexpr uint32_t kRsTile = kRsBlock * kRsItems;
