@@ -20,7 +20,11 @@
 // instances, deformed meshes: UpdatePlan's TLAS and BLAS rebuilds, pack.rs:758-821) refits the
 // boxes bottom-up in place instead; a refit whose cost grows past kRefitLimit times the built
 // tree's is rebuilt. Closest hits do not depend on the tree (ties go to the smallest id), so
-// both give the same image.
+// both give the same image. Before the collapse reads the binary tree, k_tree_check verifies that
+// it partitions the leaves (a malformed tree fails the build instead of reaching the traversal).
+// Tried and dropped: PLOC clustering (Meister and Bittner, TVCG 2018) over the Morton order in
+// place of the radix tree: 13-18% lower binary surface-area cost and 6% faster shadow rays, but
+// 19% slower closest-hit rays on the 1080p procedural scene (421 against 451 Msamples/s).
 #include "vh_internal.h"
 
 namespace vh {
@@ -246,6 +250,7 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const uint32_t* __restr
     }
 }
 
+
 // 5. Karras hierarchy over the sorted keys. Inner node i has children encoded as index | 0x80000000
 // for a leaf; parents of inner nodes in parent[0..n-2], of leaves in parent[n-1 .. 2n-2].
 __device__ __forceinline__ int delta(const uint32_t* keys, int n, int i, int j)
@@ -313,6 +318,47 @@ __device__ __forceinline__ void triBox(const float4* tris, uint32_t k, float lo[
     hi[1] = fmaxf(a.y, fmaxf(b.y, c.y));
     hi[2] = fmaxf(a.z, fmaxf(b.z, c.z));
 }
+
+// The binary tree the collapse reads must partition the leaves 0 .. n-1: child references in
+// range, each inner node but the root the child of exactly one node, and each node's range the
+// two adjacent ranges of its children. Any violation sets *err.
+__global__ __launch_bounds__(256) void k_tree_check(const uint2* __restrict__ child, const uint2* __restrict__ range, uint32_t n,
+                                                    uint32_t* __restrict__ parents, uint32_t* __restrict__ err)
+{
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n - 1; q += gridDim.x * blockDim.x) {
+        uint2 c = child[q], r = range[q];
+        uint32_t cs[2] = {c.x, c.y};
+        uint2 cr[2];
+        bool bad = r.x > r.y || r.y >= n;
+        for (int k = 0; k < 2; k++) {
+            uint32_t ref = cs[k];
+            if (ref & 0x80000000u) {
+                uint32_t leaf = ref & 0x7fffffffu;
+                bad |= leaf >= n;
+                cr[k] = make_uint2(leaf, leaf);
+            } else {
+                bad |= ref == 0 || ref >= n - 1;
+                if (ref != 0 && ref < n - 1) {
+                    atomicAdd(&parents[ref], 1u);
+                    cr[k] = range[ref];
+                } else {
+                    cr[k] = make_uint2(0u, 0u);
+                }
+            }
+        }
+        bad |= cr[0].x != r.x || cr[1].y != r.y || cr[0].y + 1 != cr[1].x;
+        if (bad)
+            atomicOr(err, 1u);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tree_check_parents(const uint32_t* __restrict__ parents, uint32_t n, uint32_t* __restrict__ err)
+{
+    for (uint32_t q = 1 + blockIdx.x * blockDim.x + threadIdx.x; q < n - 1; q += gridDim.x * blockDim.x)
+        if (parents[q] != 1)
+            atomicOr(err, 2u);
+}
+
 
 // 6. boxes of inner nodes, bottom-up; box[i] = (lo.xyz, -), (hi.xyz, -)
 __global__ __launch_bounds__(256) void k_boxes(const float4* __restrict__ tris, int n, const uint2* __restrict__ child,
@@ -639,6 +685,28 @@ static void apply_bounds(BvhBuild& b, const uint32_t host_bnd[8])
     }
 }
 
+// Structural check of the binary tree before the collapse reads it (k_tree_check).
+static int check_tree(const DevBuf& child, const DevBuf& range, uint32_t n, hipStream_t s)
+{
+    DevBuf parents, err;
+    VH_CHECK(parents.alloc(4 * (size_t)(n - 1)));
+    VH_CHECK(err.alloc(4));
+    k_fill_u32<<<grid_for(n - 1, 256), 256, 0, s>>>(parents.as<uint32_t>(), n - 1, 0u);
+    k_fill_u32<<<1, 64, 0, s>>>(err.as<uint32_t>(), 1, 0u);
+    k_tree_check<<<grid_for(n - 1, 256), 256, 0, s>>>(child.as<uint2>(), range.as<uint2>(), n, parents.as<uint32_t>(),
+                                                      err.as<uint32_t>());
+    k_tree_check_parents<<<grid_for(n - 1, 256), 256, 0, s>>>(parents.as<uint32_t>(), n, err.as<uint32_t>());
+    VH_CHECK(hipGetLastError());
+    uint32_t e = 0;
+    uint2 root;
+    VH_CHECK(hipMemcpyAsync(&e, err.p, 4, hipMemcpyDeviceToHost, s));
+    VH_CHECK(hipMemcpyAsync(&root, range.p, sizeof(root), hipMemcpyDeviceToHost, s));
+    VH_CHECK(hipStreamSynchronize(s));
+    if (e != 0 || root.x != 0 || root.y != n - 1)
+        return fail(VH_ERR_KERNEL, "BVH hierarchy failed its structural check (" + std::to_string(e) + ")");
+    return VH_OK;
+}
+
 int refit_bvh(BvhBuild& b, const DeviceTables& T, hipStream_t s, double& ratio)
 {
     ratio = 1.0;
@@ -727,14 +795,15 @@ int build_bvh(BvhBuild& out, const DeviceTables& T, hipStream_t s)
     }
     VH_CHECK(child.alloc(sizeof(uint2) * (n - 1)));
     VH_CHECK(range.alloc(sizeof(uint2) * (n - 1)));
+    VH_CHECK(box.alloc(sizeof(float4) * 2 * (size_t)(n - 1)));
     VH_CHECK(parent.alloc(4 * (size_t)(2 * n - 1)));
     VH_CHECK(flags.alloc(4 * (size_t)(n - 1)));
-    VH_CHECK(box.alloc(sizeof(float4) * 2 * (size_t)(n - 1)));
     k_fill_u32<<<grid_for(n - 1, 256), 256, 0, s>>>(flags.as<uint32_t>(), n - 1, 0u);
     k_karras<<<grid_for(n - 1, 256), 256, 0, s>>>(kin, (int)n, child.as<uint2>(), range.as<uint2>(), parent.as<uint32_t>());
     k_boxes<<<grid_for(n, 256), 256, 0, s>>>(out.tris.as<float4>(), (int)n, child.as<uint2>(), parent.as<uint32_t>(),
                                              flags.as<uint32_t>(), box.as<float4>());
     VH_CHECK(hipGetLastError());
+    TRY(check_tree(child, range, n, s));
 
     // level-by-level collapse; wide nodes are at most n - 1 (each holds at least two children)
     uint32_t maxWide = n;
