@@ -29,6 +29,7 @@ constexpr int kIn = 96, kHid = 64, kOut = 16, kNL = 6;
 constexpr int kRows = 64;   // rows per block tile (4 waves x 16)
 constexpr int kLdX = 104;   // padded leading dimension of 96-wide rows in LDS
 constexpr int kLdH = 72;    // padded leading dimension of 64-wide rows in LDS
+constexpr int kLdT = 72;    // padded leading dimension of transposed tiles (64 rows of samples) in LDS
 
 __host__ __device__ constexpr int layerK(int l) { return l == 0 ? kIn : kHid; }
 __host__ __device__ constexpr int layerN(int l) { return l == kNL - 1 ? kOut : kHid; }
@@ -173,36 +174,37 @@ __global__ __launch_bounds__(256) void k_nrc_infer(NrcNet net, const float4* rec
 // One layer of the backward pass for a 64-row tile: dW (and db) accumulate into this wave's tiles,
 // then (l > 0) dZ of the layer below = (dZ W^T) * relu'(input).
 template <int L, int NT>
-__device__ __forceinline__ void backLayer(const NrcNet& net, const __bf16* hin, int ldin, __bf16* dz, __bf16* dzNext, __bf16* xt,
-                                          __bf16* dzt, f32x4_t (&acc)[NT], float& db)
+__device__ __forceinline__ void backLayer(const NrcNet& net, const __bf16* hinT, const __bf16* dz, const __bf16* dzT, __bf16* dzNext,
+                                          __bf16* dzNextT, f32x4_t (&acc)[NT], float& db)
 {
+    // Layer L's backward over one 64-row tile. hinT: the layer's input transposed ([k][row]); dz
+    // and dzT: its output gradient, row-major and transposed; writes the input's gradient (through
+    // the ReLU of the layer below) in both layouts. The forward pass stores every activation
+    // transposed as well, so the weight gradient hinT x dz reads MFMA operands straight from LDS.
     constexpr int K = layerK(L), N = layerN(L), NP = layerNpad(L);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
-    // transposes: xt[k][i] = hin[i][k], dzt[n][i] = dz[i][n]
-    for (int e = threadIdx.x; e < K * kRows; e += 256) {
-        int k = e / kRows, i = e % kRows;
-        xt[k * kLdH + i] = hin[i * ldin + k];
-    }
-    for (int e = threadIdx.x; e < N * kRows; e += 256) {
-        int nn = e / kRows, i = e % kRows;
-        dzt[nn * kLdH + i] = dz[i * kLdH + nn];
-    }
-    __syncthreads();
     constexpr int NTILE = N / 16;
 #pragma unroll
     for (int j = 0; j < NT; j++) {
         int t = wave + 4 * j;
         int kt = t / NTILE, nt = t % NTILE;
-        acc[j] = mma16(xt + kt * 16 * kLdH, kLdH, dzt + nt * 16 * kLdH, kLdH, kRows, acc[j]);
+        acc[j] = mma16(hinT + kt * 16 * kLdT, kLdT, dzT + nt * 16 * kLdT, kLdT, kRows, acc[j]);
     }
-    if ((int)threadIdx.x < N) {
+    if ((int)threadIdx.x < N) { // the bias gradient: one contiguous row of dzT
+        const bf16x8_t* row = reinterpret_cast<const bf16x8_t*>(dzT + threadIdx.x * kLdT);
         float s = 0.0f;
-        for (int i = 0; i < kRows; i++)
-            s += (float)dz[i * kLdH + threadIdx.x];
+#pragma unroll
+        for (int q = 0; q < kRows / 8; q++) {
+            bf16x8_t v = row[q];
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                s += (float)v[e];
+        }
         db += s;
     }
     if constexpr (L > 0) {
         const __bf16* W = net.wb + wbOff(L);
+#pragma unroll
         for (int kt = 0; kt < K / 16; kt++) {
             f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
             c = mma16(dz + wave * 16 * kLdH, kLdH, W + kt * 16 * NP, NP, NP, c);
@@ -210,8 +212,10 @@ __device__ __forceinline__ void backLayer(const NrcNet& net, const __bf16* hin, 
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 int row = wave * 16 + g * 4 + t;
-                float h = (float)hin[row * ldin + col];
-                dzNext[row * kLdH + col] = (__bf16)(h > 0.0f ? c[t] : 0.0f);
+                float h = (float)hinT[col * kLdT + row];
+                __bf16 v = (__bf16)(h > 0.0f ? c[t] : 0.0f);
+                dzNext[row * kLdH + col] = v;
+                dzNextT[col * kLdT + row] = v;
             }
         }
     }
@@ -249,13 +253,14 @@ __host__ __device__ __forceinline__ void sliceRange(uint32_t n, uint32_t slice, 
 __global__ __launch_bounds__(256) void k_nrc_train(NrcNet net, const float4* samples, const uint32_t* countPtr, uint32_t countImm,
                                                    uint32_t cap, uint32_t slice, uint32_t slices, float* partials, float* lossPartials)
 {
-    __shared__ __attribute__((aligned(16))) __bf16 s_x[kRows * kLdX];
-    __shared__ __attribute__((aligned(16))) __bf16 s_h[5][kRows * kLdH];
+    __shared__ __attribute__((aligned(16))) __bf16 s_x[kRows * kLdX];      // encoded inputs, row-major
+    __shared__ __attribute__((aligned(16))) __bf16 s_xT[kIn * kLdT];       // and transposed
+    __shared__ __attribute__((aligned(16))) __bf16 s_hr[2][kRows * kLdH];  // hidden activations, row-major (ping-pong)
+    __shared__ __attribute__((aligned(16))) __bf16 s_hT[5][kHid * kLdT];   // every hidden activation, transposed
     __shared__ float s_y[kRows][kOut];
     __shared__ float s_t[kRows][4];
     __shared__ __attribute__((aligned(16))) __bf16 s_dz[2][kRows * kLdH];
-    __shared__ __attribute__((aligned(16))) __bf16 s_xt[kIn * kLdH];
-    __shared__ __attribute__((aligned(16))) __bf16 s_dzt[kHid * kLdH];
+    __shared__ __attribute__((aligned(16))) __bf16 s_dzT[2][kHid * kLdT];
     __shared__ float s_loss[256];
     uint32_t total = countPtr ? *countPtr : countImm;
     if (total > cap)
@@ -274,32 +279,36 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcNet net, const float4* sam
     a5[0] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
     float lossAcc = 0.0f;
     for (uint32_t tile = first + blockIdx.x * kRows; tile < n; tile += gridDim.x * kRows) {
-        // encode
-        uint32_t q = tile + wave * 16 + r;
-        __bf16* xr = s_x + (wave * 16 + r) * kLdX + g * 24;
+        // encode, row-major and transposed
+        const int row = wave * 16 + r;
+        uint32_t q = tile + row;
+        __bf16* xr = s_x + row * kLdX + g * 24;
         if (q < n) {
             const float4* s = samples + (size_t)q * 4;
             encodePart(net, s[0], s[1], s[2], g, xr);
             if (g == 0) {
                 float4 t = s[3];
-                s_t[wave * 16 + r][0] = t.x;
-                s_t[wave * 16 + r][1] = t.y;
-                s_t[wave * 16 + r][2] = t.z;
-                s_t[wave * 16 + r][3] = 1.0f;
+                s_t[row][0] = t.x;
+                s_t[row][1] = t.y;
+                s_t[row][2] = t.z;
+                s_t[row][3] = 1.0f;
             }
         } else {
             for (int k = 0; k < 24; k++)
                 xr[k] = (__bf16)0.0f;
             if (g == 0)
-                s_t[wave * 16 + r][3] = 0.0f;
+                s_t[row][3] = 0.0f;
         }
+        for (int k = 0; k < 24; k++)
+            s_xT[(g * 24 + k) * kLdT + row] = xr[k];
         __syncthreads();
         // forward; B operands stream from the global bf16 copy (L2-resident, 47 KB)
 #pragma unroll
         for (int l = 0; l < kNL; l++) {
             const int K = layerK(l), N = layerN(l);
-            const __bf16* in = l == 0 ? s_x : s_h[l - 1];
+            const __bf16* in = l == 0 ? s_x : s_hr[(l - 1) & 1];
             const int ldin = l == 0 ? kLdX : kLdH;
+#pragma unroll
             for (int nt = 0; nt < N / 16; nt++) {
                 f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
                 c = mma16(in + wave * 16 * ldin, ldin, net.wt + wtOff(l) + nt * 16 * K, K, K, c);
@@ -307,11 +316,14 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcNet net, const float4* sam
                 float bias = net.theta[bOff(l) + col];
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
-                    int row = wave * 16 + g * 4 + t;
-                    if (l < kNL - 1)
-                        s_h[l][row * kLdH + col] = (__bf16)fmaxf(c[t] + bias, 0.0f);
-                    else
-                        s_y[row][col] = c[t] + bias;
+                    int rw = wave * 16 + g * 4 + t;
+                    if (l < kNL - 1) {
+                        __bf16 v = (__bf16)fmaxf(c[t] + bias, 0.0f);
+                        s_hr[l & 1][rw * kLdH + col] = v;
+                        s_hT[l][col * kLdT + rw] = v;
+                    } else {
+                        s_y[rw][col] = c[t] + bias;
+                    }
                 }
             }
             __syncthreads();
@@ -329,15 +341,17 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcNet net, const float4* sam
                     lossAcc += (y - t) * (y - t) / den;
                 }
                 s_dz[0][i * kLdH + c] = (__bf16)gv;
+                if (c < kOut)
+                    s_dzT[0][c * kLdT + i] = (__bf16)gv;
             }
         }
         __syncthreads();
-        backLayer<5, 1>(net, s_h[4], kLdH, s_dz[0], s_dz[1], s_xt, s_dzt, a5, d5);
-        backLayer<4, 4>(net, s_h[3], kLdH, s_dz[1], s_dz[0], s_xt, s_dzt, a4, d4);
-        backLayer<3, 4>(net, s_h[2], kLdH, s_dz[0], s_dz[1], s_xt, s_dzt, a3, d3);
-        backLayer<2, 4>(net, s_h[1], kLdH, s_dz[1], s_dz[0], s_xt, s_dzt, a2, d2);
-        backLayer<1, 4>(net, s_h[0], kLdH, s_dz[0], s_dz[1], s_xt, s_dzt, a1, d1);
-        backLayer<0, 6>(net, s_x, kLdX, s_dz[1], s_dz[0], s_xt, s_dzt, a0, d0);
+        backLayer<5, 1>(net, s_hT[4], s_dz[0], s_dzT[0], s_dz[1], s_dzT[1], a5, d5);
+        backLayer<4, 4>(net, s_hT[3], s_dz[1], s_dzT[1], s_dz[0], s_dzT[0], a4, d4);
+        backLayer<3, 4>(net, s_hT[2], s_dz[0], s_dzT[0], s_dz[1], s_dzT[1], a3, d3);
+        backLayer<2, 4>(net, s_hT[1], s_dz[1], s_dzT[1], s_dz[0], s_dzT[0], a2, d2);
+        backLayer<1, 4>(net, s_hT[0], s_dz[0], s_dzT[0], s_dz[1], s_dzT[1], a1, d1);
+        backLayer<0, 6>(net, s_xT, s_dz[1], s_dzT[1], s_dz[0], s_dzT[0], a0, d0);
     }
     float* part = partials + (size_t)blockIdx.x * kParams;
     storeLayer<0, 6>(part, a0, d0);
