@@ -168,3 +168,25 @@ def test_minibatch_steps_take_disjoint_slices(ctx):
     acc = r.new_accumulation(64, 64)
     c.accumulate(acc, 4)
     assert np.all(np.isfinite(acc.mean())) and c.stats()["samples"] > 0
+
+
+def test_training_paths_defer_their_shadow_rays_without_changing_the_samples(ctx):
+    # The same training paths with their shadow rays deferred (at most 7 delta lights) and traced
+    # inline (8 more point lights of zero intensity force the fused shading): a frame's training
+    # paths leave the same number of samples, and training on them gives the same loss
+    from verion_amd import scene as sc
+    base = scenes.cornell()
+    fused = scenes.cornell()
+    for k in range(8):
+        fused.add_light(sc.PointLight(position=(0.1 * k, 0.5, 0.5), color=(1.0, 1.0, 1.0), intensity=0.0))
+    stats = []
+    for s in (base, fused):
+        r = Renderer(ctx, s)
+        r.set_integrator(WAVEFRONT)
+        c = Nrc(r, train_paths=4096, train_steps=1, warmup_frames=0)
+        c.set_params(nrc.init(10))
+        acc = r.new_accumulation(32, 32)
+        c.accumulate(acc, 1)
+        stats.append(c.stats())
+    assert stats[0]["samples"] == stats[1]["samples"] > 0, stats
+    assert abs(stats[0]["loss"] - stats[1]["loss"]) <= 1e-4 * abs(stats[1]["loss"]), stats

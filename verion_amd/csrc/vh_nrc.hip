@@ -504,6 +504,8 @@ struct vh_nrc {
     // pathsDone[b] / adamDone[b] order its writes and reads between the streams.
     struct TrainSet {
         DevBuf trainPix, vertices, state0, state1, hits, counts;
+        // deferred shadow rays of the training paths (kMaxShadow records per path)
+        DevBuf shRec, shVis, shQueue, shNRec, shOut, shPixel, shTotal;
         hipStream_t stream = nullptr;
     };
     static constexpr uint32_t kSets = 2, kBufs = 3;
@@ -661,6 +663,13 @@ int vh_nrc_create(vh_renderer* r, const vh_nrc_config* cfg, vh_nrc** out)
         HIPTRY(t.state1.alloc(4ull * 17 * tp));
         HIPTRY(t.hits.alloc(16 * tp));
         HIPTRY(t.counts.alloc(4 * kCountWords));
+        HIPTRY(t.shRec.alloc(4ull * kShadowWords * kMaxShadow * tp));
+        HIPTRY(t.shVis.alloc(4ull * kMaxShadow * tp));
+        HIPTRY(t.shQueue.alloc(4ull * kMaxShadow * tp));
+        HIPTRY(t.shNRec.alloc(4 * tp));
+        HIPTRY(t.shOut.alloc(4 * tp));
+        HIPTRY(t.shPixel.alloc(4 * tp));
+        HIPTRY(t.shTotal.alloc(12 * tp));
         HIPTRY(hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking));
     }
     for (uint32_t b = 0; b < vh_nrc::kBufs; b++) {
@@ -796,6 +805,16 @@ int vh_nrc_accumulate(vh_nrc* nn, vh_accumulation* a, uint32_t frames)
         T.counts = ts.counts.as<uint32_t>();
         T.n = nn->cfg.train_paths;
         T.words = 17;
+        if (r->info.light_count + 1 <= kMaxShadow) { // shadow rays deferred to the cooperative shadow kernel
+            T.shK = kMaxShadow;
+            T.shRec = ts.shRec.as<float>();
+            T.shVis = ts.shVis.as<uint32_t>();
+            T.shQueue = ts.shQueue.as<uint32_t>();
+            T.shNRec = ts.shNRec.as<uint32_t>();
+            T.shOut = ts.shOut.as<uint32_t>();
+            T.shPixel = ts.shPixel.as<uint32_t>();
+            T.shTotal = ts.shTotal.as<float>();
+        }
         Params PT = base;
         PT.w[6] = 0x40000000u + tf;
         TRY(launch_nrc_train_paths(S, PT, T, ts.trainPix.as<uint32_t>(), nn->cfg.train_paths, (uint32_t)npix, tf,

@@ -357,6 +357,8 @@ __global__ void k_nrc_pick(uint32_t* trainPix, uint32_t nTrain, uint32_t npix, u
         counts[0] = nTrain;
         for (uint32_t k = 1; k <= kMaxDepthNrc; k++)
             counts[k] = 0;
+        for (uint32_t k = 0; k < kMaxDepthNrc; k++)
+            counts[kShadowCounts + k] = 0;
         *sampleCount = 0;
     }
 }
@@ -452,6 +454,130 @@ __global__ __launch_bounds__(kWfBlock) void k_nrc_t_shade(Scene S, Params P, Wav
         uint32_t slot = compact(keep, &W.counts[d + 1]);
         if (keep && slot < W.n)
             storePath<false>(out, W.n, slot, st, tp);
+    }
+}
+
+// The training samples of finished training path tp with final total T: one per recorded
+// vertex, target (T - total at the vertex) / throughput at the vertex. Every thread of the block
+// calls it (block-wide compaction into the sample buffer).
+__device__ __forceinline__ void emitSamples(bool finished, uint32_t tp, V3 T, const float4* vertices, float4* samples,
+                                            uint32_t* sampleCount, uint32_t sampleCap)
+{
+    uint32_t nvalid = 0;
+    const float4* vrec = vertices + (size_t)tp * kMaxDepthNrc * kNrcVertexWords;
+    if (finished && !bad(T)) {
+        for (uint32_t j = 0; j < kMaxDepthNrc; j++)
+            nvalid += vrec[j * kNrcVertexWords + 4].w != 0.0f ? 1u : 0u;
+    }
+    uint32_t first = compactN(nvalid, sampleCount);
+    if (nvalid) {
+        uint32_t o = first;
+        for (uint32_t j = 0; j < kMaxDepthNrc && o < sampleCap; j++) {
+            const float4* v = vrec + j * kNrcVertexWords;
+            if (v[4].w == 0.0f)
+                continue;
+            float4 thr = v[3], tae = v[4];
+            float4 tgt = make_float4(thr.x > 1.0e-6f ? (T.x - tae.x) / thr.x : 0.0f, thr.y > 1.0e-6f ? (T.y - tae.y) / thr.y : 0.0f,
+                                     thr.z > 1.0e-6f ? (T.z - tae.z) / thr.z : 0.0f, 1.0f);
+            float4* sp = samples + (size_t)o * 4;
+            sp[0] = v[0];
+            sp[1] = v[1];
+            sp[2] = v[2];
+            sp[3] = tgt;
+            o++;
+        }
+    }
+}
+
+// Training paths with deferred shadow rays (as k_wf_shade_deferred for the render): shade and
+// record the vertices; a path that ends without pending shadow rays gives its samples now, one
+// with pending rays gives them in k_nrc_t_resolve once its total is final.
+__global__ __launch_bounds__(kWfBlock, VH_SHADE_WAVES) void k_nrc_t_shade_deferred(Scene S, Params P, Wavefront W, const uint32_t* trainPix,
+                                                                                 float4* vertices, float4* samples, uint32_t* sampleCount,
+                                                                                 uint32_t sampleCap, uint32_t d)
+{
+    Frame F{};
+    Ctx c{S, F, P};
+    uint32_t count = W.counts[d];
+    const uint32_t* in = W.state[d & 1u];
+    uint32_t* out = W.state[(d + 1) & 1u];
+    for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += gridDim.x * kWfBlock) {
+        uint32_t i = base + threadIdx.x;
+        bool keep = false;
+        PathState st;
+        uint32_t tp = 0, nrec = 0;
+        if (i < count) {
+            if (d == 0) {
+                tp = i;
+                uint32_t p = trainPix[i];
+                generatePath(P, p % P.w[0], p / P.w[0], P.w[6], st);
+                for (uint32_t j = 0; j < kMaxDepthNrc; j++)
+                    vertices[((size_t)tp * kMaxDepthNrc + j) * kNrcVertexWords + 4].w = 0.0f;
+            } else {
+                loadPath<false>(in, W.n, i, st, tp);
+            }
+            uint4 hw = W.hits[i];
+            Hit h;
+            h.hit = hw.x != 0;
+            h.inst = hw.y;
+            h.prim = hw.z;
+            h.t = asf(hw.w);
+            h.b1 = h.b2 = 0.0f;
+            h.gid = 0;
+            NrcIo io{vertices, nullptr, tp, i, 0u, false};
+            ShadowSink sink{W.shRec, W.n, W.shK, i, 0u};
+            keep = shadePath<false, false, 1, true>(c, st, h, d, tp, &io, &sink) && d + 1 < P.w[7];
+            nrec = sink.count;
+            W.shNRec[i] = nrec;
+        }
+        emitSamples(i < count && !keep && nrec == 0, tp, i < count ? st.parts.total : v3(0.0f), vertices, samples, sampleCount,
+                    sampleCap);
+        uint32_t slot = compact(keep, &W.counts[d + 1]);
+        if (keep && slot < W.n)
+            storePath<false>(out, W.n, slot, st, tp);
+        uint32_t q = compactN(nrec, &W.counts[kShadowCounts + d]);
+        if (nrec > 0) {
+            W.shOut[i] = keep ? slot : 0xffffffffu;
+            W.shPixel[i] = tp;
+            put3(reinterpret_cast<uint32_t*>(W.shTotal), W.n, 0, i, st.parts.total);
+            for (uint32_t k = 0; k < nrec; k++)
+                if (q + k < W.shK * W.n)
+                    W.shQueue[q + k] = k * W.n + i;
+        }
+    }
+}
+
+// k_wf_resolve for training paths: the visible shadow contributions in record order, then the
+// total goes to the path's next slot or, for a finished path, into its training samples.
+__global__ __launch_bounds__(kWfBlock) void k_nrc_t_resolve(Wavefront W, float4* vertices, float4* samples, uint32_t* sampleCount,
+                                                           uint32_t sampleCap, uint32_t d)
+{
+    const uint32_t count = W.counts[d];
+    uint32_t* out = W.state[(d + 1) & 1u];
+    const size_t stride = (size_t)W.shK * W.n;
+    for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += gridDim.x * kWfBlock) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t nrec = i < count ? W.shNRec[i] : 0u;
+        V3 total = v3(0.0f);
+        bool finished = false;
+        uint32_t tp = 0;
+        if (nrec > 0) {
+            total = get3(reinterpret_cast<const uint32_t*>(W.shTotal), W.n, 0, i);
+            for (uint32_t k = 0; k < nrec; k++) {
+                uint32_t rid = k * W.n + i;
+                if (W.shVis[rid] == 0) {
+                    const float* r = W.shRec + rid;
+                    total = total + v3(r[7 * stride], r[8 * stride], r[9 * stride]);
+                }
+            }
+            uint32_t os = W.shOut[i];
+            tp = W.shPixel[i];
+            if (os != 0xffffffffu)
+                put3(out, W.n, 13, os, total);
+            else
+                finished = true;
+        }
+        emitSamples(finished, tp, total, vertices, samples, sampleCount, sampleCap);
     }
 }
 
@@ -584,7 +710,13 @@ int launch_nrc_train_paths(const Scene& S, const Params& P, const Wavefront& W, 
     k_nrc_pick<<<(nTrain + 255) / 256, 256, 0, s>>>(trainPix, nTrain, npix, frame, W.counts, sampleCount);
     for (uint32_t d = 0; d < P.w[7]; d++) {
         k_nrc_t_extend<<<grid, kWfBlock, 0, s>>>(S, P, W, trainPix, d);
-        k_nrc_t_shade<<<grid, kWfBlock, 0, s>>>(S, P, W, trainPix, vertices, samples, sampleCount, sampleCap, d);
+        if (W.shK > 0) { // deferred shadow rays, traced by the wave-cooperative shadow kernel
+            k_nrc_t_shade_deferred<<<grid, kWfBlock, 0, s>>>(S, P, W, trainPix, vertices, samples, sampleCount, sampleCap, d);
+            k_wf_shadow<<<grid, kWfBlock, 0, s>>>(S, P, W, d);
+            k_nrc_t_resolve<<<grid, kWfBlock, 0, s>>>(W, vertices, samples, sampleCount, sampleCap, d);
+        } else {
+            k_nrc_t_shade<<<grid, kWfBlock, 0, s>>>(S, P, W, trainPix, vertices, samples, sampleCount, sampleCap, d);
+        }
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("NRC training paths: ") + hipGetErrorString(e));

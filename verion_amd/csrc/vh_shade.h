@@ -915,7 +915,7 @@ template <bool RESTIR, bool SPLIT, int NRC = 0, bool DEFER = false>
 __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t depth, uint32_t pixel, NrcIo* io = nullptr,
                           ShadowSink* sink = nullptr)
 {
-    static_assert(!DEFER || (!RESTIR && !SPLIT && NRC == 0), "deferred shadow rays are for the plain integrator");
+    static_assert(!DEFER || (!RESTIR && !SPLIT && NRC != 2), "deferred shadow rays are for the plain integrator and its training paths");
     const Scene& S = c.S;
     const Params& P = c.P;
     V3 sky = v3(P.f(20), P.f(21), P.f(22));
