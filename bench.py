@@ -210,11 +210,36 @@ def main():
         base, parity = cpu_baseline(packed, W, H, gpu1.film())
         line["cpu_baseline"] = base
         line["parity_full_size"] = parity
+    line["cornell_256"] = cornell_256(ctx)
     if not args.no_nrc:
         line["time_to_rmse"] = time_to_rmse(W, H, args.nrc_ref_spp)
     if not args.no_batch:
         line["batch"] = batch_jobs()
     print(json.dumps(line), flush=True)
+
+
+def cornell_256(ctx):
+    """The Cornell box workload (36 triangles, 256x256, 16 spp, depth 8): Msamples/s of both
+    integrators over 64 accumulations of 16 spp each, after 4 untimed ones."""
+    from verion_amd import scenes
+    from verion_amd.backend import MEGAKERNEL, WAVEFRONT, Renderer
+
+    r = Renderer(ctx, scenes.cornell())
+    out = {"workload": "Cornell box, 256x256, 16 spp per accumulation"}
+    for name, integ in (("megakernel", MEGAKERNEL), ("wavefront", WAVEFRONT)):
+        r.set_integrator(integ)
+        for _ in range(4):
+            r.render(256, 256, 16)
+        accs = [r.new_accumulation(256, 256) for _ in range(64)]
+        t0 = time.perf_counter()
+        for a in accs:
+            r.accumulate(a, 16)
+        dt = time.perf_counter() - t0
+        for a in accs:
+            a.close()
+        out[name + "_msamples_per_s"] = round(64 * 256 * 256 * 16 / dt / 1e6, 1)
+    log(f"cornell 256: {out}")
+    return out
 
 
 def batch_jobs():

@@ -671,6 +671,23 @@ int vh_accumulate(vh_renderer* r, vh_accumulation* a, uint32_t samples)
             P.w[31] = (uint32_t)n;
             rc = launch_wf_sample(S, F, P, W, restir, split, cfg, timing ? &stageEv[kStageEvents * kMaxDepth * k] : nullptr);
         }
+    } else if (!restir && !split && !a->adaptiveOn && samples > 1 && n < kParallelPaths) {
+        // a small image: several samples per dispatch in parallel, added to the film in order
+        uint32_t chunk = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(samples, 1024), std::max<uint64_t>(1, kParallelPaths / n));
+        uint64_t bytes = 12ull * n * chunk;
+        if (a->staged.bytes < bytes) {
+            TRY(check_fits(bytes));
+            HIPTRY(a->staged.alloc(bytes));
+        }
+        uint32_t k = 0;
+        while (k < samples && rc == VH_OK) {
+            Params P = base;
+            P.w[6] = a->samples + k;
+            uint32_t m = std::min(samples - k, chunk);
+            P.w[35] = m;
+            rc = launch_megakernel_samples(S, F, P, m, a->staged.as<float>(), s);
+            k += m;
+        }
     } else {
         uint32_t k = 0;
         while (k < samples && rc == VH_OK) {

@@ -59,6 +59,53 @@ __global__ __launch_bounds__(64) void k_pathtrace(Scene S, Frame F, Params P, ui
     }
 }
 
+// The megakernel with the samples of a dispatch in parallel, for images too small to fill the
+// GPU one thread per pixel (256 x 256 is 1,024 waves for 256 CUs): thread (pixel, k), k = blockIdx.z,
+// traces sample P.w[6] + k and stages its radiance (zero when not finite, as finishPath does);
+// k_film_add_samples then adds the staged samples to the FP64 film in sample order, the additions
+// finishPath makes one sample at a time, so the film is the same bit for bit. Plain integrator
+// only (no adaptive statistics, lighting split or ReSTIR, which need the samples in order).
+__global__ __launch_bounds__(64) void k_pathtrace_samples(Scene S, Params P, float* __restrict__ staged)
+{
+    uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7u);
+    uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
+    if (x >= P.w[0] || y >= P.w[1])
+        return;
+    Frame F{};
+    Ctx c{S, F, P};
+    uint32_t pixel = y * P.w[0] + x, k = blockIdx.z;
+    PathState st;
+    generatePath(P, x, y, P.w[6] + k, st);
+    for (uint32_t depth = 0; depth < P.w[7]; depth++) {
+        Hit h = trace<false>(S.bvh, st.org, st.dir, 1.0e30f);
+        if (!shadePath<false, false>(c, st, h, depth, pixel))
+            break;
+    }
+    V3 radiance = bad(st.parts.total) ? v3(0.0f) : st.parts.total;
+    float* o = staged + 3 * ((size_t)k * P.w[0] * P.w[1] + pixel);
+    o[0] = radiance.x;
+    o[1] = radiance.y;
+    o[2] = radiance.z;
+}
+
+__global__ __launch_bounds__(256) void k_film_add_samples(double* __restrict__ film, uint32_t npix, uint32_t samples,
+                                                          const float* __restrict__ staged)
+{
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+        double* f = film + 3 * (size_t)p;
+        double r = f[0], g = f[1], b = f[2];
+        for (uint32_t k = 0; k < samples; k++) {
+            const float* v = staged + 3 * ((size_t)k * npix + p);
+            r += (double)v[0];
+            g += (double)v[1];
+            b += (double)v[2];
+        }
+        f[0] = r;
+        f[1] = g;
+        f[2] = b;
+    }
+}
+
 // wf_args at depth 0 (wf_args.slang:12-17, and its clear of counts[0] at :21-22): the depth-0
 // count (every pixel, or none yet when wf_select fills the queue) and empty later queues
 __global__ void k_wf_reset(uint32_t* counts, uint32_t n, uint32_t adaptive, uint32_t maxDepth)
@@ -654,6 +701,16 @@ int launch_megakernel(const Scene& S, const Frame& F, const Params& P, uint32_t 
         k_pathtrace<false, false><<<grid, 64, 0, s>>>(S, F, P, samples);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("pathtrace launch: ") + hipGetErrorString(e));
+}
+
+int launch_megakernel_samples(const Scene& S, const Frame& F, const Params& P, uint32_t samples, float* staged, hipStream_t s)
+{
+    dim3 grid((P.w[0] + 7) / 8, (P.w[1] + 7) / 8, samples);
+    k_pathtrace_samples<<<grid, 64, 0, s>>>(S, P, staged);
+    uint32_t npix = P.w[0] * P.w[1];
+    k_film_add_samples<<<blocks_for(npix, 256, 4096), 256, 0, s>>>(F.film, npix, samples, staged);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("megakernel samples: ") + hipGetErrorString(e));
 }
 
 int launch_wf_sample(const Scene& S, const Frame& F, const Params& P0, const Wavefront& W, bool restir, bool split, const LaunchCfg& cfg,
