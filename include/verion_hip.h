@@ -101,6 +101,9 @@ typedef struct vh_restir {
                             first bounce's hit; off while the lighting split is on, like the reference */
     uint32_t candidates;
     uint32_t history;
+    uint32_t neighbours; /* random neighbours reused per frame besides the pixel's own reservoir, 0..16;
+                            0 = 2, the reference's fixed RESTIR_NEIGHBOURS (pt_core.slang:706), whose
+                            random stream is then unchanged. No counterpart in the reference. */
 } vh_restir;
 
 /* MemoryUsage (lib.rs:242-249). */
@@ -145,6 +148,13 @@ const char* vh_last_error(void);
 /* Error::OutOfMemory{needed, available} of the last VH_ERR_OUT_OF_MEMORY on this thread. */
 uint64_t vh_last_error_needed(void);
 uint64_t vh_last_error_available(void);
+
+/* Host-only helper (no device work): the cutpoint table the renderer builds over the emitter power
+ * CDF (`emit_cdf`, pack.rs) so that a draw u selects the same emitter as the binary search of
+ * pt_core.slang:1343-1351 while searching only between two cutpoints: guide[k] (k = 0 .. 2^bits)
+ * = the first index whose CDF value reaches k / 2^bits, or n - 1 when none does. `guide` holds
+ * 2^bits + 1 words; bits <= 16; the CDF must be non-decreasing. No counterpart in the reference. */
+int vh_cdf_guide(const float* cdf, uint32_t n, uint32_t bits, uint32_t* guide);
 
 /* --- context (RayTracingBackend::context / memory, lib.rs:19-20) ------------------------------ */
 

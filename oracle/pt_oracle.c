@@ -969,6 +969,14 @@ static path_t generate_path(const ctx_t* c, uint32_t px, uint32_t py, uint32_t s
 #define RESTIR_HALF 28u
 #define RESTIR_STRIDE 80u
 
+/* Random neighbours per reuse: bits 8-15 of the ReSTIR mode word params[36] (an extension of this
+   build; 0 = the reference's fixed RESTIR_NEIGHBOURS = 2, pt_core.slang:706). */
+static inline uint32_t restir_neighbours(const ctx_t* c)
+{
+    uint32_t k = (c->P[36] >> 8) & 0xffu;
+    return k ? k : RESTIR_NEIGHBOURS;
+}
+
 static inline uint32_t restir_base(uint32_t pixel, uint32_t which) { return pixel * RESTIR_STRIDE + which * RESTIR_HALF; }
 
 static int restir_light(const ctx_t* c, uint32_t e, uint32_t recv, f3 x, f3 p, f3 n, f3* wi, float* dist, float* cosi, f3* g, float* plsa)
@@ -1047,7 +1055,7 @@ static restir_pick restir_di(const ctx_t* c, uint32_t pixel, uint32_t recv, f3 p
     }
     if (c->P[6] > 0) {
         int px = (int)(pixel % w), py = (int)(pixel / w);
-        for (uint32_t k = 0; k < 1 + RESTIR_NEIGHBOURS; k++) {
+        for (uint32_t k = 0; k < 1 + restir_neighbours(c); k++) {
             int nx = px, ny = py;
             if (k > 0) {
                 nx += (int)(rng_next(rng) * (float)(2 * RESTIR_RADIUS + 1)) - RESTIR_RADIUS;
@@ -1175,7 +1183,7 @@ static void restir_gi_prepare(const ctx_t* c, uint32_t pixel, f3 p, f3 n, float 
     f3 cSel = v3(0.0f, 0.0f, 0.0f), xSel = cSel, nSel = cSel, lSel = cSel;
     if (c->P[6] > 0) {
         int px = (int)(pixel % w), py = (int)(pixel / w);
-        for (uint32_t k = 0; k < 1 + RESTIR_NEIGHBOURS; k++) {
+        for (uint32_t k = 0; k < 1 + restir_neighbours(c); k++) {
             int nx = px, ny = py;
             if (k > 0) {
                 nx += (int)(rng_next(rng) * (float)(2 * RESTIR_RADIUS + 1)) - RESTIR_RADIUS;

@@ -11,6 +11,7 @@ import pytest
 import oracle
 from helpers import params
 from verion_amd import gltf, scenes
+from verion_amd.scene import PointLight, SpotLight
 from verion_amd.backend import MEGAKERNEL, WAVEFRONT, Renderer, Restir
 
 pytestmark = pytest.mark.gpu
@@ -115,3 +116,49 @@ def test_gi_leaves_no_stains_on_the_cornell_box_walls(ctx):
     block, chroma = worst_block(img, reference), worst_chroma(img, reference)
     assert block < 0.3, f"worst block differs by {block}"
     assert chroma < 0.15, f"worst color balance differs by {chroma}"
+
+
+@pytest.mark.parametrize("integrator", [MEGAKERNEL, WAVEFRONT])
+def test_five_neighbour_reuse_matches_oracle(room, integrator):
+    # the many-lights workload's 5 spatial neighbours (the reference fixes 2, pt_core.slang:706)
+    r, o = room
+    r.set_integrator(integrator)
+    r.set_restir(Restir(di=True, gi=True, candidates=8, history=4, neighbours=5))
+    acc = r.new_accumulation(48, 36)
+    for _ in range(5):
+        r.accumulate(acc, 1)
+    st = o.render(params(r.packed, 48, 36, restir=(True, True, 8, 4, 5)), 5)
+    r.set_restir(None)
+    r.set_integrator(MEGAKERNEL)
+    got = acc.film()
+    assert rel(got, st["film"]) <= TOL, f"exact {np.mean(got == st['film']):.4f}"
+
+
+@pytest.fixture(scope="module")
+def lit_glass_room(ctx):
+    # rough walls, ceiling panels, dispersive glass and two delta lights: past the resampled depths
+    # the wavefront defers every kind of shadow ray (delta lights and emitters), GI paths through
+    # the glass are dropped, and GI paths finish in the shadow-resolve kernel
+    s = scenes.closed_room(0.9, glass=True)
+    s.add_light(PointLight(position=(0.5, 1.0, 0.3), color=(1.0, 0.8, 0.6), intensity=3.0))
+    s.add_light(SpotLight(position=(-0.5, 1.2, -0.2), direction=(0.2, -1.0, 0.1), intensity=4.0))
+    r = Renderer(ctx, s)
+    return r, oracle.OracleScene(r.packed)
+
+
+@pytest.mark.parametrize("di,gi", [(True, False), (True, True), (False, True)])
+def test_wavefront_defers_shadow_rays_past_the_resampling(lit_glass_room, di, gi):
+    r, o = lit_glass_room
+    out = []
+    for integ in (MEGAKERNEL, WAVEFRONT):
+        r.set_integrator(integ)
+        r.set_restir(Restir(di=di, gi=gi, candidates=4, history=4))
+        acc = r.new_accumulation(40, 32)
+        for _ in range(4):
+            r.accumulate(acc, 1)
+        out.append(acc.film())
+    r.set_restir(None)
+    r.set_integrator(MEGAKERNEL)
+    assert np.array_equal(out[0], out[1])
+    st = o.render(params(r.packed, 40, 32, restir=(di, gi, 4, 4)), 4)
+    assert rel(out[1], st["film"]) <= TOL

@@ -38,7 +38,8 @@ class Adaptive(C.Structure):
 
 
 class Restir(C.Structure):
-    _fields_ = [("di", C.c_uint32), ("gi", C.c_uint32), ("candidates", C.c_uint32), ("history", C.c_uint32)]
+    _fields_ = [("di", C.c_uint32), ("gi", C.c_uint32), ("candidates", C.c_uint32), ("history", C.c_uint32),
+                ("neighbours", C.c_uint32)]
 
 
 class NrcConfig(C.Structure):

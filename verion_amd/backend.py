@@ -59,6 +59,7 @@ class Restir:
     gi: bool = False
     candidates: int = 4
     history: int = 4
+    neighbours: int = 0  # random neighbours per reuse; 0 = the reference's 2 (pt_core.slang:706)
 
 
 _lib = None
@@ -322,7 +323,7 @@ class Renderer(_Owned):
         _check(library().vh_set_adaptive(self.h, C.byref(a) if a else None))
 
     def set_restir(self, restir: Optional[Restir]):
-        r = abi.Restir(int(restir.di), int(restir.gi), restir.candidates, restir.history) if restir else None
+        r = abi.Restir(int(restir.di), int(restir.gi), restir.candidates, restir.history, restir.neighbours) if restir else None
         _check(library().vh_set_restir(self.h, C.byref(r) if r else None))
 
     def set_seed(self, seed: int):

@@ -229,9 +229,50 @@ Params base_params(const vh_renderer* r, uint32_t w, uint32_t h, bool lighting, 
         p.w[36] = (re.di ? 1u : 0u) | (re.gi ? 2u : 0u);
         p.w[37] = re.candidates;
         p.w[38] = re.history;
+        if (p.w[36] != 0)
+            p.w[36] |= re.neighbours << 8; // 0 = the reference's 2
     }
     p.seedMix = r->seed == 0 ? 0u : pcg(r->seed ^ 0x5851F42Du);
     return p;
+}
+
+// Cutpoints of a power CDF for emitterIndex: guide[k] = the first index whose value reaches
+// k / 2^bits (n - 1 when none does), k = 0 .. 2^bits. A CDF that is not non-decreasing (never
+// made by the packer) gets bits = 0, i.e. the full binary search.
+uint32_t cdf_guide_bits(uint32_t n)
+{
+    uint32_t bits = 0;
+    while (bits < 16 && (1u << bits) < n)
+        bits++;
+    return bits;
+}
+
+bool cdf_guide(const float* cdf, uint32_t n, uint32_t bits, uint32_t* guide)
+{
+    for (uint32_t i = 0; i + 1 < n; i++)
+        if (!(cdf[i] <= cdf[i + 1]))
+            return false;
+    uint32_t G = 1u << bits, i = 0;
+    for (uint32_t k = 0; k <= G; k++) {
+        float t = (float)k / (float)G; // exact: k <= 2^16, G a power of two
+        while (i + 1 < n && !(cdf[i] >= t))
+            i++;
+        guide[k] = i;
+    }
+    return true;
+}
+
+int upload_emitter_guide(vh_renderer* r, const vh_scene_tables* t)
+{
+    uint32_t n = (uint32_t)t->info.emitter_count;
+    uint32_t bits = n > 1 ? cdf_guide_bits(n) : 0;
+    r->emitGuideHost.assign((1u << bits) + 1, 0u);
+    if (n > 1 && !cdf_guide(t->emit_cdf, n, bits, r->emitGuideHost.data())) {
+        bits = 0;
+        r->emitGuideHost.assign({0u, n - 1});
+    }
+    r->emitGuideBits = bits;
+    return reupload(r->ctx, r->emitGuide, r->emitGuideHost.data(), r->emitGuideHost.size());
 }
 
 int upload_tables(vh_renderer* r, const vh_scene_tables* t)
@@ -256,6 +297,7 @@ int upload_tables(vh_renderer* r, const vh_scene_tables* t)
     TRY(upload(c, r->lights, t->lights, t->lights_len));
     TRY(upload(c, r->emitters, t->emitters, t->emitters_len));
     TRY(upload(c, r->emitCdf, t->emit_cdf, t->emit_cdf_len));
+    TRY(upload_emitter_guide(r, t));
     TRY(upload(c, r->envPixels, t->env_pixels, t->env_pixels_len));
     TRY(upload(c, r->envMarginal, t->env_marginal, t->env_marginal_len));
     TRY(upload(c, r->envCond, t->env_cond, t->env_cond_len));
@@ -285,10 +327,11 @@ int ensure_wavefront(vh_accumulation* a)
         return VH_OK;
     uint64_t n = a->pixels();
     uint32_t words = 17 + (a->lighting ? 17 : 0);
-    // deferred shadow rays for the plain integrator: one record per delta light and one for the
-    // emitter or environment sample; scenes with more lights than kMaxShadow - 1 shade fused
-    uint32_t K = a->r->info.light_count + 1;
-    bool defer = !a->lighting && !a->restirOn && K <= kMaxShadow;
+    // deferred shadow rays (the plain integrator, ReSTIR DI, ReSTIR GI past its first bounce): one
+    // record per delta light, one for the emitter or environment sample and one for ReSTIR DI's
+    // winner; scenes with more lights than that leaves room for, and the lighting split, shade fused
+    uint32_t K = a->shadowRecords();
+    bool defer = !a->lighting && K <= kMaxShadow;
     uint64_t shBytes = defer ? (4ull * (kShadowWords + 2) * K * n + 4ull * 6 * n) : 0;
     uint64_t bytes = 2 * 4 * words * n + 16 * n + 4 * n + 4 * kCountWords + shBytes;
     TRY(check_fits(bytes));
@@ -320,6 +363,15 @@ using namespace vh;
 extern "C" {
 
 const char* vh_last_error(void) { return g_error.c_str(); }
+
+int vh_cdf_guide(const float* cdf, uint32_t n, uint32_t bits, uint32_t* guide)
+{
+    if (!cdf || !guide || n == 0 || bits > 16)
+        return fail(VH_ERR_INVALID_ARGUMENT, "vh_cdf_guide: null table, empty CDF or more than 16 bits");
+    if (!cdf_guide(cdf, n, bits, guide))
+        return fail(VH_ERR_INVALID_ARGUMENT, "vh_cdf_guide: the CDF is not non-decreasing");
+    return VH_OK;
+}
 uint64_t vh_last_error_needed(void) { return g_needed; }
 uint64_t vh_last_error_available(void) { return g_available; }
 
@@ -444,6 +496,7 @@ static int update_tables(vh_renderer* r, const vh_scene_tables* t, const vh_upda
     if (plan.emitters) {
         TRY(reupload(c, r->emitters, t->emitters, t->emitters_len));
         TRY(reupload(c, r->emitCdf, t->emit_cdf, t->emit_cdf_len));
+        TRY(upload_emitter_guide(r, t));
         r->info.emitter_count = t->info.emitter_count;
     }
     if (plan.instances)
@@ -538,6 +591,8 @@ int vh_set_restir(vh_renderer* r, const vh_restir* restir)
 {
     if (!r)
         return fail(VH_ERR_INVALID_ARGUMENT, "null renderer");
+    if (restir && restir->neighbours > kRestirMaxNeighbours)
+        return fail(VH_ERR_INVALID_ARGUMENT, "ReSTIR reuses at most " + std::to_string(kRestirMaxNeighbours) + " neighbours");
     r->restirOn = restir != nullptr;
     if (restir)
         r->restir = *restir;
@@ -579,8 +634,8 @@ int vh_accumulation_create(vh_renderer* r, uint32_t width, uint32_t height, int 
     if (width == 0 || height == 0)
         return fail(VH_ERR_INVALID_ARGUMENT, "zero image size");
     uint64_t n = (uint64_t)width * height;
-    if (n >= (1ull << 31))
-        return fail(VH_ERR_INVALID_ARGUMENT, "image larger than 2^31 pixels");
+    if (n >= (1ull << 30)) // pixel ids carry two flag bits in the deferred shading
+        return fail(VH_ERR_INVALID_ARGUMENT, "image of 2^30 pixels or more");
     uint64_t bytes = 24 * n + (lighting ? 4 * kLightingStride * n : 0);
     TRY(check_fits(bytes));
     std::unique_ptr<vh_accumulation> a(new vh_accumulation());
@@ -662,8 +717,6 @@ int vh_accumulate(vh_renderer* r, vh_accumulation* a, uint32_t samples)
     HIPTRY(hipEventRecord(e0, s));
     if (wave) {
         Wavefront W = a->wavefront(split);
-        if (restir)
-            W.shK = 0; // ReSTIR shades fused
         LaunchCfg cfg{s, r->wfBlocks ? r->wfBlocks : std::max(256u, r->ctx->cus * 8)};
         for (uint32_t k = 0; k < samples && rc == VH_OK; k++) {
             Params P = base;

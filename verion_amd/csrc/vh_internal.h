@@ -172,7 +172,9 @@ struct vh_context {
 struct vh_renderer {
     vh_context* ctx = nullptr;
     vh::DevBuf positions, indices, uvs, vnormals, instances, materials, lights, emitters, emitCdf, envPixels, envMarginal,
-        envCond, texels, texInfo, instStart;
+        envCond, texels, texInfo, instStart, emitGuide;
+    std::vector<uint32_t> emitGuideHost; // kept until the upload has been read
+    uint32_t emitGuideBits = 0;
     vh::BvhBuild bvh;
     vh_scene_info info{};
     vh_view view{};
@@ -206,6 +208,8 @@ struct vh_renderer {
         s.lights = lights.as<uint32_t>();
         s.emitters = emitters.as<float>();
         s.emitCdf = emitCdf.as<float>();
+        s.emitGuide = emitGuide.as<uint32_t>();
+        s.emitGuideBits = emitGuideBits;
         s.envPixels = envPixels.as<float>();
         s.envMarginal = envMarginal.as<float>();
         s.envCond = envCond.as<float>();
@@ -240,6 +244,9 @@ struct vh_accumulation {
     bool wfReady = false;
     uint64_t wfBytes = 0;
     uint64_t pixels() const { return (uint64_t)width * height; }
+    // shadow records a deferred path may make: one per delta light, one emitter or environment
+    // sample, and ReSTIR DI's winner
+    uint32_t shadowRecords() const { return r->info.light_count + 1 + (restirOn && restir.di ? 1u : 0u); }
     vh::Wavefront wavefront(bool split) const
     {
         vh::Wavefront W;
@@ -252,7 +259,7 @@ struct vh_accumulation {
         W.words = 17 + (split ? 17 : 0);
         W.tileW = r->tileW;
         W.tileH = r->tileH;
-        if (!split && shK > 0 && r->info.light_count + 1 <= shK) { // (a scene update may add lights)
+        if (!split && shK > 0 && shadowRecords() <= shK) { // (a scene update may add lights)
             W.shK = shK;
             W.shRec = shRec.as<float>();
             W.shVis = shVis.as<uint32_t>();

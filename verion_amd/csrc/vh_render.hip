@@ -273,7 +273,13 @@ __global__ __launch_bounds__(kWfBlock, VH_SHADE_WAVES) void k_wf_shade(Scene S, 
 // wf_shade of the plain integrator with deferred shadow rays: shade, record the NEE shadow rays
 // (with the contribution each adds when unoccluded), append survivors to the next depth. A path
 // with records stays pending (its total without them, its next slot or its end) until
-// k_wf_resolve; a path without records is stored or finished here.
+// k_wf_resolve; a path without records is stored or finished here. RESTIR: ReSTIR's first hit,
+// where DI's winner's shadow ray is one more record (k_wf_resolve zeroes the stored reservoir
+// weight when it is blocked). GI: its reuse at the first hit was made by k_gi_prepare; the first
+// hit's own light (the path total after the first hit's NEE, which GI keeps apart) is written by
+// k_wf_resolve once its shadow rays are known; a path that carries a second-hit sample finishes
+// through giFinish, here or in k_wf_resolve.
+template <bool GI, bool RESTIR>
 __global__ __launch_bounds__(kWfBlock, VH_SHADE_WAVES) void k_wf_shade_deferred(Scene S, Frame F, Params P, Wavefront W, uint32_t d)
 {
     Ctx c{S, F, P};
@@ -284,7 +290,7 @@ __global__ __launch_bounds__(kWfBlock, VH_SHADE_WAVES) void k_wf_shade_deferred(
         uint32_t i = base + threadIdx.x;
         bool keep = false;
         PathState st;
-        uint32_t pixel = 0, nrec = 0;
+        uint32_t pixel = 0, nrec = 0, tag = 0;
         if (i < count) {
             if (d == 0) {
                 pixel = P.w[32] != 0 ? W.queue0[i] : tilePixel(i, P.w[0], P.w[1], W.tileW, W.tileH);
@@ -301,11 +307,20 @@ __global__ __launch_bounds__(kWfBlock, VH_SHADE_WAVES) void k_wf_shade_deferred(
             h.b1 = h.b2 = 0.0f;
             h.gid = 0;
             ShadowSink sink{W.shRec, W.n, W.shK, i, 0u};
-            keep = shadePath<false, false, 0, true>(c, st, h, d, pixel, nullptr, &sink) && d + 1 < P.w[7];
+            keep = shadePath<RESTIR, false, 0, true, GI ? 2 : 0>(c, st, h, d, pixel, nullptr, &sink) && d + 1 < P.w[7];
             nrec = sink.count;
-            W.shNRec[i] = nrec;
-            if (!keep && nrec == 0)
-                finishPath<false>(c, st, pixel);
+            W.shNRec[i] = nrec | (sink.restirRec << 8);
+            tag = pixel;
+            if (!keep && nrec == 0) {
+                finishPath<false, GI>(c, st, pixel);
+            } else if (GI && !keep && st.gi) {
+                // finished in k_wf_resolve: finishPath's dispersion check now, its pixel bit there
+                if (st.chan >= 0)
+                    c.F.restir[restirScratch(pixel) + 22] = asf(0u);
+                tag |= kPendingGI;
+            }
+            if (GI && d == 0 && st.gi)
+                tag |= kPendingFirst;
         }
         uint32_t slot = compact(keep, &W.counts[d + 1]);
         if (keep && slot < W.n)
@@ -313,12 +328,46 @@ __global__ __launch_bounds__(kWfBlock, VH_SHADE_WAVES) void k_wf_shade_deferred(
         uint32_t q = compactN(nrec, &W.counts[kShadowCounts + d]);
         if (nrec > 0) {
             W.shOut[i] = keep ? slot : 0xffffffffu;
-            W.shPixel[i] = pixel;
+            W.shPixel[i] = tag;
             put3(reinterpret_cast<uint32_t*>(W.shTotal), W.n, 0, i, st.parts.total);
             for (uint32_t k = 0; k < nrec; k++)
                 if (q + k < W.shK * W.n)
                     W.shQueue[q + k] = k * W.n + i;
         }
+    }
+}
+
+// ReSTIR GI's reuse at the first hit (restirGIPrepare, pt_core.slang:901-972) for the depth-0 queue,
+// ahead of k_wf_shade_deferred: each lane replays its first hit's shading up to the reuse
+// (shadePath's probe), then the wave resamples its lanes' neighbours together, tracing their
+// visibility rays with the wave-cooperative traversal (restirGIPrepareWave). The grid stride is
+// wave-uniform: every lane of a wave reaches the cooperative traversal.
+__global__ __launch_bounds__(kWfBlock, VH_SHADE_WAVES) void k_gi_prepare(Scene S, Frame F, Params P, Wavefront W)
+{
+    Ctx c{S, F, P};
+    const uint32_t count = W.counts[0];
+    __shared__ WaveTris waveTris[kWfBlock / 64];
+    WaveTris& sh = waveTris[threadIdx.x >> 6];
+    for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += gridDim.x * kWfBlock) {
+        uint32_t i = base + threadIdx.x;
+        GiSite site;
+        site.active = false;
+        uint32_t pixel = 0;
+        if (i < count) {
+            pixel = P.w[32] != 0 ? W.queue0[i] : tilePixel(i, P.w[0], P.w[1], W.tileW, W.tileH);
+            PathState st;
+            generatePath(P, pixel % P.w[0], pixel / P.w[0], P.w[6], st);
+            uint4 hw = W.hits[i];
+            Hit h;
+            h.hit = hw.x != 0;
+            h.inst = hw.y;
+            h.prim = hw.z;
+            h.t = asf(hw.w);
+            h.b1 = h.b2 = 0.0f;
+            h.gid = 0;
+            (void)shadePath<true, false, 0, false, 1>(c, st, h, 0, pixel, nullptr, nullptr, &site);
+        }
+        restirGIPrepareWave(c, site, pixel, sh);
     }
 }
 
@@ -362,6 +411,7 @@ __global__ __launch_bounds__(kWfBlock, VH_EXTEND_WAVES) void k_wf_shadow(Scene S
 
 // Adds the visible shadow contributions of every pending path at depth d in record order, then
 // stores the total in the path's next slot or finishes the path.
+template <bool GI>
 __global__ __launch_bounds__(kWfBlock) void k_wf_resolve(Scene S, Frame F, Params P, Wavefront W, uint32_t d)
 {
     Ctx c{S, F, P};
@@ -369,7 +419,8 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_resolve(Scene S, Frame F, Param
     uint32_t* out = W.state[(d + 1) & 1u];
     const size_t stride = (size_t)W.shK * W.n;
     for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < count; i += gridDim.x * kWfBlock) {
-        uint32_t nrec = W.shNRec[i];
+        uint32_t nw = W.shNRec[i];
+        uint32_t nrec = nw & 0xffu, restirRec = nw >> 8;
         if (nrec == 0)
             continue;
         V3 total = get3(reinterpret_cast<const uint32_t*>(W.shTotal), W.n, 0, i);
@@ -378,7 +429,17 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_resolve(Scene S, Frame F, Param
             if (W.shVis[rid] == 0) {
                 const float* r = W.shRec + rid;
                 total = total + v3(r[7 * stride], r[8 * stride], r[9 * stride]);
+            } else if (k + 1 == restirRec) {
+                // the ReSTIR DI winner is blocked: its reservoir is stored with W = 0 (pt_core.slang:872-879)
+                F.restir[restirBase(W.shPixel[i] & ~(kPendingGI | kPendingFirst), P.w[6] & 1u) + 4] = 0.0f;
             }
+        }
+        if (GI && (W.shPixel[i] & kPendingFirst) != 0) {
+            // ReSTIR GI's first-hit light: the total after the first hit (pt_core.slang:1531-1540)
+            float* Sc = F.restir + restirScratch(W.shPixel[i] & ~(kPendingGI | kPendingFirst));
+            Sc[18] = total.x;
+            Sc[19] = total.y;
+            Sc[20] = total.z;
         }
         uint32_t os = W.shOut[i];
         if (os != 0xffffffffu) {
@@ -388,7 +449,10 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_resolve(Scene S, Frame F, Param
             st.parts = Parts{total, v3(0.0f), v3(0.0f), v3(0.0f), v3(0.0f), v3(0.0f)};
             st.visLit = 0.0f;
             st.visTotal = 0.0f;
-            finishPath<false>(c, st, W.shPixel[i]);
+            uint32_t pixel = W.shPixel[i];
+            st.gi = GI && (pixel & kPendingGI) != 0;
+            st.chan = -1; // a dispersive path's GI sample was dropped when it was recorded
+            finishPath<false, GI>(c, st, pixel & ~(kPendingGI | kPendingFirst));
         }
     }
 }
@@ -720,6 +784,8 @@ int launch_wf_sample(const Scene& S, const Frame& F, const Params& P0, const Wav
     uint32_t grid = cfg.wfGrid ? cfg.wfGrid : 1024;
     Params P = P0;
     uint32_t maxDepth = P.w[7];
+    // ReSTIR DI defers its winner's shadow ray with the others; GI's reuse runs first, in k_gi_prepare
+    const bool gi = restir && (P.w[36] & 2u) != 0;
     k_wf_reset<<<1, 64, 0, s>>>(W.counts, W.n, P.w[32], maxDepth);
     if (P.w[32] != 0)
         k_wf_select<<<blocks_for(W.n, kWfBlock, grid), kWfBlock, 0, s>>>(F, P, W);
@@ -730,14 +796,24 @@ int launch_wf_sample(const Scene& S, const Frame& F, const Params& P0, const Wav
         k_wf_extend<<<grid, kWfBlock, 0, s>>>(S, P, W, d);
         if (ev)
             (void)hipEventRecord(ev[kStageEvents * d + 1], s);
-        if (W.shK > 0 && !restir && !split) {
-            k_wf_shade_deferred<<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+        if (W.shK > 0 && !split) {
+            if (gi && d == 0)
+                k_gi_prepare<<<grid, kWfBlock, 0, s>>>(S, F, P, W);
+            if (gi)
+                k_wf_shade_deferred<true, true><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+            else if (restir && d == 0)
+                k_wf_shade_deferred<false, true><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+            else
+                k_wf_shade_deferred<false, false><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
             if (ev)
                 (void)hipEventRecord(ev[kStageEvents * d + 2], s);
             k_wf_shadow<<<grid, kWfBlock, 0, s>>>(S, P, W, d);
             if (ev)
                 (void)hipEventRecord(ev[kStageEvents * d + 3], s);
-            k_wf_resolve<<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+            if (gi)
+                k_wf_resolve<true><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+            else
+                k_wf_resolve<false><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
             if (ev)
                 (void)hipEventRecord(ev[kStageEvents * d + 4], s);
             continue;
@@ -793,9 +869,9 @@ int launch_wf_depth(const Scene& S, const Frame& F, const Params& P0, const Wave
     uint32_t grid = cfg.wfGrid ? cfg.wfGrid : 1024;
     k_wf_extend<<<grid, kWfBlock, 0, cfg.stream>>>(S, P, W, d);
     if (shade && W.shK > 0) {
-        k_wf_shade_deferred<<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
+        k_wf_shade_deferred<false, false><<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
         k_wf_shadow<<<grid, kWfBlock, 0, cfg.stream>>>(S, P, W, d);
-        k_wf_resolve<<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
+        k_wf_resolve<false><<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
     } else if (shade) {
         k_wf_shade<false, false><<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
     }

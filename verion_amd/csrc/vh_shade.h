@@ -25,6 +25,7 @@ constexpr uint32_t kAlbedoWords = kAlbedoDiel + kDielN * kDielN * kDielNIor * 2;
 constexpr float kRestirMinAlpha = 0.15f;
 constexpr int kRestirRadius = 6;
 constexpr uint32_t kRestirNeighbours = 2;
+constexpr uint32_t kRestirMaxNeighbours = 16;
 constexpr uint32_t kRestirHalf = 28;
 constexpr uint32_t kRestirStride = 80;
 constexpr float kRestirMaxJacobian = 10.0f;
@@ -40,6 +41,7 @@ struct Scene {
     const uint32_t* lights;
     const float* emitters;
     const float* emitCdf;
+    const uint32_t* emitGuide; // 2^emitGuideBits + 1 cutpoints into emitCdf (emitterIndex)
     const float* envPixels;
     const float* envMarginal;
     const float* envCond;
@@ -47,6 +49,7 @@ struct Scene {
     const uint32_t* texInfo;
     Bvh bvh;
     uint32_t translucent; // some material has opacity < 1 or an opacity texture
+    uint32_t emitGuideBits;
 };
 
 // Running sums of one accumulation.
@@ -555,6 +558,34 @@ __device__ void generatePath(const Params& P, uint32_t px, uint32_t py, uint32_t
 
 // ---------------------------------------------------------------- ReSTIR DI (pt_core.slang:702-995)
 
+// The emitter a power-CDF draw u in [0, 1) selects: the first index whose CDF value reaches u, or
+// the last emitter (the binary search of pt_core.slang:1343-1351 and :744-753). The guide table
+// (Chen and Asau's cutpoint method) holds, for each of the 2^bits intervals [k, k + 1) / 2^bits,
+// the first index whose CDF reaches k / 2^bits, so the answer for u lies between the cutpoints of
+// u's interval and the same binary search runs over that range only: the same index in ~2 steps
+// instead of log2(n). u * 2^bits is exact (u is a multiple of 2^-24), so the interval is exact.
+__device__ __forceinline__ uint32_t emitterIndex(const Scene& S, float u)
+{
+    uint32_t k = (uint32_t)(u * (float)(1u << S.emitGuideBits));
+    k = umin(k, (1u << S.emitGuideBits) - 1u);
+    uint32_t lo = S.emitGuide[k], hi = S.emitGuide[k + 1];
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) / 2;
+        if (S.emitCdf[mid] >= u)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+// Random neighbours per reuse: bits 8-15 of word 36 (the ReSTIR mode word); 0 = the reference's 2.
+__device__ __forceinline__ uint32_t restirNeighbours(const Params& P)
+{
+    uint32_t k = (P.w[36] >> 8) & 0xffu;
+    return k ? k : kRestirNeighbours;
+}
+
 __device__ __forceinline__ uint32_t restirBase(uint32_t pixel, uint32_t which) { return pixel * kRestirStride + which * kRestirHalf; }
 
 __device__ bool restirLight(const Ctx& c, uint32_t e, uint32_t recv, V3 x, V3 p, V3 n, V3& wi, float& dist, float& cosi, V3& g, float& plsa)
@@ -586,13 +617,15 @@ struct RestirPick {
     bool occ;
 };
 
+// DEFER: the winner's shadow ray is left to the caller's shadow record (stored W as if visible;
+// k_wf_resolve zeroes it when the ray is blocked).
+template <bool DEFER = false>
 __device__ RestirPick restirDI(const Ctx& c, uint32_t pixel, uint32_t recv, V3 p, V3 n, float pHitT, const Surface& surf, V3 T, V3 B,
                                V3 woL, uint32_t& rng)
 {
     const Params& P = c.P;
     uint32_t w = P.w[0];
     uint32_t cur = P.w[6] & 1u;
-    uint32_t emitterCount = P.w[23];
     float cap = (float)umax(P.w[38], 1u);
     float wsum = 0.0f, mTotal = 0.0f;
     uint32_t selE = 0;
@@ -601,14 +634,7 @@ __device__ RestirPick restirDI(const Ctx& c, uint32_t pixel, uint32_t recv, V3 p
     uint32_t ncand = umax(P.w[37], 1u);
     for (uint32_t k = 0; k < ncand; k++) {
         float ue = rnext(rng), ua = rnext(rng), ub = rnext(rng);
-        uint32_t lo = 0, hi = emitterCount - 1;
-        while (lo < hi) {
-            uint32_t mid = (lo + hi) / 2;
-            if (c.S.emitCdf[mid] >= ue)
-                hi = mid;
-            else
-                lo = mid + 1;
-        }
+        uint32_t lo = emitterIndex(c.S, ue);
         const float* E = c.S.emitters + (size_t)lo * 16;
         V3 a = v3(E[0], E[1], E[2]), b = v3(E[3], E[4], E[5]), cc = v3(E[6], E[7], E[8]);
         float su = sqrtf(ua);
@@ -635,7 +661,7 @@ __device__ RestirPick restirDI(const Ctx& c, uint32_t pixel, uint32_t recv, V3 p
     }
     if (P.w[6] > 0) {
         int px = (int)(pixel % w), py = (int)(pixel / w);
-        for (uint32_t k = 0; k < 1 + kRestirNeighbours; k++) {
+        for (uint32_t k = 0; k < 1 + restirNeighbours(P); k++) {
             int nx = px, ny = py;
             if (k > 0) {
                 nx += (int)(rnext(rng) * (float)(2 * kRestirRadius + 1)) - kRestirRadius;
@@ -678,7 +704,7 @@ __device__ RestirPick restirDI(const Ctx& c, uint32_t pixel, uint32_t recv, V3 p
     r.x = selX;
     r.W = r.valid ? wsum / (mTotal * selPhat) : 0.0f;
     r.occ = false;
-    if (r.valid) {
+    if (r.valid && !DEFER) {
         V3 wi, g;
         float dist, cosi, plsa;
         restirLight(c, selE - 1, recv, selX, p, n, wi, dist, cosi, g, plsa);
@@ -738,7 +764,7 @@ __device__ void restirGIPrepare(const Ctx& c, uint32_t pixel, V3 p, V3 n, float 
     V3 cSel = v3(0.0f), xSel = v3(0.0f), nSel = v3(0.0f), lSel = v3(0.0f);
     if (P.w[6] > 0) {
         int px = (int)(pixel % w), py = (int)(pixel / w);
-        for (uint32_t k = 0; k < 1 + kRestirNeighbours; k++) {
+        for (uint32_t k = 0; k < 1 + restirNeighbours(P); k++) {
             int nx = px, ny = py;
             if (k > 0) {
                 nx += (int)(rnext(rng) * (float)(2 * kRestirRadius + 1)) - kRestirRadius;
@@ -802,6 +828,117 @@ __device__ void restirGIPrepare(const Ctx& c, uint32_t pixel, V3 p, V3 n, float 
     Sc[13] = lSel.y;
     Sc[14] = lSel.z;
     Sc[22] = asf(0u);
+}
+
+// What restirGIPrepare reads at the first hit, captured by shadePath's probe mode (GIP = 1).
+struct GiSite {
+    bool active; // the first hit takes ReSTIR GI
+    V3 p, n, woL, T, B;
+    float pHitT;
+    Surface surf;
+    uint32_t rng;
+};
+
+// restirGIPrepare for the wave's 64 first hits together: the same candidates, weights and random
+// draws per lane, with each candidate's visibility ray traced by the wave-cooperative traversal
+// (every lane of the wave calls this, `site.active` or not). The lane's random state after the
+// reuse goes to scratch word 23, where shadePath's prepared mode (GIP = 2) picks it up.
+__device__ void restirGIPrepareWave(const Ctx& c, const GiSite& site, uint32_t pixel, WaveTris& sh)
+{
+    const Params& P = c.P;
+    uint32_t w = P.w[0];
+    uint32_t cur = P.w[6] & 1u;
+    float cap = (float)max(P.w[38], 1u);
+    float wsum = 0.0f, mTotal = 0.0f, pSel = 0.0f;
+    V3 cSel = v3(0.0f), xSel = v3(0.0f), nSel = v3(0.0f), lSel = v3(0.0f);
+    uint32_t rng = site.rng;
+    const bool act = site.active;
+    const V3 p = site.p, n = site.n;
+    if (P.w[6] > 0) {
+        int px = (int)(pixel % w), py = (int)(pixel / w);
+        for (uint32_t k = 0; k < 1 + restirNeighbours(P); k++) {
+            bool need = false;
+            V3 wi = v3(0.0f, 0.0f, 1.0f), cc = v3(0.0f), x2 = v3(0.0f), n2 = v3(0.0f), l2 = v3(0.0f);
+            float dR = 0.0f, phat = 0.0f, rW = 0.0f, mq = 0.0f, jac = 0.0f;
+            do {
+                if (!act)
+                    break;
+                int nx = px, ny = py;
+                if (k > 0) {
+                    nx += (int)(rnext(rng) * (float)(2 * kRestirRadius + 1)) - kRestirRadius;
+                    ny += (int)(rnext(rng) * (float)(2 * kRestirRadius + 1)) - kRestirRadius;
+                    if (nx < 0 || ny < 0 || nx >= (int)w || ny >= (int)P.w[1])
+                        break;
+                }
+                const float* R = c.F.restir + restirBase((uint32_t)ny * w + (uint32_t)nx, cur ^ 1u);
+                if (asu(R[14]) == 0 || asu(R[27]) == 0)
+                    break;
+                V3 pq = v3(R[8], R[9], R[10]);
+                V3 nq = v3(R[11], R[12], R[13]);
+                if (dot(nq, n) < 0.9f || length(pq - p) > 0.05f * fmaxf(site.pHitT, 0.01f))
+                    break;
+                x2 = v3(R[16], R[17], R[18]);
+                n2 = v3(R[19], R[20], R[21]);
+                l2 = v3(R[22], R[23], R[24]);
+                mq = fminf(R[26], cap);
+                mTotal += mq;
+                V3 toX = x2 - p;
+                dR = length(toX);
+                wi = toX / fmaxf(dR, 1.0e-6f);
+                float dQ = length(x2 - pq);
+                float cosT = dot(n, wi);
+                float cosR = fabsf(dot(n2, wi));
+                float cosQ = fabsf(dot(n2, normalize(pq - x2)));
+                if (cosT <= 0.0f || dR < 1.0e-3f || cosR < 1.0e-3f || cosQ < 1.0e-3f)
+                    break;
+                jac = (cosR / cosQ) * (dQ * dQ) / (dR * dR);
+                if (jac > kRestirMaxJacobian || jac < 1.0f / kRestirMaxJacobian)
+                    break;
+                cc = evalReflective(c, site.surf, site.woL, v3(dot(wi, site.T), dot(wi, site.B), cosT)) * l2;
+                phat = lum(cc);
+                rW = R[25];
+                need = phat > 0.0f;
+            } while (false);
+            bool blocked;
+            if (c.S.translucent) {
+                blocked = need && occluded(c, p, wi, dR * (1.0f - 1.0e-3f));
+            } else {
+                Hit h = traceWave<true>(c.S.bvh, need ? p : v3(0.0f), wi, dR * (1.0f - 1.0e-3f), need, sh);
+                blocked = need && h.hit;
+            }
+            if (need && !blocked) {
+                float wgt = phat * rW * mq * jac;
+                wsum += wgt;
+                if (rnext(rng) * wsum < wgt) {
+                    pSel = phat;
+                    cSel = cc;
+                    xSel = x2;
+                    nSel = n2;
+                    lSel = l2;
+                }
+            }
+        }
+    }
+    if (!act)
+        return;
+    float* Sc = c.F.restir + restirScratch(pixel);
+    Sc[0] = wsum;
+    Sc[1] = mTotal;
+    Sc[2] = pSel;
+    Sc[3] = cSel.x;
+    Sc[4] = cSel.y;
+    Sc[5] = cSel.z;
+    Sc[6] = xSel.x;
+    Sc[7] = xSel.y;
+    Sc[8] = xSel.z;
+    Sc[9] = nSel.x;
+    Sc[10] = nSel.y;
+    Sc[11] = nSel.z;
+    Sc[12] = lSel.x;
+    Sc[13] = lSel.y;
+    Sc[14] = lSel.z;
+    Sc[22] = asf(0u);
+    Sc[23] = asf(rng);
 }
 
 // giFinish (pt_core.slang:1566-1617): the path's own candidate (the surface its first bounce
@@ -883,18 +1020,21 @@ struct NrcIo {
 };
 constexpr uint32_t kMaxDepthNrc = 8;
 
-// Deferred shadow rays (plain wavefront). With DEFER, shadePath does not trace its NEE shadow
-// rays: it records each one with the contribution it adds to the path total when unoccluded, in
+// Deferred shadow rays (plain wavefront, ReSTIR DI). With DEFER, shadePath does not trace its NEE
+// shadow rays (nor the ReSTIR DI winner's): it records each one with the contribution it adds to the path total when unoccluded, in
 // the order the fused code adds them; a later kernel traces the records and adds the visible
 // contributions in that order, so the path total is the same bit for bit.
 // Record k of slot i, word w (origin xyz, direction xyz, tmax, contribution rgb) lives at
 // rec[(w * K + k) * n + i].
-constexpr uint32_t kMaxShadow = 8;  // records per path at most (delta lights + one emitter/env sample)
+constexpr uint32_t kMaxShadow = 8;  // records per path at most (delta lights + one emitter/env sample + the ReSTIR DI winner)
 constexpr uint32_t kShadowWords = 10;
+constexpr uint32_t kPendingGI = 0x80000000u;    // pending pixel id bit: finish through giFinish
+constexpr uint32_t kPendingFirst = 0x40000000u; // pending pixel id bit: a GI first hit's own light
 struct ShadowSink {
     float* rec;
     uint32_t n, K, slot;
     uint32_t count;
+    uint32_t restirRec = 0; // 1 + the record of the ReSTIR DI winner, 0 when none
     __device__ __forceinline__ void push(V3 o, V3 d, float tmax, V3 c)
     {
         if (count >= K) // never: K covers every delta light plus one sample (checked on the host)
@@ -911,11 +1051,15 @@ struct ShadowSink {
 
 // ---------------------------------------------------------------- shadePath (pt_core.slang:992-1543)
 
-template <bool RESTIR, bool SPLIT, int NRC = 0, bool DEFER = false>
+// GIP (ReSTIR GI's reuse at the first hit): 0 traces it inline (restirGIPrepare); 1 is the probe
+// of k_gi_prepare, which stops at the reuse and hands its inputs to `site` (no delta lights, no
+// writes); 2 takes the reuse k_gi_prepare made (restirGIPrepareWave), i.e. its scratch words and
+// the random state after it.
+template <bool RESTIR, bool SPLIT, int NRC = 0, bool DEFER = false, int GIP = 0>
 __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t depth, uint32_t pixel, NrcIo* io = nullptr,
-                          ShadowSink* sink = nullptr)
+                          ShadowSink* sink = nullptr, GiSite* site = nullptr)
 {
-    static_assert(!DEFER || (!RESTIR && !SPLIT && NRC != 2), "deferred shadow rays are for the plain integrator and its training paths");
+    static_assert(!DEFER || (!SPLIT && NRC != 2), "deferred shadow rays are for the plain integrator, ReSTIR DI and training paths");
     const Scene& S = c.S;
     const Params& P = c.P;
     V3 sky = v3(P.f(20), P.f(21), P.f(22));
@@ -931,6 +1075,8 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
     bool alive = true;
     do {
         if (!hq.hit) {
+            if (GIP == 1)
+                return false;
             if (RESTIR && depth == 0)
                 restirClear(c, pixel);
             if (depth == 0 && P.w[42] != 0) {
@@ -1129,8 +1275,8 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
         surf.sheenAlpha = sheenR * sheenR;
         surf.sheenP = sheenMax > 0.0f ? sheenMax * sheenAlbedo(c, woL.z, sheenR) : 0.0f;
 
-        // delta lights (pt_core.slang:1224-1282)
-        for (uint32_t li = 0; li < lightCount; li++) {
+        // delta lights (pt_core.slang:1224-1282); they draw no random numbers
+        for (uint32_t li = 0; li < (GIP == 1 ? 0u : lightCount); li++) {
             const uint32_t* L = S.lights + li * 16;
             V3 col = v3(asf(L[1]), asf(L[2]), asf(L[3])) * asf(L[4]);
             V3 vec = v3(asf(L[5]), asf(L[6]), asf(L[7]));
@@ -1193,15 +1339,31 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
             bool reGI = suits && (P.w[36] & 2u) != 0 && !lightingOn;
             st.restir = reDI;
             st.gi = reGI;
+            if constexpr (GIP == 1) {
+                site->active = reGI;
+                site->p = p;
+                site->n = n;
+                site->woL = woL;
+                site->T = T;
+                site->B = B;
+                site->pHitT = pHitT;
+                site->surf = surf;
+                site->rng = rng;
+                return false;
+            }
             if (suits) {
                 restirGbuffer(c, pixel, p, n);
-                if (reGI)
-                    restirGIPrepare(c, pixel, p, n, pHitT, surf, T, B, woL, rng);
+                if (reGI) {
+                    if (GIP == 2)
+                        rng = asu(c.F.restir[restirScratch(pixel) + 23]);
+                    else
+                        restirGIPrepare(c, pixel, p, n, pHitT, surf, T, B, woL, rng);
+                }
             } else
                 restirClear(c, pixel);
         }
         if (RESTIR && reDI) {
-            RestirPick pick = restirDI(c, pixel, recvMask(c, hq.inst), p, n, pHitT, surf, T, B, woL, rng);
+            RestirPick pick = restirDI<DEFER>(c, pixel, recvMask(c, hq.inst), p, n, pHitT, surf, T, B, woL, rng);
             if (pick.valid) {
                 V3 wi, g;
                 float dist, cosi, plsa;
@@ -1213,8 +1375,12 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
                     Lobes l = evalLobes(c, surf, woL, wiL);
                     V3 cd = thr * l.diffuse * g * (wl * pick.W);
                     V3 cs = thr * l.specular * g * (wl * pick.W);
-                    if (!pick.occ)
+                    if (DEFER) {
+                        sink->push(p, wi, dist * (1.0f - 1.0e-3f), cd + cs);
+                        sink->restirRec = sink->count;
+                    } else if (!pick.occ) {
                         partsAddLight(st.parts, depth, cls, cd + cs, cd, cs, SPLIT);
+                    }
                     if (SPLIT && track) {
                         float vv = lum(cd + cs);
                         st.visTotal += vv;
@@ -1231,14 +1397,7 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
         bool chooseEnv = hasEnv && chooseLight < pEnvHere;
         if (emitterCount > 0 && !chooseEnv && !reDI) {
             float ue = rnext(rng), ua = rnext(rng), ub = rnext(rng);
-            uint32_t lo = 0, hi = emitterCount - 1;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi) / 2;
-                if (S.emitCdf[mid] >= ue)
-                    hi = mid;
-                else
-                    lo = mid + 1;
-            }
+            uint32_t lo = emitterIndex(S, ue);
             const float* E = S.emitters + (size_t)lo * 16;
             V3 a = v3(E[0], E[1], E[2]), b = v3(E[3], E[4], E[5]), ce = v3(E[6], E[7], E[8]);
             float su = sqrtf(ua);

@@ -469,7 +469,9 @@ class UpdatePlan:
 def base_params(info: SceneInfo, view: View, width: int, height: int, lighting: bool, multiscatter: bool,
                 adaptive=None, restir=None) -> np.ndarray:
     """pack.rs:828-874: the parameter block with the per-dispatch words (6, 30, 31, 35) zero.
-    `adaptive` is (threshold, min_spp) or None; `restir` is (di, gi, candidates, history) or None."""
+    `adaptive` is (threshold, min_spp) or None; `restir` is (di, gi, candidates, history[, neighbours])
+    or None. Neighbours (random neighbours reused per frame, 0 = the reference's fixed 2,
+    pt_core.slang:706) go to bits 8-15 of word 36, an extension of this build."""
     p = np.zeros(PARAM_WORDS, np.uint32)
     p[0] = width
     p[1] = height
@@ -497,6 +499,8 @@ def base_params(info: SceneInfo, view: View, width: int, height: int, lighting: 
         p[34] = int(adaptive[1])
     if restir is not None:
         p[36] = int(bool(restir[0])) | (int(bool(restir[1])) << 1)
+        if p[36] and len(restir) > 4:
+            p[36] |= int(restir[4]) << 8
         p[37] = int(restir[2])
         p[38] = int(restir[3])
     return p

@@ -4,14 +4,15 @@ Scene: scenes.many_lights() (ground + 41,000 boxes, 502,252 triangles, 10,000 em
 lognormal(0, 1) emission). Every method renders 1 spp per frame into an FP64 film at 1920x1080;
 the reference is a plain render with `ref_spp` samples and an independent seed. Methods: plain
 (deferred shadow rays), ReSTIR DI (`candidates` initial candidates, the reservoirs of the pixel
-and two random neighbours within 6 px of the previous frame, M capped at `history`:
-pt_core.slang:702-900), and DI + GI (pt_core.slang:901-980, 1566-1617). Each advances along a
+and `neighbours` random neighbours within 6 px of the previous frame, M capped at `history`:
+pt_core.slang:702-900; the workload's 5 neighbours, and the reference's fixed 2 beside it), and
+DI + GI (pt_core.slang:901-980, 1566-1617). Each advances along a
 doubling frame ladder to `frames` and is scored by relMSE after every rung; times are the GPU
 seconds the library reports. Reported per method: ms per frame, relMSE at `frames` frames, and
 the plain tracer's time to reach each method's error at `frames` frames (log-log interpolation),
 as a ratio to the method's own time (above 1: the method gets there sooner).
 
-    python -m verion_amd.restir_eval [--frames 64] [--ref-spp 1024] [--candidates 32]
+    python -m verion_amd.restir_eval [--frames 64] [--ref-spp 1024] [--candidates 32] [--neighbours 5]
 """
 from __future__ import annotations
 
@@ -27,7 +28,7 @@ from .backend import WAVEFRONT, Context, Renderer, Restir
 
 
 def evaluate(width: int = 1920, height: int = 1080, frames: int = 64, ref_spp: int = 1024, candidates: int = 32,
-             history: int = 4, seed: int = 1, log=lambda *a: None) -> dict:
+             history: int = 4, neighbours: int = 5, seed: int = 1, log=lambda *a: None) -> dict:
     ctx = Context()
     r = Renderer(ctx, scenes.many_lights())
     r.set_integrator(WAVEFRONT)
@@ -65,8 +66,10 @@ def evaluate(width: int = 1920, height: int = 1080, frames: int = 64, ref_spp: i
     out = {"plain": {"ms_per_frame": 1e3 * at[frames][0] / frames, "relmse": at[frames][1],
                      "ladder": [[f, round(t, 5), e] for f, t, e in plain]}}
     curve = [(t, e) for _, t, e in plain]
-    for name, restir in (("restir_di", Restir(di=True, gi=False, candidates=candidates, history=history)),
-                         ("restir_di_gi", Restir(di=True, gi=True, candidates=candidates, history=history))):
+    methods = (("restir_di_2_neighbours", Restir(di=True, gi=False, candidates=candidates, history=history)),
+               ("restir_di", Restir(di=True, gi=False, candidates=candidates, history=history, neighbours=neighbours)),
+               ("restir_di_gi", Restir(di=True, gi=True, candidates=candidates, history=history, neighbours=neighbours)))
+    for name, restir in methods:
         pts = run(restir, frames)
         log(name, [(f, round(t, 4), e) for f, t, e in pts])
         t, e = pts[-1][1], pts[-1][2]
@@ -76,6 +79,7 @@ def evaluate(width: int = 1920, height: int = 1080, frames: int = 64, ref_spp: i
                      "ladder": [[f, round(tt, 5), ee] for f, tt, ee in pts]}
     return {"scene": "many_lights (502,252 triangles, 10,000 emitters)", "width": width, "height": height,
             "frames": frames, "reference_spp": ref_spp, "candidates": candidates, "history": history,
+            "neighbours": neighbours,
             "relmse": "mean over pixels and channels of (x-r)^2/(r^2+0.01)", **out}
 
 
@@ -87,8 +91,9 @@ def main():
     ap.add_argument("--ref-spp", type=int, default=1024)
     ap.add_argument("--candidates", type=int, default=32)
     ap.add_argument("--history", type=int, default=4)
+    ap.add_argument("--neighbours", type=int, default=5)
     a = ap.parse_args()
-    res = evaluate(a.width, a.height, a.frames, a.ref_spp, a.candidates, a.history,
+    res = evaluate(a.width, a.height, a.frames, a.ref_spp, a.candidates, a.history, a.neighbours,
                    log=lambda *x: print(*x, file=sys.stderr, flush=True))
     print(json.dumps(res), flush=True)
 
