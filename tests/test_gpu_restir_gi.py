@@ -12,7 +12,7 @@ import oracle
 from helpers import params
 from verion_amd import gltf, scenes
 from verion_amd.scene import PointLight, SpotLight
-from verion_amd.backend import MEGAKERNEL, WAVEFRONT, Renderer, Restir
+from verion_amd.backend import MEGAKERNEL, WAVEFRONT, Adaptive, Renderer, Restir
 
 pytestmark = pytest.mark.gpu
 TOL = 1.0e-4
@@ -162,3 +162,26 @@ def test_wavefront_defers_shadow_rays_past_the_resampling(lit_glass_room, di, gi
     assert np.array_equal(out[0], out[1])
     st = o.render(params(r.packed, 40, 32, restir=(di, gi, 4, 4)), 4)
     assert rel(out[1], st["film"]) <= TOL
+
+
+def test_resampling_with_adaptive_sampling(lit_glass_room):
+    # adaptive sampling selects the depth-0 queue (k_wf_select): the GI reuse and the DI winner's
+    # deferred shadow ray take their pixels from it
+    r, o = lit_glass_room
+    films = []
+    r.set_adaptive(Adaptive(threshold=0.2, min_spp=2))
+    for integ in (MEGAKERNEL, WAVEFRONT):
+        r.set_integrator(integ)
+        r.set_restir(Restir(di=True, gi=True, candidates=4, history=4, neighbours=3))
+        acc = r.new_accumulation(40, 32)
+        for _ in range(6):
+            r.accumulate(acc, 1)
+        films.append((acc.film(), acc.stats()))
+    r.set_restir(None)
+    r.set_adaptive(None)
+    r.set_integrator(MEGAKERNEL)
+    assert np.array_equal(films[0][0], films[1][0]) and np.array_equal(films[0][1], films[1][1])
+    st = o.render(params(r.packed, 40, 32, adaptive=(0.2, 2), restir=(True, True, 4, 4, 3)), 6)
+    assert rel(films[1][0], st["film"]) <= TOL
+    n = films[1][1].reshape(-1, 3)[:, 0]
+    assert (n < 6).any() and (n == 6).any(), "some pixels stopped early, some did not"
