@@ -34,7 +34,7 @@ struct Bvh {
 };
 
 // Watertight ray/triangle test. Inputs are the world-space vertices; on a hit with 0 < t < tmax
-// it returns t and the weights of vertices 1 and 2.
+// it returns t and (BARY) the weights of vertices 1 and 2.
 struct RayPre {
     int kx, ky, kz;
     float sx, sy, sz;
@@ -61,6 +61,7 @@ VH_HD RayPre ray_pre(V3 d)
 
 VH_HD float pick(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
+template <bool BARY = true>
 VH_HD bool intersect(const RayPre& r, V3 o, V3 p0, V3 p1, V3 p2, float tmax, float& tout, float& b1, float& b2)
 {
     V3 A = p0 - o, B = p1 - o, C = p2 - o;
@@ -94,8 +95,10 @@ VH_HD bool intersect(const RayPre& r, V3 o, V3 p0, V3 p1, V3 p2, float tmax, flo
     if (!(t > 0.0f && t < tmax))
         return false;
     tout = t;
-    b1 = V / det;
-    b2 = W / det;
+    if (BARY) {
+        b1 = V / det;
+        b2 = W / det;
+    }
     return true;
 }
 
@@ -306,7 +309,9 @@ __device__ __forceinline__ bool traceStep(const Bvh& bvh, TraceState& s, uint32_
 // start instead of one tightened after each test, which only adds tests whose hits lose: the hit,
 // the smallest t with ties to the smallest id, is the same. Lanes whose ray is done keep testing
 // the other lanes' triangles until the wave's last ray ends. All 64 lanes of a wave must call it
-// together (the wave-uniform loops of the wavefront kernels).
+// together (the wave-uniform loops of the wavefront kernels). Its callers need the hit's ids and
+// distance only (shading recomputes the barycentrics from the hit point), so it returns no
+// barycentric weights (b1 = b2 = 0).
 
 constexpr uint32_t kWaveTests = 64 * 4 * kLeafGroup; // 64 lanes x 4 leaf children x kLeafMax triangles at most
 
@@ -316,7 +321,7 @@ struct WaveTris {
     uint4 base[64];   // first triangle (leaf order) of each child slot of the node
     uint32_t axes[64]; // kx | ky << 2 | kz << 4
     unsigned long long key[64]; // (t bits << 32) | global id: the best hit of the step
-    float4 win[64];   // b1, b2, instance, primitive of that hit
+    uint2 win[64];    // instance, primitive of that hit
     uint16_t list[kWaveTests]; // pending tests: owner lane << (2 + kLeafBits) | child slot << kLeafBits | triangle
 };
 
@@ -395,7 +400,7 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
             bool found = false;
             uint32_t owner = 0;
             unsigned long long mine = 0;
-            float4 w = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            uint2 w = make_uint2(0u, 0u);
             if (q < total) {
                 uint32_t e = sh.list[q];
                 owner = e >> (2 + kLeafBits);
@@ -409,13 +414,14 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
                 rp.sx = r1.x;
                 rp.sy = r1.y;
                 rp.sz = r1.z;
-                const float4* tr = bvh.tris + 3 * (size_t)(sel4(sh.base[owner], c) + k);
+                const uint32_t first = reinterpret_cast<const uint32_t*>(sh.base)[4 * owner + c];
+                const float4* tr = bvh.tris + 3 * (size_t)(first + k);
                 float4 a = tr[0], bb = tr[1], cc = tr[2];
                 float t, b1, b2;
-                if (intersect(rp, v3(r0.x, r0.y, r0.z), xyz(a), xyz(bb), xyz(cc), r0.w, t, b1, b2) && t < r1.w) {
+                if (intersect<false>(rp, v3(r0.x, r0.y, r0.z), xyz(a), xyz(bb), xyz(cc), r0.w, t, b1, b2) && t < r1.w) {
                     found = true;
                     mine = ((unsigned long long)asu(t) << 32) | asu(a.w); // t > 0: its bits order like the value
-                    w = make_float4(b1, b2, bb.w, cc.w);
+                    w = make_uint2(asu(bb.w), asu(cc.w));
                     atomicMin(&sh.key[owner], mine);
                 }
             }
@@ -430,14 +436,12 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
                 float t = asf((uint32_t)(kk >> 32));
                 uint32_t gid = (uint32_t)kk;
                 if (!h.hit || t < h.t || (t == h.t && gid < h.gid)) {
-                    float4 w = sh.win[lane];
+                    uint2 w = sh.win[lane];
                     h.hit = true;
                     h.t = t;
-                    h.b1 = w.x;
-                    h.b2 = w.y;
                     h.gid = gid;
-                    h.inst = asu(w.z);
-                    h.prim = asu(w.w);
+                    h.inst = w.x;
+                    h.prim = w.y;
                     s.best = t;
                 }
             }
