@@ -23,6 +23,14 @@ namespace vh {
 namespace {
 
 constexpr uint32_t kWfBlock = 256;
+// The traversal kernels (extend, shadow) synchronise within a wave only: one wave per block lets
+// each wave retire on its own instead of holding its block's slot until the slowest of its waves
+// ends (their grid is given in kWfBlock-sized blocks like the others' and scaled at launch).
+#ifndef VH_TRACE_BLOCK
+#define VH_TRACE_BLOCK 64
+#endif
+constexpr uint32_t kTraceBlock = VH_TRACE_BLOCK;
+constexpr uint32_t kTraceScale = kWfBlock / kTraceBlock;
 
 
 // Minimum waves per SIMD the compiler must fit the extend and shade kernels into (their register
@@ -196,8 +204,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_select(Frame F, Params P, Wavef
 }
 
 // wf_extend (wf_extend.slang:6-29): closest hit of every live path at depth d
-// wf_extend (wf_extend.slang:6-29): closest hit of every live path at depth d
-__global__ __launch_bounds__(kWfBlock, VH_EXTEND_WAVES) void k_wf_extend(Scene S, Params P, Wavefront W, uint32_t d)
+__global__ __launch_bounds__(kTraceBlock, VH_EXTEND_WAVES) void k_wf_extend(Scene S, Params P, Wavefront W, uint32_t d)
 {
     const uint32_t count = W.counts[d];
     const uint32_t* st = W.state[d & 1u];
@@ -215,10 +222,10 @@ __global__ __launch_bounds__(kWfBlock, VH_EXTEND_WAVES) void k_wf_extend(Scene S
         tmax = 1.0e30f;
     };
     auto done = [&](uint32_t i, const Hit& h) { W.hits[i] = make_uint4(h.hit ? 1u : 0u, h.inst, h.prim, asu(h.t)); };
-    __shared__ WaveTris waveTris[kWfBlock / 64];
+    __shared__ WaveTris waveTris[kTraceBlock / 64];
     WaveTris& sh = waveTris[threadIdx.x >> 6];
     // wave-uniform grid stride: the 64 lanes of a wave trace together (traceWave)
-    for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += gridDim.x * kWfBlock) {
+    for (uint32_t base = blockIdx.x * kTraceBlock; base < count; base += gridDim.x * kTraceBlock) {
         uint32_t i = base + threadIdx.x;
         bool act = i < count;
         V3 org = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
@@ -372,7 +379,7 @@ __global__ __launch_bounds__(kWfBlock, VH_SHADE_WAVES) void k_gi_prepare(Scene S
 }
 
 // The shadow rays recorded at depth d (occluded, pt_core.slang:580-613), one per thread.
-__global__ __launch_bounds__(kWfBlock, VH_EXTEND_WAVES) void k_wf_shadow(Scene S, Params P, Wavefront W, uint32_t d)
+__global__ __launch_bounds__(kTraceBlock, VH_EXTEND_WAVES) void k_wf_shadow(Scene S, Params P, Wavefront W, uint32_t d)
 {
     Frame F{};
     Ctx c{S, F, P};
@@ -385,9 +392,9 @@ __global__ __launch_bounds__(kWfBlock, VH_EXTEND_WAVES) void k_wf_shadow(Scene S
         tmax = r[6 * stride];
     };
     if (!S.translucent) { // any hit ends an opaque shadow ray: the wave-cooperative traversal
-        __shared__ WaveTris waveTris[kWfBlock / 64];
+        __shared__ WaveTris waveTris[kTraceBlock / 64];
         WaveTris& sh = waveTris[threadIdx.x >> 6];
-        for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += gridDim.x * kWfBlock) {
+        for (uint32_t base = blockIdx.x * kTraceBlock; base < count; base += gridDim.x * kTraceBlock) {
             uint32_t q = base + threadIdx.x;
             bool act = q < count;
             V3 o = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
@@ -401,7 +408,7 @@ __global__ __launch_bounds__(kWfBlock, VH_EXTEND_WAVES) void k_wf_shadow(Scene S
         return;
     }
     // translucent scenes: closest hits per layer (occluded)
-    for (uint32_t q = blockIdx.x * kWfBlock + threadIdx.x; q < count; q += gridDim.x * kWfBlock) {
+    for (uint32_t q = blockIdx.x * kTraceBlock + threadIdx.x; q < count; q += gridDim.x * kTraceBlock) {
         V3 o, dir;
         float tmax;
         load(q, o, dir, tmax);
@@ -793,7 +800,7 @@ int launch_wf_sample(const Scene& S, const Frame& F, const Params& P0, const Wav
         P.w[30] = d;
         if (ev)
             (void)hipEventRecord(ev[kStageEvents * d], s);
-        k_wf_extend<<<grid, kWfBlock, 0, s>>>(S, P, W, d);
+        k_wf_extend<<<grid * kTraceScale, kTraceBlock, 0, s>>>(S, P, W, d);
         if (ev)
             (void)hipEventRecord(ev[kStageEvents * d + 1], s);
         if (W.shK > 0 && !split) {
@@ -807,7 +814,7 @@ int launch_wf_sample(const Scene& S, const Frame& F, const Params& P0, const Wav
                 k_wf_shade_deferred<false, false><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
             if (ev)
                 (void)hipEventRecord(ev[kStageEvents * d + 2], s);
-            k_wf_shadow<<<grid, kWfBlock, 0, s>>>(S, P, W, d);
+            k_wf_shadow<<<grid * kTraceScale, kTraceBlock, 0, s>>>(S, P, W, d);
             if (ev)
                 (void)hipEventRecord(ev[kStageEvents * d + 3], s);
             if (gi)
@@ -845,7 +852,7 @@ int launch_nrc_train_paths(const Scene& S, const Params& P, const Wavefront& W, 
         k_nrc_t_extend<<<grid, kWfBlock, 0, s>>>(S, P, W, trainPix, d);
         if (W.shK > 0) { // deferred shadow rays, traced by the wave-cooperative shadow kernel
             k_nrc_t_shade_deferred<<<grid, kWfBlock, 0, s>>>(S, P, W, trainPix, vertices, samples, sampleCount, sampleCap, d);
-            k_wf_shadow<<<grid, kWfBlock, 0, s>>>(S, P, W, d);
+            k_wf_shadow<<<grid * kTraceScale, kTraceBlock, 0, s>>>(S, P, W, d);
             k_nrc_t_resolve<<<grid, kWfBlock, 0, s>>>(W, vertices, samples, sampleCount, sampleCap, d);
         } else {
             k_nrc_t_shade<<<grid, kWfBlock, 0, s>>>(S, P, W, trainPix, vertices, samples, sampleCount, sampleCap, d);
@@ -867,10 +874,10 @@ int launch_wf_depth(const Scene& S, const Frame& F, const Params& P0, const Wave
     Params P = P0;
     P.w[30] = d;
     uint32_t grid = cfg.wfGrid ? cfg.wfGrid : 1024;
-    k_wf_extend<<<grid, kWfBlock, 0, cfg.stream>>>(S, P, W, d);
+    k_wf_extend<<<grid * kTraceScale, kTraceBlock, 0, cfg.stream>>>(S, P, W, d);
     if (shade && W.shK > 0) {
         k_wf_shade_deferred<false, false><<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
-        k_wf_shadow<<<grid, kWfBlock, 0, cfg.stream>>>(S, P, W, d);
+        k_wf_shadow<<<grid * kTraceScale, kTraceBlock, 0, cfg.stream>>>(S, P, W, d);
         k_wf_resolve<false><<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
     } else if (shade) {
         k_wf_shade<false, false><<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
