@@ -34,6 +34,9 @@ namespace {
 
 constexpr uint32_t kBxBlock = 256;   // 4 waves
 constexpr uint32_t kBxWaves = kBxBlock / 64;
+// The traversal kernels (extend, shadow) run one wave per block, so each wave retires on its own;
+// their grid is launched kBxTraceScale times larger to cover the same waves.
+constexpr uint32_t kBxTraceScale = kBxWaves;
 constexpr uint32_t kBxMaxJobs = 255; // one job per thread of the prefix, one more thread for its total
 constexpr uint32_t kCountStride = 2 * kMaxDepth + 1; // per job: live paths per depth, then shadow records per depth
 constexpr uint32_t kShadowOff = kMaxDepth + 1;
@@ -97,6 +100,40 @@ __device__ uint32_t buildPrefix(uint32_t work, uint32_t njobs, uint32_t* s_prefi
     uint32_t excl = before + incl - x;
     if (t <= njobs)
         s_prefix[t] = excl; // t == njobs (x = 0) writes the total
+    __syncthreads();
+    return s_prefix[njobs];
+}
+
+// buildPrefix for a block of one wave: each lane takes kPer consecutive jobs. work(j) is job j's
+// live count; the prefix is over its 64-slot waves.
+template <class Work>
+__device__ uint32_t buildPrefixWave(Work work, uint32_t njobs, uint32_t* s_prefix)
+{
+    constexpr uint32_t kPer = (kBxMaxJobs + 1 + 63) / 64;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t x[kPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; k++) {
+        uint32_t j = lane * kPer + k;
+        x[k] = j < njobs ? (work(j) + 63u) / 64u : 0u;
+        sum += x[k];
+    }
+    uint32_t incl = sum;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(incl, off, 64);
+        if (lane >= off)
+            incl += y;
+    }
+    uint32_t run = incl - sum;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; k++) {
+        uint32_t j = lane * kPer + k;
+        if (j <= njobs) // j == njobs (x = 0) writes the total
+            s_prefix[j] = run;
+        run += x[k];
+    }
     __syncthreads();
     return s_prefix[njobs];
 }
@@ -197,16 +234,15 @@ __global__ __launch_bounds__(kBxBlock) void k_bx_select(const BatchJob* __restri
 }
 
 // wf_extend (wf_extend.slang:6-29) for every job's live queue at depth d
-__global__ __launch_bounds__(kBxBlock) void k_bx_extend(const BatchJob* __restrict__ jobs, BatchDev B, uint32_t d)
+__global__ __launch_bounds__(64) void k_bx_extend(const BatchJob* __restrict__ jobs, BatchDev B, uint32_t d)
 {
     __shared__ uint32_t s_prefix[kBxMaxJobs + 1];
     const uint32_t pass = *B.pass;
-    const uint32_t total = buildPrefix(threadIdx.x < B.njobs ? B.counts[threadIdx.x * kCountStride + d] : 0u, B.njobs, s_prefix);
+    const uint32_t total = buildPrefixWave([&](uint32_t j) { return B.counts[j * kCountStride + d]; }, B.njobs, s_prefix);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t* st = B.state[d & 1u];
-    __shared__ WaveTris waveTris[kBxWaves];
-    WaveTris& sh = waveTris[threadIdx.x >> 6];
-    for (uint32_t gw = blockIdx.x * kBxWaves + (threadIdx.x >> 6); gw < total; gw += gridDim.x * kBxWaves) {
+    __shared__ WaveTris sh;
+    for (uint32_t gw = blockIdx.x; gw < total; gw += gridDim.x) {
         const uint32_t j = findJob(s_prefix, B.njobs, gw);
         const BatchJob& J = jobs[j];
         const uint32_t count = B.counts[j * kCountStride + d];
@@ -297,17 +333,16 @@ __global__ __launch_bounds__(kBxBlock) void k_bx_shade(const BatchJob* __restric
 }
 
 // The shadow rays every job recorded at depth d (occluded, pt_core.slang:580-613), one per lane.
-__global__ __launch_bounds__(kBxBlock) void k_bx_shadow(const BatchJob* __restrict__ jobs, BatchDev B, uint32_t d)
+__global__ __launch_bounds__(64) void k_bx_shadow(const BatchJob* __restrict__ jobs, BatchDev B, uint32_t d)
 {
     __shared__ uint32_t s_prefix[kBxMaxJobs + 1];
     const uint32_t total =
-        buildPrefix(threadIdx.x < B.njobs ? B.counts[threadIdx.x * kCountStride + kShadowOff + d] : 0u, B.njobs, s_prefix);
+        buildPrefixWave([&](uint32_t j) { return B.counts[j * kCountStride + kShadowOff + d]; }, B.njobs, s_prefix);
     const uint32_t lane = threadIdx.x & 63u;
     const size_t stride = (size_t)B.shK * B.n;
     const Frame F{};
-    __shared__ WaveTris waveTris[kBxWaves];
-    WaveTris& sh = waveTris[threadIdx.x >> 6];
-    for (uint32_t gw = blockIdx.x * kBxWaves + (threadIdx.x >> 6); gw < total; gw += gridDim.x * kBxWaves) {
+    __shared__ WaveTris sh;
+    for (uint32_t gw = blockIdx.x; gw < total; gw += gridDim.x) {
         const uint32_t j = findJob(s_prefix, B.njobs, gw);
         const BatchJob& J = jobs[j];
         const uint32_t count = B.counts[j * kCountStride + kShadowOff + d];
@@ -494,10 +529,10 @@ int issue_pass(vh_batch* b)
     if (b->anyAdaptive)
         k_bx_select<<<grid, kBxBlock, 0, s>>>(jobs, B);
     for (uint32_t d = 0; d < kMaxDepth; d++) {
-        k_bx_extend<<<grid, kBxBlock, 0, s>>>(jobs, B, d);
+        k_bx_extend<<<grid * kBxTraceScale, 64, 0, s>>>(jobs, B, d);
         if (B.shK > 0) {
             k_bx_shade<true><<<grid, kBxBlock, 0, s>>>(jobs, B, d);
-            k_bx_shadow<<<grid, kBxBlock, 0, s>>>(jobs, B, d);
+            k_bx_shadow<<<grid * kBxTraceScale, 64, 0, s>>>(jobs, B, d);
             k_bx_resolve<<<grid, kBxBlock, 0, s>>>(jobs, B, d);
         } else {
             k_bx_shade<false><<<grid, kBxBlock, 0, s>>>(jobs, B, d);

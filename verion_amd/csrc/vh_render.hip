@@ -349,13 +349,13 @@ __global__ __launch_bounds__(kWfBlock, VH_SHADE_WAVES) void k_wf_shade_deferred(
 // (shadePath's probe), then the wave resamples its lanes' neighbours together, tracing their
 // visibility rays with the wave-cooperative traversal (restirGIPrepareWave). The grid stride is
 // wave-uniform: every lane of a wave reaches the cooperative traversal.
-__global__ __launch_bounds__(kWfBlock, VH_SHADE_WAVES) void k_gi_prepare(Scene S, Frame F, Params P, Wavefront W)
+__global__ __launch_bounds__(kTraceBlock, VH_SHADE_WAVES) void k_gi_prepare(Scene S, Frame F, Params P, Wavefront W)
 {
     Ctx c{S, F, P};
     const uint32_t count = W.counts[0];
-    __shared__ WaveTris waveTris[kWfBlock / 64];
+    __shared__ WaveTris waveTris[kTraceBlock / 64];
     WaveTris& sh = waveTris[threadIdx.x >> 6];
-    for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += gridDim.x * kWfBlock) {
+    for (uint32_t base = blockIdx.x * kTraceBlock; base < count; base += gridDim.x * kTraceBlock) {
         uint32_t i = base + threadIdx.x;
         GiSite site;
         site.active = false;
@@ -481,13 +481,13 @@ __global__ void k_nrc_pick(uint32_t* trainPix, uint32_t nTrain, uint32_t npix, u
     }
 }
 
-__global__ __launch_bounds__(kWfBlock) void k_nrc_t_extend(Scene S, Params P, Wavefront W, const uint32_t* trainPix, uint32_t d)
+__global__ __launch_bounds__(kTraceBlock) void k_nrc_t_extend(Scene S, Params P, Wavefront W, const uint32_t* trainPix, uint32_t d)
 {
     uint32_t count = W.counts[d];
     const uint32_t* st = W.state[d & 1u];
-    __shared__ WaveTris waveTris[kWfBlock / 64];
+    __shared__ WaveTris waveTris[kTraceBlock / 64];
     WaveTris& sh = waveTris[threadIdx.x >> 6];
-    for (uint32_t base = blockIdx.x * kWfBlock; base < count; base += gridDim.x * kWfBlock) {
+    for (uint32_t base = blockIdx.x * kTraceBlock; base < count; base += gridDim.x * kTraceBlock) {
         uint32_t i = base + threadIdx.x;
         bool act = i < count;
         V3 org = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
@@ -805,7 +805,7 @@ int launch_wf_sample(const Scene& S, const Frame& F, const Params& P0, const Wav
             (void)hipEventRecord(ev[kStageEvents * d + 1], s);
         if (W.shK > 0 && !split) {
             if (gi && d == 0)
-                k_gi_prepare<<<grid, kWfBlock, 0, s>>>(S, F, P, W);
+                k_gi_prepare<<<grid * kTraceScale, kTraceBlock, 0, s>>>(S, F, P, W);
             if (gi)
                 k_wf_shade_deferred<true, true><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
             else if (restir && d == 0)
@@ -849,7 +849,7 @@ int launch_nrc_train_paths(const Scene& S, const Params& P, const Wavefront& W, 
     uint32_t grid = blocks_for(nTrain, kWfBlock, cfg.wfGrid ? cfg.wfGrid : 1024);
     k_nrc_pick<<<(nTrain + 255) / 256, 256, 0, s>>>(trainPix, nTrain, npix, frame, W.counts, sampleCount);
     for (uint32_t d = 0; d < P.w[7]; d++) {
-        k_nrc_t_extend<<<grid, kWfBlock, 0, s>>>(S, P, W, trainPix, d);
+        k_nrc_t_extend<<<grid * kTraceScale, kTraceBlock, 0, s>>>(S, P, W, trainPix, d);
         if (W.shK > 0) { // deferred shadow rays, traced by the wave-cooperative shadow kernel
             k_nrc_t_shade_deferred<<<grid, kWfBlock, 0, s>>>(S, P, W, trainPix, vertices, samples, sampleCount, sampleCap, d);
             k_wf_shadow<<<grid * kTraceScale, kTraceBlock, 0, s>>>(S, P, W, d);
