@@ -112,6 +112,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-nrc", action="store_true", help="skip the NRC vs plain time-to-relMSE measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the 32-job batch measurement")
+    ap.add_argument("--no-cornell", action="store_true", help="skip the Cornell box measurement (for profiler runs)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (8 blocks per CU, 8x8 tiles)")
     ap.add_argument("--shape", type=str, default=None, help="fixed launch shape BLOCKS,TILE_W,TILE_H instead of tuning")
     ap.add_argument("--nrc-ref-spp", type=int, default=4096,
@@ -210,7 +211,8 @@ def main():
         base, parity = cpu_baseline(packed, W, H, gpu1.film())
         line["cpu_baseline"] = base
         line["parity_full_size"] = parity
-    line["cornell_256"] = cornell_256(ctx)
+    if not args.no_cornell:
+        line["cornell_256"] = cornell_256(ctx)
     if not args.no_nrc:
         line["time_to_rmse"] = time_to_rmse(W, H, args.nrc_ref_spp)
     if not args.no_batch:
