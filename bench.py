@@ -11,7 +11,7 @@ Prints one JSON line with the throughput (`value`, plain wavefront), a roofline 
 dominant kernel (achieved algorithmic bytes per launch over its HIP-event launch time, against
 8 TB/s), the CPU oracle's throughput on a bounded sample of the same workload, and `time_to_rmse`:
 GPU seconds to relMSE 1e-3 of the neural radiance cache and of the plain tracer on the same scene
-at 1080p, 1 spp per frame, against a plain reference render (4096 spp by default); and `batch`:
+at 1080p, 1 spp per frame, against a plain reference render (16,384 spp, the workload's, by default); and `batch`:
 p50/p99 job latency and aggregate Msamples/s of 32 uneven jobs in one shared wavefront against
 one job at a time.
 
@@ -115,7 +115,7 @@ def main():
     ap.add_argument("--no-cornell", action="store_true", help="skip the Cornell box measurement (for profiler runs)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (8 blocks per CU, 8x8 tiles)")
     ap.add_argument("--shape", type=str, default=None, help="fixed launch shape BLOCKS,TILE_W,TILE_H instead of tuning")
-    ap.add_argument("--nrc-ref-spp", type=int, default=4096,
+    ap.add_argument("--nrc-ref-spp", type=int, default=16384,
                     help="spp of the plain reference render the relMSE ladders are scored against")
     ap.add_argument("--no-counting", action="store_true",
                     help="skip the traversal-counting pass (for profiler runs: every launch is then a timed-workload launch)")
