@@ -301,7 +301,11 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcNet net, const float4* sam
         }
         for (int k = 0; k < 24; k++)
             s_xT[(g * 24 + k) * kLdT + row] = xr[k];
-        __syncthreads();
+        // a wave's forward pass reads only its own 16 rows (row-major copies): a wave barrier
+        // until the output layer, whose loss step reads every row. The transposed copies are read
+        // by the backward pass, after that block barrier, and the next tile writes them only
+        // after the backward pass's last block barrier.
+        waveSync();
         // forward; B operands stream from the global bf16 copy (L2-resident, 47 KB)
 #pragma unroll
         for (int l = 0; l < kNL; l++) {
@@ -326,7 +330,10 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcNet net, const float4* sam
                     }
                 }
             }
-            __syncthreads();
+            if (l == kNL - 1)
+                __syncthreads();
+            else
+                waveSync();
         }
         // relative L2 loss gradient, d(loss)/dy = 2 (y - t) / (y^2 + 0.01) / B for RGB
         if (threadIdx.x < kRows) {
@@ -362,11 +369,13 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcNet net, const float4* sam
     storeLayer<5, 1>(part, a5, d5);
     s_loss[threadIdx.x] = lossAcc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        float s = 0.0f;
-        for (int i = 0; i < 256; i++)
-            s += s_loss[i];
-        lossPartials[blockIdx.x] = s;
+    if (threadIdx.x < 64) { // fixed order: four per lane, then a butterfly (lane 0's is kept)
+        float s = ((s_loss[threadIdx.x] + s_loss[threadIdx.x + 64]) + s_loss[threadIdx.x + 128]) + s_loss[threadIdx.x + 192];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            s += __shfl_xor(s, off, 64);
+        if (threadIdx.x == 0)
+            lossPartials[blockIdx.x] = s;
     }
 }
 
@@ -389,7 +398,8 @@ __global__ void k_nrc_reduce_groups(const float* partials, uint32_t blocks, floa
     groupSums[(size_t)g * kParams + p] = s;
 }
 
-// grad[p] = sum of the group sums in group order; loss likewise over the blocks' loss partials.
+// grad[p] = sum of the group sums in group order; the loss = the blocks' loss partials summed in a
+// fixed order.
 __global__ void k_nrc_reduce(const float* groupSums, uint32_t groups, float* grad, const float* lossPartials, uint32_t blocks,
                              float* loss, const uint32_t* countPtr, uint32_t countImm, uint32_t cap, uint32_t slice, uint32_t slices)
 {
@@ -400,18 +410,25 @@ __global__ void k_nrc_reduce(const float* groupSums, uint32_t groups, float* gra
             s += groupSums[(size_t)g * kParams + p];
         grad[p] = s;
     }
-    if (p == 0) {
-        uint32_t total = countPtr ? *countPtr : countImm;
-        if (total > cap)
-            total = cap;
-        uint32_t lo, hi;
-        sliceRange(total, slice, slices, lo, hi);
-        uint32_t n = hi - lo;
+    if (blockIdx.x == 0 && threadIdx.x < 64) {
+        // the loss partials by one wave in a fixed order: lane sums b = lane, lane + 64, ...;
+        // then a fixed butterfly over the lanes (lane 0's result is deterministic)
         float s = 0.0f;
-        for (uint32_t b = 0; b < blocks; b++)
+        for (uint32_t b = threadIdx.x; b < blocks; b += 64)
             s += lossPartials[b];
-        loss[0] = s / (float)(n > 0 ? n : 1);
-        loss[1] = (float)n;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            s += __shfl_xor(s, off, 64);
+        if (threadIdx.x == 0) {
+            uint32_t total = countPtr ? *countPtr : countImm;
+            if (total > cap)
+                total = cap;
+            uint32_t lo, hi;
+            sliceRange(total, slice, slices, lo, hi);
+            uint32_t n = hi - lo;
+            loss[0] = s / (float)(n > 0 ? n : 1);
+            loss[1] = (float)n;
+        }
     }
 }
 
