@@ -121,21 +121,27 @@ constexpr uint32_t kLeafGroup = 1u << kLeafBits;
 // test would take for an infinite box, so the traversal skips it by this reference.
 constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
 
-// Box test of the four children: returns a 4-bit mask of hits and their entry distances.
-__device__ __forceinline__ uint32_t slab4(const float4* n, float ix, float iy, float iz, float ox, float oy, float oz,
+// Box test of the four children: returns a 4-bit mask of hits and their entry distances. `near`
+// holds, per axis, which of the node's lo / hi rows the ray enters through (bit a: the ray's
+// inverse direction on axis a is negative, so it enters through hi), so each axis takes its entry
+// and exit planes by address instead of a min and a max per child: the same distances as
+// min(lo, hi) / max(lo, hi) of the two slab distances for every box (lo <= hi), and an empty slot
+// (lo = +inf, hi = -inf) never hits.
+__device__ __forceinline__ uint32_t slab4(const float4* n, uint32_t near, float ix, float iy, float iz, float ox, float oy, float oz,
                                           float tmax, float tn[4])
 {
-    float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
-    float a[6][4] = {{lx.x, lx.y, lx.z, lx.w}, {hx.x, hx.y, hx.z, hx.w}, {ly.x, ly.y, ly.z, ly.w},
-                     {hy.x, hy.y, hy.z, hy.w}, {lz.x, lz.y, lz.z, lz.w}, {hz.x, hz.y, hz.z, hz.w}};
+    const uint32_t sx = near & 1u, sy = (near >> 1) & 1u, sz = near >> 2;
+    float4 ex = n[sx], fx = n[1u - sx], ey = n[2u + sy], fy = n[3u - sy], ez = n[4u + sz], fz = n[5u - sz];
+    float a[6][4] = {{ex.x, ex.y, ex.z, ex.w}, {fx.x, fx.y, fx.z, fx.w}, {ey.x, ey.y, ey.z, ey.w},
+                     {fy.x, fy.y, fy.z, fy.w}, {ez.x, ez.y, ez.z, ez.w}, {fz.x, fz.y, fz.z, fz.w}};
     uint32_t mask = 0;
 #pragma unroll
     for (int c = 0; c < 4; c++) {
         float x0 = fmaf(a[0][c], ix, ox), x1 = fmaf(a[1][c], ix, ox);
         float y0 = fmaf(a[2][c], iy, oy), y1 = fmaf(a[3][c], iy, oy);
         float z0 = fmaf(a[4][c], iz, oz), z1 = fmaf(a[5][c], iz, oz);
-        float t0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-        float t1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax)) * 1.0000004f;
+        float t0 = fmaxf(fmaxf(x0, y0), fmaxf(z0, 0.0f));
+        float t1 = fminf(fminf(x1, y1), fminf(z1, tmax)) * 1.0000004f;
         tn[c] = t0;
         mask |= (t0 <= t1 ? 1u : 0u) << c;
     }
@@ -169,6 +175,7 @@ __device__ __forceinline__ void count_traversal(const Bvh& bvh, uint32_t visited
 struct TraceState {
     V3 o;
     float ix, iy, iz, ox, oy, oz;
+    uint32_t near; // per axis (bit a): the inverse direction is negative (slab4)
     RayPre pre;
     float tmax, best; // the ray's limit, and the closest accepted t (or tmax)
     Hit h;
@@ -191,6 +198,7 @@ __device__ __forceinline__ void traceBegin(TraceState& s, V3 o, V3 d, float tmax
     s.ox = -o.x * s.ix;
     s.oy = -o.y * s.iy;
     s.oz = -o.z * s.iz;
+    s.near = (s.ix < 0.0f ? 1u : 0u) | (s.iy < 0.0f ? 2u : 0u) | (s.iz < 0.0f ? 4u : 0u);
     s.tmax = tmax;
     s.best = tmax;
     s.node = 0;
@@ -226,7 +234,7 @@ __device__ __forceinline__ bool traceStep(const Bvh& bvh, TraceState& s, uint32_
     const float4* n = bvh.nodes + 8 * (size_t)s.node;
     s.visited++;
     float tn[4];
-    uint32_t mask = slab4(n, s.ix, s.iy, s.iz, s.ox, s.oy, s.oz, s.best, tn);
+    uint32_t mask = slab4(n, s.near, s.ix, s.iy, s.iz, s.ox, s.oy, s.oz, s.best, tn);
     uint4 child = *reinterpret_cast<const uint4*>(n + 6);
     uint4 count = *reinterpret_cast<const uint4*>(n + 7);
     const float kInf = __builtin_inff();
@@ -363,7 +371,7 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
         const float4* n = bvh.nodes + 8 * (size_t)s.node;
         s.visited++;
         float tn[4];
-        uint32_t mask = slab4(n, s.ix, s.iy, s.iz, s.ox, s.oy, s.oz, s.best, tn);
+        uint32_t mask = slab4(n, s.near, s.ix, s.iy, s.iz, s.ox, s.oy, s.oz, s.best, tn);
         uint4 child = *reinterpret_cast<const uint4*>(n + 6);
         uint4 count = *reinterpret_cast<const uint4*>(n + 7);
 #pragma unroll
