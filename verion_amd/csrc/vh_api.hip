@@ -724,21 +724,25 @@ int vh_accumulate(vh_renderer* r, vh_accumulation* a, uint32_t samples)
             P.w[31] = (uint32_t)n;
             rc = launch_wf_sample(S, F, P, W, restir, split, cfg, timing ? &stageEv[kStageEvents * kMaxDepth * k] : nullptr);
         }
-    } else if (!restir && !split && !a->adaptiveOn && samples > 1 && n < kParallelPaths) {
-        // a small image: several samples per dispatch in parallel, added to the film in order
+    } else if (!restir && !split && !a->adaptiveOn) {
+        // the plain megakernel: the samples of a dispatch (up to kParallelPaths paths) in parallel
+        // on persistent waves, added to the film in sample order (1080p procedural scene, 1 spp
+        // per call: 359 Msamples/s against 338 for one thread per pixel)
         uint32_t chunk = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(samples, 1024), std::max<uint64_t>(1, kParallelPaths / n));
-        uint64_t bytes = 12ull * n * chunk;
+        const uint64_t stagedBytes = (12ull * n * chunk + 255) & ~255ull;
+        uint64_t bytes = stagedBytes + 256; // + the persistent megakernel's work counter
         if (a->staged.bytes < bytes) {
             TRY(check_fits(bytes));
             HIPTRY(a->staged.alloc(bytes));
         }
+        uint32_t* next = reinterpret_cast<uint32_t*>(a->staged.as<char>() + stagedBytes);
         uint32_t k = 0;
         while (k < samples && rc == VH_OK) {
             Params P = base;
             P.w[6] = a->samples + k;
             uint32_t m = std::min(samples - k, chunk);
             P.w[35] = m;
-            rc = launch_megakernel_samples(S, F, P, m, a->staged.as<float>(), s);
+            rc = launch_megakernel_samples(S, F, P, m, a->staged.as<float>(), next, r->ctx->cus, s);
             k += m;
         }
     } else {

@@ -139,8 +139,10 @@ inline uint32_t blocks_for(uint64_t n, uint32_t block, uint32_t cap)
 }
 
 int launch_megakernel(const Scene& S, const Frame& F, const Params& P, uint32_t samples, bool restir, bool split, hipStream_t s);
-// Plain megakernel with `samples` samples in parallel (staged: 3 floats per pixel and sample).
-int launch_megakernel_samples(const Scene& S, const Frame& F, const Params& P, uint32_t samples, float* staged, hipStream_t s);
+// Plain megakernel with `samples` samples in parallel (staged: 3 floats per pixel and sample;
+// next: a work counter word).
+int launch_megakernel_samples(const Scene& S, const Frame& F, const Params& P, uint32_t samples, float* staged, uint32_t* next,
+                              uint32_t cus, hipStream_t s);
 // Paths in flight that fill the GPU: smaller images take several samples per dispatch in parallel.
 constexpr uint64_t kParallelPaths = 1u << 21;
 int launch_wf_sample(const Scene& S, const Frame& F, const Params& P, const Wavefront& W, bool restir, bool split, const LaunchCfg& cfg,
