@@ -736,13 +736,25 @@ int vh_accumulate(vh_renderer* r, vh_accumulation* a, uint32_t samples)
             HIPTRY(a->staged.alloc(bytes));
         }
         uint32_t* next = reinterpret_cast<uint32_t*>(a->staged.as<char>() + stagedBytes);
+        // deferred shadow rays, one record per delta light and one emitter or environment sample,
+        // for every resident lane (scenes with more lights trace them in the shading)
+        const uint32_t K = r->info.light_count + 1;
+        float* shRec = nullptr;
+        if (K <= kMaxShadow) {
+            uint64_t recBytes = 4ull * kShadowWords * K * (uint64_t)r->ctx->cus * kPersistWavesPerCu * 64;
+            if (a->persistRec.bytes < recBytes) {
+                TRY(check_fits(recBytes));
+                HIPTRY(a->persistRec.alloc(recBytes));
+            }
+            shRec = a->persistRec.as<float>();
+        }
         uint32_t k = 0;
         while (k < samples && rc == VH_OK) {
             Params P = base;
             P.w[6] = a->samples + k;
             uint32_t m = std::min(samples - k, chunk);
             P.w[35] = m;
-            rc = launch_megakernel_samples(S, F, P, m, a->staged.as<float>(), next, r->ctx->cus, s);
+            rc = launch_megakernel_samples(S, F, P, m, a->staged.as<float>(), next, shRec, K, r->ctx->cus, s);
             k += m;
         }
     } else {

@@ -139,10 +139,12 @@ inline uint32_t blocks_for(uint64_t n, uint32_t block, uint32_t cap)
 }
 
 int launch_megakernel(const Scene& S, const Frame& F, const Params& P, uint32_t samples, bool restir, bool split, hipStream_t s);
-// Plain megakernel with `samples` samples in parallel (staged: 3 floats per pixel and sample;
-// next: a work counter word).
+// Plain megakernel with `samples` samples in parallel on persistent waves (kPersistWavesPerCu per
+// CU; staged: 3 floats per pixel and sample; next: a work counter word; shRec: kShadowWords x shK
+// floats per resident lane for the deferred shadow rays, or null to trace them in the shading).
+constexpr uint32_t kPersistWavesPerCu = 12;
 int launch_megakernel_samples(const Scene& S, const Frame& F, const Params& P, uint32_t samples, float* staged, uint32_t* next,
-                              uint32_t cus, hipStream_t s);
+                              float* shRec, uint32_t shK, uint32_t cus, hipStream_t s);
 // Paths in flight that fill the GPU: smaller images take several samples per dispatch in parallel.
 constexpr uint64_t kParallelPaths = 1u << 21;
 int launch_wf_sample(const Scene& S, const Frame& F, const Params& P, const Wavefront& W, bool restir, bool split, const LaunchCfg& cfg,
@@ -234,6 +236,7 @@ struct vh_accumulation {
     vh::DevBuf film, stats, light, restirBuf, state0, state1, hits, queue0, counts;
     vh::DevBuf shRec, shVis, shQueue, shNRec, shOut, shPixel, shTotal; // deferred shadow rays
     vh::DevBuf staged; // the megakernel's per-sample radiance when samples run in parallel
+    vh::DevBuf persistRec; // the persistent megakernel's shadow records (per resident lane)
     uint32_t shK = 0;
     uint32_t samples = 0;
     double gpuSeconds = 0.0;

@@ -75,20 +75,27 @@ __global__ __launch_bounds__(64) void k_pathtrace(Scene S, Frame F, Params P, ui
 // (pixel, sample) items from `next` (one atomic per wave, items in sample order, 8 x 8 tiles
 // within a sample), each lane carries its path one depth per step, and a lane whose path ended
 // stages its radiance (zero when not finite, as finishPath does) and takes the next item at the
-// following step, so the wave's lanes stay busy instead of waiting for its longest path.
-// k_film_add_samples then adds the staged samples to the FP64 film in sample order, the additions
-// finishPath makes one sample at a time, so the film is the same bit for bit as k_pathtrace's.
-// Plain integrator only (no adaptive statistics, lighting split or ReSTIR, which need the
-// samples in order). A wave ends when the items are drained and its lanes' paths are done (at
-// most P.w[7] steps after its last fetch). Cornell box 256 x 256 x 16 spp: 411 Msamples/s with
-// one thread per (pixel, sample), 677 with persistent waves.
+// following step, so the wave's lanes stay busy instead of waiting for its longest path. The
+// loop is wave-uniform, so every ray is traced by the whole wave: the closest hits with the
+// wave-cooperative traversal and, with DEFER, the NEE shadow rays the shading records (one
+// record slot of the wave at a time, the visible contributions added in record order, as
+// k_wf_shadow and k_wf_resolve do). k_film_add_samples then adds the staged samples to the FP64
+// film in sample order, the additions finishPath makes one sample at a time, so the film is the
+// same bit for bit as k_pathtrace's. Plain integrator only (no adaptive statistics, lighting
+// split or ReSTIR, which need the samples in order). A wave ends when the items are drained and
+// its lanes' paths are done (at most P.w[7] steps after its last fetch). Shadow records of lane l
+// of the grid: shRec[(word K + record) lanes + l].
+template <bool DEFER>
 __global__ __launch_bounds__(64) void k_pathtrace_persist(Scene S, Params P, float* __restrict__ staged, uint32_t* __restrict__ next,
-                                                          uint32_t items)
+                                                          uint32_t items, float* __restrict__ shRec, uint32_t shK)
 {
     Frame F{};
     Ctx c{S, F, P};
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t w = P.w[0], npix = P.w[0] * P.w[1];
+    const uint32_t slot = blockIdx.x * 64 + lane, lanes = gridDim.x * 64;
+    const size_t stride = (size_t)shK * lanes;
+    __shared__ WaveTris sh;
     PathState st;
     uint32_t pixel = 0, k = 0, depth = 0;
     bool have = false, drained = false;
@@ -113,18 +120,46 @@ __global__ __launch_bounds__(64) void k_pathtrace_persist(Scene S, Params P, flo
         }
         if (__ballot(have) == 0)
             break;
+        Hit h = traceWave<false>(S.bvh, have ? st.org : v3(0.0f), have ? st.dir : v3(0.0f, 0.0f, 1.0f), 1.0e30f, have, sh);
+        bool alive = false;
+        uint32_t nrec = 0;
         if (have) {
-            Hit h = trace<false>(S.bvh, st.org, st.dir, 1.0e30f);
-            const bool alive = shadePath<false, false>(c, st, h, depth, pixel) && depth + 1 < P.w[7];
+            ShadowSink sink{shRec, lanes, shK, slot, 0u};
+            alive = shadePath<false, false, 0, DEFER>(c, st, h, depth, pixel, nullptr, DEFER ? &sink : nullptr) && depth + 1 < P.w[7];
+            nrec = sink.count;
             depth++;
-            if (!alive) {
-                V3 radiance = bad(st.parts.total) ? v3(0.0f) : st.parts.total;
-                float* o = staged + 3 * ((size_t)k * npix + pixel);
-                o[0] = radiance.x;
-                o[1] = radiance.y;
-                o[2] = radiance.z;
-                have = false;
+        }
+        if constexpr (DEFER) {
+            for (uint32_t r = 0; r < shK; r++) {
+                const bool act = r < nrec;
+                if (__ballot(act) == 0) // wave-uniform
+                    break;
+                const float* rec = shRec + (size_t)r * lanes + slot;
+                V3 o = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
+                float tmax = 1.0e30f;
+                if (act) {
+                    o = v3(rec[0], rec[stride], rec[2 * stride]);
+                    dir = v3(rec[3 * stride], rec[4 * stride], rec[5 * stride]);
+                    tmax = rec[6 * stride];
+                }
+                bool blocked;
+                if (S.translucent) {
+                    blocked = act && occluded(c, o, dir, tmax);
+                } else {
+                    Hit hs = traceWave<true>(S.bvh, o, dir, tmax, act, sh);
+                    blocked = act && hs.hit;
+                }
+                if (act && !blocked)
+                    st.parts.total = st.parts.total + v3(rec[7 * stride], rec[8 * stride], rec[9 * stride]);
             }
+        }
+        if (have && !alive) {
+            V3 radiance = bad(st.parts.total) ? v3(0.0f) : st.parts.total;
+            float* o = staged + 3 * ((size_t)k * npix + pixel);
+            o[0] = radiance.x;
+            o[1] = radiance.y;
+            o[2] = radiance.z;
+            have = false;
         }
     }
 }
@@ -814,15 +849,18 @@ int launch_megakernel(const Scene& S, const Frame& F, const Params& P, uint32_t 
 }
 
 int launch_megakernel_samples(const Scene& S, const Frame& F, const Params& P, uint32_t samples, float* staged, uint32_t* next,
-                              uint32_t cus, hipStream_t s)
+                              float* shRec, uint32_t shK, uint32_t cus, hipStream_t s)
 {
-    // resident waves: 12 per CU (Cornell box 256 x 256 x 16 spp: 12, 24 and 48 per CU gave 677,
-    // 672 and 647 Msamples/s)
+    // resident waves: kPersistWavesPerCu = 12 per CU (Cornell box 256 x 256 x 16 spp: 12, 24 and
+    // 48 per CU gave 677, 672 and 647 Msamples/s)
     const uint32_t npix = P.w[0] * P.w[1];
     const uint32_t items = npix * samples;
-    const uint32_t waves = std::min<uint32_t>((items + 63) / 64, cus * 12);
+    const uint32_t waves = std::min<uint32_t>((items + 63) / 64, cus * kPersistWavesPerCu);
     k_zero_u32<<<1, 64, 0, s>>>(next);
-    k_pathtrace_persist<<<waves, 64, 0, s>>>(S, P, staged, next, items);
+    if (shRec && P.w[5] + 1 <= shK)
+        k_pathtrace_persist<true><<<waves, 64, 0, s>>>(S, P, staged, next, items, shRec, shK);
+    else
+        k_pathtrace_persist<false><<<waves, 64, 0, s>>>(S, P, staged, next, items, nullptr, 0u);
     k_film_add_samples<<<blocks_for(npix, 256, 4096), 256, 0, s>>>(F.film, npix, samples, staged);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("megakernel samples: ") + hipGetErrorString(e));
