@@ -211,6 +211,8 @@ def main():
         base, parity = cpu_baseline(packed, W, H, gpu1.film())
         line["cpu_baseline"] = base
         line["parity_full_size"] = parity
+    if args.integrator == "wavefront":
+        line["megakernel_1080p"] = megakernel_same_scene(r, W, H)
     if not args.no_cornell:
         line["cornell_256"] = cornell_256(ctx)
     if not args.no_nrc:
@@ -218,6 +220,27 @@ def main():
     if not args.no_batch:
         line["batch"] = batch_jobs()
     print(json.dumps(line), flush=True)
+
+
+def megakernel_same_scene(r, W, H, steps=16):
+    """The other integrator on the same scene and image: the plain megakernel (persistent waves,
+    every ray traced wave-cooperatively), 1 spp per call like the timed steps, after 2 untimed."""
+    from verion_amd.backend import MEGAKERNEL, WAVEFRONT
+
+    r.set_integrator(MEGAKERNEL)
+    r.set_stage_timing(False)
+    acc = r.new_accumulation(W, H)
+    for _ in range(2):
+        r.accumulate(acc, 1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.accumulate(acc, 1)
+    dt = time.perf_counter() - t0
+    acc.close()
+    r.set_integrator(WAVEFRONT)
+    out = {"msamples_per_s": round(steps * W * H / dt / 1e6, 1), "steps": steps}
+    log(f"megakernel at {W}x{H}: {out}")
+    return out
 
 
 def cornell_256(ctx):

@@ -142,7 +142,7 @@ int launch_megakernel(const Scene& S, const Frame& F, const Params& P, uint32_t 
 // Plain megakernel with `samples` samples in parallel on persistent waves (kPersistWavesPerCu per
 // CU; staged: 3 floats per pixel and sample; next: a work counter word; shRec: kShadowWords x shK
 // floats per resident lane for the deferred shadow rays, or null to trace them in the shading).
-constexpr uint32_t kPersistWavesPerCu = 12;
+constexpr uint32_t kPersistWavesPerCu = 16; // 4 per SIMD, the kernel's occupancy target
 int launch_megakernel_samples(const Scene& S, const Frame& F, const Params& P, uint32_t samples, float* staged, uint32_t* next,
                               float* shRec, uint32_t shK, uint32_t cus, hipStream_t s);
 // Paths in flight that fill the GPU: smaller images take several samples per dispatch in parallel.

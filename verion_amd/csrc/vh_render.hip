@@ -84,9 +84,15 @@ __global__ __launch_bounds__(64) void k_pathtrace(Scene S, Frame F, Params P, ui
 // same bit for bit as k_pathtrace's. Plain integrator only (no adaptive statistics, lighting
 // split or ReSTIR, which need the samples in order). A wave ends when the items are drained and
 // its lanes' paths are done (at most P.w[7] steps after its last fetch). Shadow records of lane l
-// of the grid: shRec[(word K + record) lanes + l].
+// of the grid: shRec[(word K + record) lanes + l]. Cornell box 256 x 256 x 16 spp: 411
+// Msamples/s with one thread per (pixel, sample), 677 with persistent waves, 857 with every ray
+// traced wave-cooperatively, ~1000 at 4 waves per SIMD.
+// Occupancy target 4 waves per SIMD (128 VGPRs, 80 B of spills), 16 resident waves per CU.
+// Measured (Cornell 256 x 256 x 16 spp / 1080p procedural, 1 spp per call, Msamples/s): 2 per
+// SIMD (177 VGPRs) 848 / 471, 3 (168) 1001 / 547, 4 (128) 997-1039 / 566-568, 5 (96) 840 / 540,
+// 6 (80) 794 / 508.
 template <bool DEFER>
-__global__ __launch_bounds__(64) void k_pathtrace_persist(Scene S, Params P, float* __restrict__ staged, uint32_t* __restrict__ next,
+__global__ __launch_bounds__(64, 4) void k_pathtrace_persist(Scene S, Params P, float* __restrict__ staged, uint32_t* __restrict__ next,
                                                           uint32_t items, float* __restrict__ shRec, uint32_t shK)
 {
     Frame F{};
@@ -851,8 +857,7 @@ int launch_megakernel(const Scene& S, const Frame& F, const Params& P, uint32_t 
 int launch_megakernel_samples(const Scene& S, const Frame& F, const Params& P, uint32_t samples, float* staged, uint32_t* next,
                               float* shRec, uint32_t shK, uint32_t cus, hipStream_t s)
 {
-    // resident waves: kPersistWavesPerCu = 12 per CU (Cornell box 256 x 256 x 16 spp: 12, 24 and
-    // 48 per CU gave 677, 672 and 647 Msamples/s)
+    // resident waves: kPersistWavesPerCu per CU, the kernel's occupancy
     const uint32_t npix = P.w[0] * P.w[1];
     const uint32_t items = npix * samples;
     const uint32_t waves = std::min<uint32_t>((items + 63) / 64, cus * kPersistWavesPerCu);
