@@ -290,3 +290,9 @@ def test_many_delta_lights_shade_fused_and_match_the_oracle(ctx):
     assert acc.wavefront_bytes() == 2 * 4 * 17 * 576 + 16 * 576 + 4 * 576 + 128  # no shadow records
     st = oracle.OracleScene(r.packed).render(params(r.packed, 24, 24), 2)
     assert np.array_equal(acc.film(), st["film"])
+    # the persistent megakernel likewise traces them in the shading (its non-deferred form)
+    r.set_integrator(MEGAKERNEL)
+    mega = r.new_accumulation(24, 24)
+    r.accumulate(mega, 2)
+    r.set_integrator(WAVEFRONT)
+    assert np.array_equal(mega.film(), st["film"])
