@@ -65,6 +65,7 @@ struct BatchDev {
     uint32_t n;                    // slots
     uint32_t njobs;
     uint32_t tileW, tileH;         // depth-0 pixel order; tileW = 0: scanlines
+    uint32_t traceWaves;           // waves a short queue is spread over (raysPerWave)
     // deferred shadow rays (shK = 0: shade traces them itself), as in the single-accumulation
     // wavefront; job j's shadow queue is the segment [shK slotBase_j, shK (slotBase_j + cap_j))
     uint32_t shK;
@@ -105,18 +106,30 @@ __device__ uint32_t buildPrefix(uint32_t work, uint32_t njobs, uint32_t* s_prefi
 }
 
 // buildPrefix for a block of one wave: each lane takes kPer consecutive jobs. work(j) is job j's
-// live count; the prefix is over its 64-slot waves.
+// live count; the prefix is over its waves of `per` rays, with per = raysPerWave(the live count of
+// all jobs, traceWaves) (wave-uniform), which it also returns in *perOut.
 template <class Work>
-__device__ uint32_t buildPrefixWave(Work work, uint32_t njobs, uint32_t* s_prefix)
+__device__ uint32_t buildPrefixWave(Work work, uint32_t njobs, uint32_t traceWaves, uint32_t* s_prefix, uint32_t* perOut)
 {
     constexpr uint32_t kPer = (kBxMaxJobs + 1 + 63) / 64;
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t x[kPer];
-    uint32_t sum = 0;
+    uint32_t rays = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) {
         uint32_t j = lane * kPer + k;
-        x[k] = j < njobs ? (work(j) + 63u) / 64u : 0u;
+        x[k] = j < njobs ? work(j) : 0u;
+        rays += x[k];
+    }
+#pragma unroll
+    for (uint32_t off = 32; off > 0; off >>= 1)
+        rays += (uint32_t)__shfl_xor((int)rays, off, 64);
+    const uint32_t per = raysPerWave(rays, traceWaves);
+    *perOut = per;
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; k++) {
+        x[k] = (x[k] + per - 1u) / per;
         sum += x[k];
     }
     uint32_t incl = sum;
@@ -238,7 +251,9 @@ __global__ __launch_bounds__(64) void k_bx_extend(const BatchJob* __restrict__ j
 {
     __shared__ uint32_t s_prefix[kBxMaxJobs + 1];
     const uint32_t pass = *B.pass;
-    const uint32_t total = buildPrefixWave([&](uint32_t j) { return B.counts[j * kCountStride + d]; }, B.njobs, s_prefix);
+    uint32_t per;
+    const uint32_t total =
+        buildPrefixWave([&](uint32_t j) { return B.counts[j * kCountStride + d]; }, B.njobs, B.traceWaves, s_prefix, &per);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t* st = B.state[d & 1u];
     __shared__ WaveTris sh;
@@ -246,8 +261,8 @@ __global__ __launch_bounds__(64) void k_bx_extend(const BatchJob* __restrict__ j
         const uint32_t j = findJob(s_prefix, B.njobs, gw);
         const BatchJob& J = jobs[j];
         const uint32_t count = B.counts[j * kCountStride + d];
-        const uint32_t i = (gw - s_prefix[j]) * 64 + lane;
-        const bool act = i < count; // the wave's lanes trace together (traceWave); idle lanes help
+        const uint32_t i = (gw - s_prefix[j]) * per + lane;
+        const bool act = lane < per && i < count; // the wave's lanes trace together (traceWave); idle lanes help
         const uint32_t slot = J.slotBase + i;
         V3 org = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
         if (act && d == 0) {
@@ -336,8 +351,9 @@ __global__ __launch_bounds__(kBxBlock) void k_bx_shade(const BatchJob* __restric
 __global__ __launch_bounds__(64) void k_bx_shadow(const BatchJob* __restrict__ jobs, BatchDev B, uint32_t d)
 {
     __shared__ uint32_t s_prefix[kBxMaxJobs + 1];
-    const uint32_t total =
-        buildPrefixWave([&](uint32_t j) { return B.counts[j * kCountStride + kShadowOff + d]; }, B.njobs, s_prefix);
+    uint32_t per;
+    const uint32_t total = buildPrefixWave([&](uint32_t j) { return B.counts[j * kCountStride + kShadowOff + d]; }, B.njobs,
+                                           B.traceWaves, s_prefix, &per);
     const uint32_t lane = threadIdx.x & 63u;
     const size_t stride = (size_t)B.shK * B.n;
     const Frame F{};
@@ -346,8 +362,8 @@ __global__ __launch_bounds__(64) void k_bx_shadow(const BatchJob* __restrict__ j
         const uint32_t j = findJob(s_prefix, B.njobs, gw);
         const BatchJob& J = jobs[j];
         const uint32_t count = B.counts[j * kCountStride + kShadowOff + d];
-        const uint32_t i = (gw - s_prefix[j]) * 64 + lane;
-        const bool act = i < count;
+        const uint32_t i = (gw - s_prefix[j]) * per + lane;
+        const bool act = lane < per && i < count;
         const Ctx c{J.S, F, J.P};
         uint32_t rid = act ? B.shQueue[B.shK * J.slotBase + i] : 0u;
         const float* r = B.shRec + rid;
@@ -481,6 +497,7 @@ struct vh_batch {
         B.njobs = (uint32_t)jobs.size();
         B.tileW = tuning.tile_width;
         B.tileH = tuning.tile_height;
+        B.traceWaves = ctx->cus * kTraceWavesPerCu;
         B.shK = shK;
         B.shRec = shRec.as<float>();
         B.shVis = shVis.as<uint32_t>();
@@ -528,11 +545,12 @@ int issue_pass(vh_batch* b)
     k_bx_begin<<<1, kBxBlock, 0, s>>>(jobs, B);
     if (b->anyAdaptive)
         k_bx_select<<<grid, kBxBlock, 0, s>>>(jobs, B);
+    const uint32_t tgrid = std::max(grid * kBxTraceScale, B.traceWaves); // one-wave blocks
     for (uint32_t d = 0; d < kMaxDepth; d++) {
-        k_bx_extend<<<grid * kBxTraceScale, 64, 0, s>>>(jobs, B, d);
+        k_bx_extend<<<tgrid, 64, 0, s>>>(jobs, B, d);
         if (B.shK > 0) {
             k_bx_shade<true><<<grid, kBxBlock, 0, s>>>(jobs, B, d);
-            k_bx_shadow<<<grid * kBxTraceScale, 64, 0, s>>>(jobs, B, d);
+            k_bx_shadow<<<tgrid, 64, 0, s>>>(jobs, B, d);
             k_bx_resolve<<<grid, kBxBlock, 0, s>>>(jobs, B, d);
         } else {
             k_bx_shade<false><<<grid, kBxBlock, 0, s>>>(jobs, B, d);

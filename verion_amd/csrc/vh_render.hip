@@ -304,10 +304,13 @@ __global__ __launch_bounds__(kTraceBlock, VH_EXTEND_WAVES) void k_wf_extend(Scen
     auto done = [&](uint32_t i, const Hit& h) { W.hits[i] = make_uint4(h.hit ? 1u : 0u, h.inst, h.prim, asu(h.t)); };
     __shared__ WaveTris waveTris[kTraceBlock / 64];
     WaveTris& sh = waveTris[threadIdx.x >> 6];
-    // wave-uniform grid stride: the 64 lanes of a wave trace together (traceWave)
-    for (uint32_t base = blockIdx.x * kTraceBlock; base < count; base += gridDim.x * kTraceBlock) {
+    // wave-uniform grid stride: the 64 lanes of a wave trace together (traceWave), `per` of them
+    // with rays of their own
+    static_assert(kTraceBlock == 64, "one wave per block");
+    const uint32_t per = raysPerWave(count, W.traceWaves);
+    for (uint32_t base = blockIdx.x * per; base < count; base += gridDim.x * per) {
         uint32_t i = base + threadIdx.x;
-        bool act = i < count;
+        bool act = threadIdx.x < per && i < count;
         V3 org = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
         float tmax = 1.0e30f;
         if (act)
@@ -474,9 +477,10 @@ __global__ __launch_bounds__(kTraceBlock, VH_EXTEND_WAVES) void k_wf_shadow(Scen
     if (!S.translucent) { // any hit ends an opaque shadow ray: the wave-cooperative traversal
         __shared__ WaveTris waveTris[kTraceBlock / 64];
         WaveTris& sh = waveTris[threadIdx.x >> 6];
-        for (uint32_t base = blockIdx.x * kTraceBlock; base < count; base += gridDim.x * kTraceBlock) {
+        const uint32_t per = raysPerWave(count, W.traceWaves);
+        for (uint32_t base = blockIdx.x * per; base < count; base += gridDim.x * per) {
             uint32_t q = base + threadIdx.x;
-            bool act = q < count;
+            bool act = threadIdx.x < per && q < count;
             V3 o = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
             float tmax = 1.0e30f;
             if (act)
@@ -567,9 +571,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_nrc_t_extend(Scene S, Params P,
     const uint32_t* st = W.state[d & 1u];
     __shared__ WaveTris waveTris[kTraceBlock / 64];
     WaveTris& sh = waveTris[threadIdx.x >> 6];
-    for (uint32_t base = blockIdx.x * kTraceBlock; base < count; base += gridDim.x * kTraceBlock) {
+    const uint32_t per = raysPerWave(count, W.traceWaves);
+    for (uint32_t base = blockIdx.x * per; base < count; base += gridDim.x * per) {
         uint32_t i = base + threadIdx.x;
-        bool act = i < count;
+        bool act = threadIdx.x < per && i < count;
         V3 org = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
         if (act && d == 0) {
             uint32_t p = trainPix[i];
@@ -876,6 +881,7 @@ int launch_wf_sample(const Scene& S, const Frame& F, const Params& P0, const Wav
 {
     hipStream_t s = cfg.stream;
     uint32_t grid = cfg.wfGrid ? cfg.wfGrid : 1024;
+    const uint32_t tgrid = std::max(grid * kTraceScale, W.traceWaves); // one-wave blocks
     Params P = P0;
     uint32_t maxDepth = P.w[7];
     // ReSTIR DI defers its winner's shadow ray with the others; GI's reuse runs first, in k_gi_prepare
@@ -887,7 +893,7 @@ int launch_wf_sample(const Scene& S, const Frame& F, const Params& P0, const Wav
         P.w[30] = d;
         if (ev)
             (void)hipEventRecord(ev[kStageEvents * d], s);
-        k_wf_extend<<<grid * kTraceScale, kTraceBlock, 0, s>>>(S, P, W, d);
+        k_wf_extend<<<tgrid, kTraceBlock, 0, s>>>(S, P, W, d);
         if (ev)
             (void)hipEventRecord(ev[kStageEvents * d + 1], s);
         if (W.shK > 0 && !split) {
@@ -901,7 +907,7 @@ int launch_wf_sample(const Scene& S, const Frame& F, const Params& P0, const Wav
                 k_wf_shade_deferred<false, false><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
             if (ev)
                 (void)hipEventRecord(ev[kStageEvents * d + 2], s);
-            k_wf_shadow<<<grid * kTraceScale, kTraceBlock, 0, s>>>(S, P, W, d);
+            k_wf_shadow<<<tgrid, kTraceBlock, 0, s>>>(S, P, W, d);
             if (ev)
                 (void)hipEventRecord(ev[kStageEvents * d + 3], s);
             if (gi)
@@ -935,11 +941,12 @@ int launch_nrc_train_paths(const Scene& S, const Params& P, const Wavefront& W, 
     hipStream_t s = cfg.stream;
     uint32_t grid = blocks_for(nTrain, kWfBlock, cfg.wfGrid ? cfg.wfGrid : 1024);
     k_nrc_pick<<<(nTrain + 255) / 256, 256, 0, s>>>(trainPix, nTrain, npix, frame, W.counts, sampleCount);
+    const uint32_t tgrid = std::max(grid * kTraceScale, W.traceWaves); // one-wave blocks
     for (uint32_t d = 0; d < P.w[7]; d++) {
-        k_nrc_t_extend<<<grid * kTraceScale, kTraceBlock, 0, s>>>(S, P, W, trainPix, d);
+        k_nrc_t_extend<<<tgrid, kTraceBlock, 0, s>>>(S, P, W, trainPix, d);
         if (W.shK > 0) { // deferred shadow rays, traced by the wave-cooperative shadow kernel
             k_nrc_t_shade_deferred<<<grid, kWfBlock, 0, s>>>(S, P, W, trainPix, vertices, samples, sampleCount, sampleCap, d);
-            k_wf_shadow<<<grid * kTraceScale, kTraceBlock, 0, s>>>(S, P, W, d);
+            k_wf_shadow<<<tgrid, kTraceBlock, 0, s>>>(S, P, W, d);
             k_nrc_t_resolve<<<grid, kWfBlock, 0, s>>>(W, vertices, samples, sampleCount, sampleCap, d);
         } else {
             k_nrc_t_shade<<<grid, kWfBlock, 0, s>>>(S, P, W, trainPix, vertices, samples, sampleCount, sampleCap, d);
@@ -961,10 +968,11 @@ int launch_wf_depth(const Scene& S, const Frame& F, const Params& P0, const Wave
     Params P = P0;
     P.w[30] = d;
     uint32_t grid = cfg.wfGrid ? cfg.wfGrid : 1024;
-    k_wf_extend<<<grid * kTraceScale, kTraceBlock, 0, cfg.stream>>>(S, P, W, d);
+    const uint32_t tgrid = std::max(grid * kTraceScale, W.traceWaves); // one-wave blocks
+    k_wf_extend<<<tgrid, kTraceBlock, 0, cfg.stream>>>(S, P, W, d);
     if (shade && W.shK > 0) {
         k_wf_shade_deferred<false, false><<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
-        k_wf_shadow<<<grid * kTraceScale, kTraceBlock, 0, cfg.stream>>>(S, P, W, d);
+        k_wf_shadow<<<tgrid, kTraceBlock, 0, cfg.stream>>>(S, P, W, d);
         k_wf_resolve<false><<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
     } else if (shade) {
         k_wf_shade<false, false><<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);

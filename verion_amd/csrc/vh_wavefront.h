@@ -11,6 +11,17 @@ namespace vh {
 
 constexpr uint32_t kBaseWords = 17; // 16 words of wf_state.slang:75 + the pixel id; WF_SPLIT (17) more follow
 
+// Rays per wave of a trace kernel. A wave's traversal lasts as long as its slowest ray, so a queue
+// too short to fill `waves` waves of 64 rays is spread over that many waves, fewer rays to each;
+// waves == 0: 64. Wave-uniform.
+__device__ __forceinline__ uint32_t raysPerWave(uint32_t count, uint32_t waves)
+{
+    if (waves == 0)
+        return 64u;
+    uint32_t per = (count + waves - 1) / waves;
+    return per < 1u ? 1u : (per > 64u ? 64u : per);
+}
+
 // Depth 0 takes pixels in tile order rather than scanline order, so that a wave's 64 camera rays
 // (and their first bounces) come from one compact patch and share BVH nodes. The image does not
 // depend on the order: a pixel takes one path per sample pass.
