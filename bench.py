@@ -64,7 +64,9 @@ def algorithmic_bytes(c: dict, pixels: int, delta_lights: int, env: bool):
     wf_shadow: per shadow ray its queue entry (4 B), origin, direction and tmax (28 B), the
                visibility word (4 B), and the nodes and triangles it visits; the resolve step
                (per shadow ray the visibility and contribution, 16 B, plus the pending record) is
-               counted with it.
+               counted with it;
+    wf_trace:  the launch that traces depth d's closest rays and depth d - 1's shadow rays
+               (opaque scenes): wf_extend's bytes and wf_shadow's without the resolve step.
     """
     rays = c["closest_rays"]
     hits = c["closest_hits"]
@@ -73,8 +75,9 @@ def algorithmic_bytes(c: dict, pixels: int, delta_lights: int, env: bool):
     extend = 24 * later + 16 * rays + NODE_BYTES * c["closest_nodes"] + TRI_BYTES * c["closest_tris"]
     shade = (16 * rays + 4 * STATE_WORDS * later + 4 * STATE_WORDS * later + hits * (244 + 64 * delta_lights + (48 if env else 0))
              + 44 * shadow + 24 * hits + 48 * pixels)
-    shadow_b = (4 + 28 + 4 + 16) * shadow + 24 * hits + NODE_BYTES * c["shadow_nodes"] + TRI_BYTES * c["shadow_tris"]
-    return {"wf_extend": extend, "wf_shade": shade, "wf_shadow": shadow_b}
+    shadow_rays = (4 + 28 + 4) * shadow + NODE_BYTES * c["shadow_nodes"] + TRI_BYTES * c["shadow_tris"]
+    shadow_b = shadow_rays + 16 * shadow + 24 * hits
+    return {"wf_extend": extend, "wf_shade": shade, "wf_shadow": shadow_b, "wf_trace": extend + shadow_rays}
 
 
 def cpu_baseline(packed, width, height, gpu_film):
@@ -168,7 +171,7 @@ def main():
     stage = acc.stage_seconds()
     kernel_s = acc.kernel_seconds()
     value = args.steps * pixels / elapsed / 1e6
-    log(f"timed {args.steps} steps in {elapsed:.3f} s; extend, shade, shadow, resolve seconds {kernel_s}")
+    log(f"timed {args.steps} steps in {elapsed:.3f} s; extend, shade, shadow, resolve, trace seconds {kernel_s}")
 
     # traversal counters of one more pass (outside the timed region) for the algorithmic bytes
     if args.no_counting:
@@ -183,18 +186,22 @@ def main():
     log(f"counters per pass: {counts}")
     delta_lights = packed.info.light_count
     algo = algorithmic_bytes(counts, pixels, delta_lights, packed.info.env_width > 0)
-    launches = args.steps * 8
-    names = ("wf_extend", "wf_shade", "wf_shadow", "wf_resolve")
+    names = ("wf_extend", "wf_shade", "wf_shadow", "wf_resolve", "wf_trace")
     per = dict(zip(names, kernel_s)) if kernel_s else {k: float("nan") for k in names}
-    dom = max(("wf_extend", "wf_shade", "wf_shadow"), key=lambda k: per[k])
-    dom_s = per[dom] / launches
-    dom_b = algo[dom] / 8
+    # launches per pass: one per depth; the trace kernel runs once more for the last depth's shadow rays
+    per_pass = {k: 8 for k in names}
+    per_pass["wf_trace"] = 9
+    ran = [k for k in names if per[k] > 0]
+    dom = max([k for k in ("wf_trace", "wf_extend", "wf_shade", "wf_shadow") if k in ran] or ["wf_extend"],
+              key=lambda k: per[k])
+    dom_s = per[dom] / (args.steps * per_pass[dom])
+    dom_b = algo[dom] / per_pass[dom]
     achieved = dom_b / dom_s / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom),
                 "launch_ms": round(dom_s * 1e3, 4), "algorithmic_bytes_per_launch": int(dom_b),
-                "ms_per_launch": {k: round(per[k] / launches * 1e3, 4) for k in names},
-                "algorithmic_gb_per_s": {k: round(algo[k] / 8 / (per[k] / launches) / 1e9, 2) for k in algo}}
+                "ms_per_launch": {k: round(per[k] / (args.steps * per_pass[k]) * 1e3, 4) for k in ran},
+                "algorithmic_gb_per_s": {k: round(algo[k] / (per[k] / args.steps) / 1e9, 2) for k in algo if k in ran}}
 
     line = {"metric": "Msamples/s at 1080p + time-to-rMSE 1e-3: NRC vs plain wavefront, 1 MI355X", "value": round(value, 3),
             "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -211,12 +211,16 @@ int vh_accumulate(vh_renderer* r, vh_accumulation* acc, uint32_t samples);
 
 uint32_t vh_acc_samples(const vh_accumulation* acc);
 double vh_acc_gpu_seconds(const vh_accumulation* acc);
-/* Extend and shade GPU seconds of the wavefront when stage timing was on; VH_ERR_UNSUPPORTED otherwise. */
+/* Extend and shade GPU seconds of the wavefront when stage timing was on; VH_ERR_UNSUPPORTED otherwise.
+ * Where one launch traces closest and shadow rays (opaque scenes with deferred shadow rays), its
+ * time counts as extend. */
 int vh_acc_stage_seconds(const vh_accumulation* acc, double out[2]);
 /* Debug export (no counterpart in the reference): the same GPU seconds split by kernel: extend,
- * shade, the deferred shadow rays and their resolve (the last two are 0 when shade traces its
- * shadow rays itself: ReSTIR, the lighting split, more than 7 delta lights). */
-int vh_acc_kernel_seconds(const vh_accumulation* acc, double out[4]);
+ * shade, the deferred shadow rays, their resolve, and trace: depth d's closest rays and depth
+ * d - 1's shadow rays in one launch, which opaque scenes with deferred shadow rays use in place of
+ * extend and shadow (shadow and resolve are 0 when shade traces its shadow rays itself: the
+ * lighting split, more delta lights than records per path). */
+int vh_acc_kernel_seconds(const vh_accumulation* acc, double out[5]);
 uint64_t vh_acc_wavefront_bytes(const vh_accumulation* acc);
 uint64_t vh_acc_restir_bytes(const vh_accumulation* acc);
 /* Pixels the next sample would trace (readback::active_pixels, readback.rs:51-61). */

@@ -296,3 +296,32 @@ def test_many_delta_lights_shade_fused_and_match_the_oracle(ctx):
     r.accumulate(mega, 2)
     r.set_integrator(WAVEFRONT)
     assert np.array_equal(mega.film(), st["film"])
+
+
+def test_stage_timing_keeps_the_image_and_books_each_launch(ctx, zoo):
+    """Stage timing puts an event after every wavefront launch and changes nothing in the image.
+    An opaque scene traces depth d's closest rays with depth d - 1's shadow rays in one launch
+    (trace); a translucent one (zoo: glass) runs extend and shadow on their own."""
+    cases = ((Renderer(ctx, scenes.cornell()), True), (zoo[0], False))
+    for r, merged in cases:
+        r.set_integrator(WAVEFRONT)
+        films = []
+        for timing in (False, True):
+            r.set_stage_timing(timing)
+            acc = r.new_accumulation(64, 48)
+            r.accumulate(acc, 3)
+            films.append(acc.film())
+            if timing:
+                ext, shade, shadow, resolve, trace = acc.kernel_seconds()
+                assert shade > 0 and resolve > 0
+                if merged:
+                    assert trace > 0 and ext == 0 and shadow == 0
+                else:
+                    assert trace == 0 and ext > 0 and shadow > 0
+                tracing, shading = acc.stage_seconds()
+                assert tracing == pytest.approx(ext + trace) and shading == pytest.approx(shade + shadow + resolve)
+            else:
+                assert acc.kernel_seconds() is None
+        r.set_stage_timing(False)
+        r.set_integrator(MEGAKERNEL)
+        assert np.array_equal(films[0], films[1])

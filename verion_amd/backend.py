@@ -234,8 +234,9 @@ class Accumulation(_Owned):
         return None if rc != OK else (float(out[0]), float(out[1]))
 
     def kernel_seconds(self):
-        """(extend, shade, deferred shadow rays, their resolve) GPU seconds, or None."""
-        out = np.zeros(4, np.float64)
+        """(extend, shade, deferred shadow rays, their resolve, trace) GPU seconds, or None; trace
+        is closest and shadow rays in one launch (opaque scenes with deferred shadow rays)."""
+        out = np.zeros(5, np.float64)
         rc = library().vh_acc_kernel_seconds(self.h, _p(out, C.c_double))
         return None if rc != OK else tuple(float(x) for x in out)
 

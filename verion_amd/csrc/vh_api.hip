@@ -706,13 +706,8 @@ int vh_accumulate(vh_renderer* r, vh_accumulation* a, uint32_t samples)
     hipEvent_t e0, e1;
     HIPTRY(hipEventCreate(&e0));
     HIPTRY(hipEventCreate(&e1));
-    std::vector<hipEvent_t> stageEv;
+    StageMarks marks;
     bool timing = wave && r->stageTiming;
-    if (timing) {
-        stageEv.resize(kStageEvents * kMaxDepth * samples);
-        for (auto& e : stageEv)
-            HIPTRY(hipEventCreate(&e));
-    }
     int rc = VH_OK;
     HIPTRY(hipEventRecord(e0, s));
     if (wave) {
@@ -722,7 +717,7 @@ int vh_accumulate(vh_renderer* r, vh_accumulation* a, uint32_t samples)
             Params P = base;
             P.w[6] = a->samples + k;
             P.w[31] = (uint32_t)n;
-            rc = launch_wf_sample(S, F, P, W, restir, split, cfg, timing ? &stageEv[kStageEvents * kMaxDepth * k] : nullptr);
+            rc = launch_wf_sample(S, F, P, W, restir, split, cfg, timing ? &marks : nullptr);
         }
     } else if (!restir && !split && !a->adaptiveOn) {
         // the plain megakernel: the samples of a dispatch (up to kParallelPaths paths) in parallel
@@ -777,20 +772,20 @@ int vh_accumulate(vh_renderer* r, vh_accumulation* a, uint32_t samples)
         a->gpuSeconds += (double)ms * 1.0e-3;
         a->samples += samples;
         if (timing) {
-            for (uint32_t k = 0; k < samples; k++)
-                for (uint32_t d = 0; d < kMaxDepth; d++) {
-                    hipEvent_t* ev = &stageEv[kStageEvents * kMaxDepth * k + kStageEvents * d];
-                    for (uint32_t st = 0; st < 4; st++) {
-                        float ms = 0.0f;
-                        (void)hipEventElapsedTime(&ms, ev[st], ev[st + 1]);
-                        a->stage[st] += (double)ms * 1.0e-3;
-                    }
+            for (size_t k = 1; k < marks.ev.size(); k++) {
+                if (marks.label[k] >= kStages)
+                    continue;
+                float ms = 0.0f;
+                hipError_t te = hipEventElapsedTime(&ms, marks.ev[k - 1], marks.ev[k]);
+                if (te != hipSuccess) {
+                    rc = check_hip(te, "stage timing");
+                    break;
                 }
-            a->stageValid = true;
+                a->stage[marks.label[k]] += (double)ms * 1.0e-3;
+            }
+            a->stageValid = rc == VH_OK;
         }
     }
-    for (auto& e : stageEv)
-        (void)hipEventDestroy(e);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return rc;
@@ -805,18 +800,18 @@ int vh_acc_stage_seconds(const vh_accumulation* a, double out[2])
         return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
     if (!a->stageValid)
         return fail(VH_ERR_UNSUPPORTED, "stage timing was off");
-    out[0] = a->stage[0];
-    out[1] = a->stage[1] + a->stage[2] + a->stage[3];
+    out[0] = a->stage[kStageExtend] + a->stage[kStageTrace];
+    out[1] = a->stage[kStageShade] + a->stage[kStageShadow] + a->stage[kStageResolve];
     return VH_OK;
 }
 
-int vh_acc_kernel_seconds(const vh_accumulation* a, double out[4])
+int vh_acc_kernel_seconds(const vh_accumulation* a, double out[5])
 {
     if (!a || !out)
         return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
     if (!a->stageValid)
         return fail(VH_ERR_UNSUPPORTED, "stage timing was off");
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < kStages; k++)
         out[k] = a->stage[k];
     return VH_OK;
 }

@@ -132,7 +132,32 @@ struct LaunchCfg {
 constexpr uint32_t kMaxDepth = 8;           // pack.rs:13
 constexpr uint32_t kSamplesPerDispatch = 8; // vulkan renderer.rs:29
 constexpr uint32_t kCounterBanks = 64;      // banks of the traversal counters (vh_trace.h)
-constexpr uint32_t kStageEvents = 5;        // per depth: start, after extend, shade, shadow rays, resolve
+
+// Stage timing of the wavefront launches: an event after each launch, labelled with the stage the
+// time since the previous event belongs to. kStageTrace: closest and shadow rays in one launch.
+enum Stage : uint8_t { kStageExtend, kStageShade, kStageShadow, kStageResolve, kStageTrace, kStages };
+struct StageMarks {
+    std::vector<hipEvent_t> ev;
+    std::vector<uint8_t> label; // label[k]: stage of ev[k-1] -> ev[k]; kStages: no interval ends at ev[k]
+    StageMarks() = default;
+    StageMarks(const StageMarks&) = delete;
+    StageMarks& operator=(const StageMarks&) = delete;
+    hipError_t mark(hipStream_t s, uint8_t stage)
+    {
+        hipEvent_t e;
+        hipError_t err = hipEventCreate(&e);
+        if (err != hipSuccess)
+            return err;
+        ev.push_back(e);
+        label.push_back(stage);
+        return hipEventRecord(e, s);
+    }
+    ~StageMarks()
+    {
+        for (hipEvent_t e : ev)
+            (void)hipEventDestroy(e);
+    }
+};
 
 // blocks of `block` threads for n threads of work, at least 1 and at most `cap`
 inline uint32_t blocks_for(uint64_t n, uint32_t block, uint32_t cap)
@@ -155,7 +180,7 @@ int launch_megakernel_samples(const Scene& S, const Frame& F, const Params& P, u
 // Paths in flight that fill the GPU: smaller images take several samples per dispatch in parallel.
 constexpr uint64_t kParallelPaths = 1u << 21;
 int launch_wf_sample(const Scene& S, const Frame& F, const Params& P, const Wavefront& W, bool restir, bool split, const LaunchCfg& cfg,
-                     hipEvent_t* stageEvents);
+                     StageMarks* marks);
 int launch_primary_hits(const Scene& S, const Params& P, uint4* out, hipStream_t s);
 int launch_nrc_train_paths(const Scene& S, const Params& P, const Wavefront& W, uint32_t* trainPix, uint32_t nTrain, uint32_t npix,
                            uint32_t frame, float4* vertices, float4* samples, uint32_t* sampleCount, uint32_t sampleCap,
@@ -247,7 +272,7 @@ struct vh_accumulation {
     uint32_t shK = 0;
     uint32_t samples = 0;
     double gpuSeconds = 0.0;
-    double stage[4] = {0.0, 0.0, 0.0, 0.0}; // extend, shade, deferred shadow rays, resolve
+    double stage[vh::kStages] = {}; // GPU seconds per vh::Stage
     bool stageValid = false;
     bool adaptiveOn = false;
     vh_adaptive adaptive{};

@@ -500,6 +500,59 @@ __global__ __launch_bounds__(kTraceBlock, VH_EXTEND_WAVES) void k_wf_shadow(Scen
     }
 }
 
+// k_wf_extend for depth d and the opaque scene's k_wf_shadow for depth d - 1 in one launch: waves
+// [0, wc) take closest rays, the rest shadow rays (kind wave-uniform). Both queues are final once
+// depth d - 1 is shaded, and neither reads what the other writes; one launch's tail instead of
+// two, which at the deep depths is most of their time.
+__global__ __launch_bounds__(kTraceBlock, VH_EXTEND_WAVES) void k_wf_trace(Scene S, Params P, Wavefront W, uint32_t d)
+{
+    static_assert(kTraceBlock == 64, "one wave per block");
+    const uint32_t nc = W.counts[d];
+    const uint32_t ns = d > 0 ? W.counts[kShadowCounts + d - 1] : 0u;
+    const uint32_t per = raysPerWave(nc + ns, W.traceWaves);
+    const uint32_t wc = (nc + per - 1) / per, waves = wc + (ns + per - 1) / per;
+    const uint32_t* st = W.state[d & 1u];
+    const size_t stride = (size_t)W.shK * W.n;
+    __shared__ WaveTris sh;
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t w = blockIdx.x; w < waves; w += gridDim.x) {
+        if (w < wc) {
+            const uint32_t i = w * per + lane;
+            const bool act = lane < per && i < nc;
+            V3 org = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
+            if (act && d == 0) {
+                uint32_t p = P.w[32] != 0 ? W.queue0[i] : tilePixel(i, P.w[0], P.w[1], W.tileW, W.tileH);
+                PathState s;
+                generatePath(P, p % P.w[0], p / P.w[0], P.w[6], s);
+                org = s.org;
+                dir = s.dir;
+            } else if (act) {
+                org = get3(st, W.n, 0, i);
+                dir = get3(st, W.n, 3, i);
+            }
+            Hit h = traceWave<false>(S.bvh, org, dir, 1.0e30f, act, sh);
+            if (act)
+                W.hits[i] = make_uint4(h.hit ? 1u : 0u, h.inst, h.prim, asu(h.t));
+        } else {
+            const uint32_t q = (w - wc) * per + lane;
+            const bool act = lane < per && q < ns;
+            V3 o = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
+            float tmax = 1.0e30f;
+            uint32_t rid = 0;
+            if (act) {
+                rid = W.shQueue[q];
+                const float* r = W.shRec + rid;
+                o = v3(r[0], r[stride], r[2 * stride]);
+                dir = v3(r[3 * stride], r[4 * stride], r[5 * stride]);
+                tmax = r[6 * stride];
+            }
+            Hit h = traceWave<true>(S.bvh, o, dir, tmax, act, sh);
+            if (act)
+                W.shVis[rid] = h.hit ? 1u : 0u;
+        }
+    }
+}
+
 // Adds the visible shadow contributions of every pending path at depth d in record order, then
 // stores the total in the path's next slot or finishes the path.
 template <bool GI>
@@ -877,7 +930,7 @@ int launch_megakernel_samples(const Scene& S, const Frame& F, const Params& P, u
 }
 
 int launch_wf_sample(const Scene& S, const Frame& F, const Params& P0, const Wavefront& W, bool restir, bool split, const LaunchCfg& cfg,
-                     hipEvent_t* ev)
+                     StageMarks* marks)
 {
     hipStream_t s = cfg.stream;
     uint32_t grid = cfg.wfGrid ? cfg.wfGrid : 1024;
@@ -886,50 +939,72 @@ int launch_wf_sample(const Scene& S, const Frame& F, const Params& P0, const Wav
     uint32_t maxDepth = P.w[7];
     // ReSTIR DI defers its winner's shadow ray with the others; GI's reuse runs first, in k_gi_prepare
     const bool gi = restir && (P.w[36] & 2u) != 0;
+    const bool defer = W.shK > 0 && !split;
+    hipError_t me = hipSuccess;
+    auto mark = [&](uint8_t stage) {
+        if (marks && me == hipSuccess)
+            me = marks->mark(s, stage);
+    };
+    auto shade = [&](uint32_t d) {
+        if (gi && d == 0)
+            k_gi_prepare<<<grid * kTraceScale, kTraceBlock, 0, s>>>(S, F, P, W);
+        if (gi)
+            k_wf_shade_deferred<true, true><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+        else if (restir && d == 0)
+            k_wf_shade_deferred<false, true><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+        else
+            k_wf_shade_deferred<false, false><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+        mark(kStageShade);
+    };
+    auto resolve = [&](uint32_t d) {
+        if (gi)
+            k_wf_resolve<true><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+        else
+            k_wf_resolve<false><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+        mark(kStageResolve);
+    };
     k_wf_reset<<<1, 64, 0, s>>>(W.counts, W.n, P.w[32], maxDepth);
     if (P.w[32] != 0)
         k_wf_select<<<blocks_for(W.n, kWfBlock, grid), kWfBlock, 0, s>>>(F, P, W);
-    for (uint32_t d = 0; d < maxDepth; d++) {
-        P.w[30] = d;
-        if (ev)
-            (void)hipEventRecord(ev[kStageEvents * d], s);
-        k_wf_extend<<<tgrid, kTraceBlock, 0, s>>>(S, P, W, d);
-        if (ev)
-            (void)hipEventRecord(ev[kStageEvents * d + 1], s);
-        if (W.shK > 0 && !split) {
-            if (gi && d == 0)
-                k_gi_prepare<<<grid * kTraceScale, kTraceBlock, 0, s>>>(S, F, P, W);
-            if (gi)
-                k_wf_shade_deferred<true, true><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
-            else if (restir && d == 0)
-                k_wf_shade_deferred<false, true><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
-            else
-                k_wf_shade_deferred<false, false><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
-            if (ev)
-                (void)hipEventRecord(ev[kStageEvents * d + 2], s);
-            k_wf_shadow<<<tgrid, kTraceBlock, 0, s>>>(S, P, W, d);
-            if (ev)
-                (void)hipEventRecord(ev[kStageEvents * d + 3], s);
-            if (gi)
-                k_wf_resolve<true><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
-            else
-                k_wf_resolve<false><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
-            if (ev)
-                (void)hipEventRecord(ev[kStageEvents * d + 4], s);
-            continue;
+    mark(kStages);
+    if (defer && !S.translucent) {
+        // opaque scenes: depth d's closest rays and depth d - 1's shadow rays in one launch, then
+        // depth d - 1's resolve, then depth d's shading; a last launch for depth maxDepth - 1's
+        // shadow rays (counts[maxDepth] stays 0)
+        for (uint32_t d = 0; d <= maxDepth; d++) {
+            P.w[30] = d;
+            k_wf_trace<<<tgrid, kTraceBlock, 0, s>>>(S, P, W, d);
+            mark(kStageTrace);
+            if (d > 0)
+                resolve(d - 1);
+            if (d < maxDepth)
+                shade(d);
         }
-        if (restir && split)
-            k_wf_shade<true, true><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
-        else if (restir)
-            k_wf_shade<true, false><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
-        else if (split)
-            k_wf_shade<false, true><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
-        else
-            k_wf_shade<false, false><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
-        if (ev)
-            for (uint32_t k = 2; k < kStageEvents; k++)
-                (void)hipEventRecord(ev[kStageEvents * d + k], s);
+    } else {
+        for (uint32_t d = 0; d < maxDepth; d++) {
+            P.w[30] = d;
+            k_wf_extend<<<tgrid, kTraceBlock, 0, s>>>(S, P, W, d);
+            mark(kStageExtend);
+            if (defer) { // translucent scenes: closest-hit shadow rays through the layers, per lane
+                shade(d);
+                k_wf_shadow<<<tgrid, kTraceBlock, 0, s>>>(S, P, W, d);
+                mark(kStageShadow);
+                resolve(d);
+            } else {
+                if (restir && split)
+                    k_wf_shade<true, true><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+                else if (restir)
+                    k_wf_shade<true, false><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+                else if (split)
+                    k_wf_shade<false, true><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+                else
+                    k_wf_shade<false, false><<<grid, kWfBlock, 0, s>>>(S, F, P, W, d);
+                mark(kStageShade);
+            }
+        }
     }
+    if (me != hipSuccess)
+        return fail(VH_ERR_KERNEL, std::string("stage timing: ") + hipGetErrorString(me));
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("wavefront launch: ") + hipGetErrorString(e));
 }
