@@ -186,7 +186,14 @@ int launch_nrc_train_paths(const Scene& S, const Params& P, const Wavefront& W, 
                            uint32_t frame, float4* vertices, float4* samples, uint32_t* sampleCount, uint32_t sampleCap,
                            const LaunchCfg& cfg);
 int launch_wf_reset(const Params& P, const Wavefront& W, hipStream_t s);
-int launch_wf_depth(const Scene& S, const Frame& F, const Params& P, const Wavefront& W, uint32_t d, bool shade, const LaunchCfg& cfg);
+// One depth of a pass driven depth by depth: extend, then (shade) the shading with its shadow
+// rays and resolve; leaveShadows: the deferred shadow rays and resolve are left to
+// launch_wf_trace_resolve of depth d + 1.
+int launch_wf_depth(const Scene& S, const Frame& F, const Params& P, const Wavefront& W, uint32_t d, bool shade, const LaunchCfg& cfg,
+                    bool leaveShadows = false);
+// Depth d's closest rays with depth d - 1's deferred shadow rays in one launch, then depth d - 1's
+// resolve (d >= 1, opaque scene, depth d - 1 shaded with leaveShadows).
+int launch_wf_trace_resolve(const Scene& S, const Frame& F, const Params& P, const Wavefront& W, uint32_t d, const LaunchCfg& cfg);
 int launch_nrc_query_shade(const Scene& S, const Frame& F, const Params& P, const Wavefront& W, float4* queries, uint32_t d,
                            const LaunchCfg& cfg);
 int launch_trace_rays(const Scene& S, const float* rays, uint64_t n, float tmax, uint4* out, hipStream_t s);

@@ -856,11 +856,16 @@ int vh_nrc_accumulate(vh_nrc* nn, vh_accumulation* a, uint32_t frames)
         P.w[6] = a->samples + f;
         P.w[31] = (uint32_t)npix;
         TRY(launch_wf_reset(P, W, s));
+        // opaque scenes: the cache depth's closest rays go with the previous depth's shadow rays
+        const bool merge = termDepth >= 1 && termDepth < maxDepth && W.shK > 0 && !S.translucent;
         for (uint32_t d = 0; d < std::min(termDepth, maxDepth); d++)
-            TRY(launch_wf_depth(S, F, P, W, d, true, cfg));
+            TRY(launch_wf_depth(S, F, P, W, d, true, cfg, merge && d + 1 == termDepth));
         if (termDepth >= maxDepth)
             return VH_OK;
-        TRY(launch_wf_depth(S, F, P, W, termDepth, false, cfg));
+        if (merge)
+            TRY(launch_wf_trace_resolve(S, F, P, W, termDepth, cfg));
+        else
+            TRY(launch_wf_depth(S, F, P, W, termDepth, false, cfg));
         TRY(launch_nrc_query_shade(S, F, P, W, nn->queries.as<float4>(), termDepth, cfg));
         uint32_t grid = std::min<uint32_t>((uint32_t)((npix + kRows - 1) / kRows), r->ctx->cus * 2);
         k_nrc_infer<1><<<grid, 256, 0, s>>>(net, nn->queries.as<float4>(), W.counts + termDepth, 0u, nullptr, F.film);

@@ -1038,7 +1038,8 @@ int launch_wf_reset(const Params& P, const Wavefront& W, hipStream_t s)
     return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("wavefront reset: ") + hipGetErrorString(e));
 }
 
-int launch_wf_depth(const Scene& S, const Frame& F, const Params& P0, const Wavefront& W, uint32_t d, bool shade, const LaunchCfg& cfg)
+int launch_wf_depth(const Scene& S, const Frame& F, const Params& P0, const Wavefront& W, uint32_t d, bool shade, const LaunchCfg& cfg,
+                    bool leaveShadows)
 {
     Params P = P0;
     P.w[30] = d;
@@ -1047,13 +1048,29 @@ int launch_wf_depth(const Scene& S, const Frame& F, const Params& P0, const Wave
     k_wf_extend<<<tgrid, kTraceBlock, 0, cfg.stream>>>(S, P, W, d);
     if (shade && W.shK > 0) {
         k_wf_shade_deferred<false, false><<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
-        k_wf_shadow<<<tgrid, kTraceBlock, 0, cfg.stream>>>(S, P, W, d);
-        k_wf_resolve<false><<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
+        if (!leaveShadows) {
+            k_wf_shadow<<<tgrid, kTraceBlock, 0, cfg.stream>>>(S, P, W, d);
+            k_wf_resolve<false><<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
+        }
     } else if (shade) {
         k_wf_shade<false, false><<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("wavefront depth: ") + hipGetErrorString(e));
+}
+
+int launch_wf_trace_resolve(const Scene& S, const Frame& F, const Params& P0, const Wavefront& W, uint32_t d, const LaunchCfg& cfg)
+{
+    if (d == 0 || W.shK == 0 || S.translucent)
+        return fail(VH_ERR_INVALID_ARGUMENT, "merged trace: needs depth >= 1 of an opaque scene with deferred shadow rays");
+    Params P = P0;
+    P.w[30] = d;
+    uint32_t grid = cfg.wfGrid ? cfg.wfGrid : 1024;
+    const uint32_t tgrid = std::max(grid * kTraceScale, W.traceWaves); // one-wave blocks
+    k_wf_trace<<<tgrid, kTraceBlock, 0, cfg.stream>>>(S, P, W, d);
+    k_wf_resolve<false><<<grid, kWfBlock, 0, cfg.stream>>>(S, F, P, W, d - 1);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("wavefront trace: ") + hipGetErrorString(e));
 }
 
 int launch_nrc_query_shade(const Scene& S, const Frame& F, const Params& P0, const Wavefront& W, float4* queries, uint32_t d,
