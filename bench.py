@@ -11,9 +11,10 @@ Prints one JSON line with the throughput (`value`, plain wavefront), a roofline 
 dominant kernel (achieved algorithmic bytes per launch over its HIP-event launch time, against
 8 TB/s), the CPU oracle's throughput on a bounded sample of the same workload, and `time_to_rmse`:
 GPU seconds to relMSE 1e-3 of the neural radiance cache and of the plain tracer on the same scene
-at 1080p, 1 spp per frame, against a plain reference render (16,384 spp, the workload's, by default); and `batch`:
+at 1080p, 1 spp per frame, against a plain reference render (16,384 spp, the workload's, by default); `batch`:
 p50/p99 job latency and aggregate Msamples/s of 32 uneven jobs in one shared wavefront against
-one job at a time.
+one job at a time; and `restir_many_lights`: ms per frame and relMSE of plain, ReSTIR DI and DI + GI
+on the many-lights workload at 1080p.
 
     python bench.py [--gpus 1] [--steps 64] [--warmup 4]
 """
@@ -80,13 +81,23 @@ def algorithmic_bytes(c: dict, pixels: int, delta_lights: int, env: bool):
     return {"wf_extend": extend, "wf_shade": shade, "wf_shadow": shadow_b, "wf_trace": extend + shadow_rays}
 
 
+def host_cores():
+    """The host cores this process may use: its CPU affinity, at most 16 (the GPU box gives one GPU a
+    share of 16 cores, whatever the machine's count)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def cpu_baseline(packed, width, height, gpu_film):
     """The oracle (a scalar C port, OpenMP over pixels) on the host cores: one sample of every pixel
     of the same workload, plus a full-size parity check of that sample against the GPU's."""
     import oracle
     from verion_amd import pack
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_cores()
     t0 = time.perf_counter()
     o = oracle.OracleScene(packed)
     build_s = time.perf_counter() - t0
@@ -99,8 +110,12 @@ def cpu_baseline(packed, width, height, gpu_film):
     parity = {"max_rel": float(rel.max()), "exact_fraction": float(np.mean(gpu_film == film)), "pixels": width * height,
               "sample": 0}
     base = {"value": round(width * height / secs / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "label": "project CPU oracle (not the reference)",
             "sample": f"{width}x{height}, 1 spp (sample 0) of the same scene on the CPU oracle; "
-                      f"{secs:.1f} s render, BVH build {build_s:.1f} s excluded"}
+                      f"{secs:.1f} s render, BVH build {build_s:.1f} s excluded",
+            "cores_note": "every core of this process's CPU affinity, capped at the 16-core share of one GPU",
+            "reference_baseline": "not measured: the reference's Vulkan backend on Mesa lavapipe needs rustc, slangc "
+                                  "and a Vulkan ICD with ray queries, none of which is in the image"}
     return base, parity
 
 
@@ -115,6 +130,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-nrc", action="store_true", help="skip the NRC vs plain time-to-relMSE measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the 32-job batch measurement")
+    ap.add_argument("--no-restir", action="store_true", help="skip the many-lights ReSTIR measurement")
     ap.add_argument("--no-cornell", action="store_true", help="skip the Cornell box measurement (for profiler runs)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (8 blocks per CU, 8x8 tiles)")
     ap.add_argument("--shape", type=str, default=None, help="fixed launch shape BLOCKS,TILE_W,TILE_H instead of tuning")
@@ -160,18 +176,28 @@ def main():
         log(f"autotuned wavefront: {tuning}")
 
     # timed region: K sample passes from sample 0; accumulate() returns after the GPU finished, so
-    # the clock brackets device-complete work on both sides. Stage timing records HIP events around
-    # every extend and shade launch on the library's stream.
-    r.set_stage_timing(True)
+    # the clock brackets device-complete work on both sides. No events between the launches.
+    r.set_stage_timing(False)
     acc = r.new_accumulation(W, H)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r.accumulate(acc, 1)
     elapsed = time.perf_counter() - t0
-    stage = acc.stage_seconds()
-    kernel_s = acc.kernel_seconds()
     value = args.steps * pixels / elapsed / 1e6
-    log(f"timed {args.steps} steps in {elapsed:.3f} s; extend, shade, shadow, resolve, trace seconds {kernel_s}")
+    acc.close()
+    # a second timed region of K passes with stage timing: HIP events recorded on the library's
+    # stream after every launch (events created before the passes) give each kernel's launch time
+    r.set_stage_timing(True)
+    acc = r.new_accumulation(W, H)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.accumulate(acc, 1)
+    elapsed_staged = time.perf_counter() - t0
+    kernel_s = acc.kernel_seconds()
+    acc.close()
+    r.set_stage_timing(False)
+    log(f"timed {args.steps} steps in {elapsed:.3f} s ({elapsed_staged:.3f} s with stage events); "
+        f"extend, shade, shadow, resolve, trace seconds {kernel_s}")
 
     # traversal counters of one more pass (outside the timed region) for the algorithmic bytes
     if args.no_counting:
@@ -197,15 +223,28 @@ def main():
     dom_s = per[dom] / (args.steps * per_pass[dom])
     dom_b = algo[dom] / per_pass[dom]
     achieved = dom_b / dom_s / 1e9
+    traffic = traffic_from_profiles(dom)
+    measured = None
+    if traffic:
+        gbs = traffic / dom_s / 1e9
+        measured = {"gb_per_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                    "source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch (copy-calibrated, committed profile) "
+                              "over this run's launch time"}
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "achieved_is": "algorithmic bytes (every node and triangle fetch counted) over launch time",
+                "hbm_measured": measured,
+                "limiter": "instruction issue and dependent-fetch latency of the traversal: node and triangle "
+                           "fetches hit L2 and the Infinity Cache, so measured HBM traffic stays far below peak",
                 "launch_ms": round(dom_s * 1e3, 4), "algorithmic_bytes_per_launch": int(dom_b),
                 "ms_per_launch": {k: round(per[k] / (args.steps * per_pass[k]) * 1e3, 4) for k in ran},
                 "algorithmic_gb_per_s": {k: round(algo[k] / (per[k] / args.steps) / 1e9, 2) for k in algo if k in ran}}
 
     line = {"metric": "Msamples/s at 1080p + time-to-rMSE 1e-3: NRC vs plain wavefront, 1 MI355X", "value": round(value, 3),
             "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+            "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+            "value_with_stage_events": round(args.steps * pixels / elapsed_staged / 1e6, 3),
+            "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "procedural 1,024 icospheres (1,310,720 tris), sun + constant env map",
                        "width": W, "height": H, "spp_per_step": 1, "max_depth": 8, "integrator": args.integrator,
@@ -224,6 +263,8 @@ def main():
         line["cornell_256"] = cornell_256(ctx)
     if not args.no_nrc:
         line["time_to_rmse"] = time_to_rmse(W, H, args.nrc_ref_spp)
+    if not args.no_restir:
+        line["restir_many_lights"] = restir_many_lights(W, H)
     if not args.no_batch:
         line["batch"] = batch_jobs()
     print(json.dumps(line), flush=True)
@@ -284,6 +325,25 @@ def batch_jobs():
     return {"jobs": res["jobs"], "pixels_total": res["pixels_total"], "spp_range": res["spp_range"],
             "adaptive": res["adaptive"], "batched": {k: res["batched"][k] for k in keep + ("tuning",)},
             "one_at_a_time": {k: res["one_at_a_time"][k] for k in keep}, "films_checked_equal": res["films_checked_equal"]}
+
+
+def restir_many_lights(width, height):
+    """The many-lights workload (502,252 triangles, 10,000 emitters, 1 spp per frame over 64 frames,
+    32 candidates, temporal + 5-neighbour spatial reuse): ms per frame and relMSE at 64 frames of
+    plain, ReSTIR DI and DI + GI against a 1,024-spp plain reference, and plain's time to each
+    method's error over the method's time."""
+    from verion_amd import restir_eval
+
+    res = restir_eval.evaluate(width, height, frames=64, ref_spp=1024, candidates=32, history=4, neighbours=5,
+                               plain_frames=256, methods=("restir_di", "restir_di_gi"), log=log)
+    out = {"scene": res["scene"], "frames": res["frames"], "reference_spp": res["reference_spp"],
+           "candidates": res["candidates"], "neighbours": res["neighbours"], "relmse": res["relmse"]}
+    for k in ("plain", "restir_di", "restir_di_gi"):
+        m = res[k]
+        out[k] = {"ms_per_frame": round(m["ms_per_frame"], 3), "relmse_64": m["relmse"]}
+        if k != "plain":
+            out[k]["plain_time_over_method_time"] = m["speedup_to_equal_error"]
+    return out
 
 
 def time_to_rmse(width, height, ref_spp):

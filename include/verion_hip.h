@@ -287,9 +287,16 @@ typedef struct vh_nrc_config {
 } vh_nrc_config;
 typedef struct vh_nrc vh_nrc;
 
-/* NULL config = {65536, 8, 1e-2, 1, 0.95, 16, 0}. The cache starts with zero parameters: set them first. */
+/* NULL config = {65536, 8, 1e-2, 1, 0.95, 16, 0}. The cache starts from vh_nrc_init_params(0). */
 int vh_nrc_create(vh_renderer* r, const vh_nrc_config* config, vh_nrc** out);
 void vh_nrc_destroy(vh_nrc* nrc);
+/* Host-only (no device work): the cache's seeded initial parameters (n = 23888). He-uniform weights
+ * (uniform in +-sqrt(6 / fan_in)), zero biases, the output layer scaled by 0.1 with its 13 unused
+ * outputs zero; weight p draws u = (pcg(pcg(seed ^ 0xA511E9B3) + p) >> 8) * 2^-24 with the pcg of
+ * pt_core.slang:69-74 and takes lim * (2u - 1). */
+int vh_nrc_init_params(uint32_t seed, float* params, uint64_t n);
+/* vh_nrc_set_params with vh_nrc_init_params(seed): a fresh cache (Adam and the average restart). */
+int vh_nrc_init(vh_nrc* nrc, uint32_t seed);
 /* Replaces the FP32 master parameters (n = 23888), their moving average, and resets Adam. */
 int vh_nrc_set_params(vh_nrc* nrc, const float* params, uint64_t n);
 int vh_nrc_get_params(vh_nrc* nrc, float* params, uint64_t n);

@@ -190,3 +190,41 @@ def test_training_paths_defer_their_shadow_rays_without_changing_the_samples(ctx
         stats.append(c.stats())
     assert stats[0]["samples"] == stats[1]["samples"] > 0, stats
     assert abs(stats[0]["loss"] - stats[1]["loss"]) <= 1e-4 * abs(stats[1]["loss"]), stats
+
+
+def test_a_fresh_cache_starts_from_the_library_initialisation_and_trains(ctx):
+    # the library stands alone: vh_nrc_create starts from vh_nrc_init_params(0), vh_nrc_init(seed)
+    # restarts from another seed, and Adam lowers the loss from there (a ReLU MLP at zero would not)
+    r = Renderer(ctx, scenes.cornell())
+    c = Nrc(r, train_paths=1024)
+    np.testing.assert_array_equal(c.params(), Nrc.init_params(0))
+    c.init(5)
+    np.testing.assert_array_equal(c.params(), Nrc.init_params(5))
+    raw, target = batch(8192, 9)
+    first = c.train(raw, target, steps=1)
+    last = c.train(raw, target, steps=300)
+    assert last < 0.2 * first, (first, last)
+
+
+def test_merged_trace_launches_render_the_nrc_frames_of_the_fused_path(ctx):
+    # Opaque scenes with at most 7 delta lights trace a depth's closest rays together with the
+    # previous depth's shadow rays (one launch) and resolve afterwards; 8 more point lights of zero
+    # intensity force the fused shading and separate launches. From the same parameters both draw
+    # the same film up to the order the training samples arrive in (mean relative difference, as
+    # the minibatch test)
+    from verion_amd import scene as sc
+    base = scenes.cornell()
+    fused = scenes.cornell()
+    for k in range(8):
+        fused.add_light(sc.PointLight(position=(0.1 * k, 0.5, 0.5), color=(1.0, 1.0, 1.0), intensity=0.0))
+    films = []
+    for s in (base, fused):
+        r = Renderer(ctx, s)
+        r.set_integrator(WAVEFRONT)
+        c = Nrc(r, train_paths=4096, train_steps=1, warmup_frames=2)
+        c.init(10)
+        acc = r.new_accumulation(64, 64)
+        c.accumulate(acc, 4)
+        films.append(acc.film())
+    diff = np.abs(films[0] - films[1]) / np.maximum(np.abs(films[1]), 1e-3)
+    assert np.all(np.isfinite(films[0])) and diff.mean() < 1e-3, diff.mean()

@@ -118,6 +118,8 @@ def library() -> C.CDLL:
         "vh_read_counters": (i32, [vp, C.POINTER(C.c_uint64)]),
         "vh_nrc_create": (i32, [vp, C.POINTER(abi.NrcConfig), C.POINTER(vp)]),
         "vh_nrc_destroy": (None, [vp]),
+        "vh_nrc_init_params": (i32, [u32, f32p, u64]),
+        "vh_nrc_init": (i32, [vp, u32]),
         "vh_nrc_set_params": (i32, [vp, f32p, u64]),
         "vh_nrc_get_params": (i32, [vp, f32p, u64]),
         "vh_nrc_infer": (i32, [vp, u64, f32p, f32p]),
@@ -406,6 +408,17 @@ class Nrc(_Owned):
         h = C.c_void_p()
         _check(library().vh_nrc_create(renderer.h, C.byref(cfg), C.byref(h)))
         self._handle = _Handle(h, library().vh_nrc_destroy, renderer._handle)
+
+    @staticmethod
+    def init_params(seed: int = 0) -> np.ndarray:
+        """The library's seeded initial parameters (vh_nrc_init_params; host only, no device)."""
+        out = np.zeros(Nrc.PARAMS, np.float32)
+        _check(library().vh_nrc_init_params(seed, _p(out, C.c_float), out.size))
+        return out
+
+    def init(self, seed: int = 0):
+        """Restarts the cache from the library's seeded initialisation (vh_nrc_init)."""
+        _check(library().vh_nrc_init(self.h, seed))
 
     def set_params(self, theta: np.ndarray):
         theta = np.ascontiguousarray(theta, np.float32)

@@ -155,6 +155,12 @@ int validate(const vh_scene_tables* t, std::vector<uint32_t>& instStart, uint64_
     uint64_t need_lights = (uint64_t)t->info.light_count * 16 + kAlbedoWords + 2 * t->instance_count;
     if (t->lights_len < need_lights)
         return fail(VH_ERR_SCENE, "light table shorter than lights + albedo tables + link masks");
+    // IES profiles (word 15 = offset + 1, pack.rs:679-686): the frame and a 361 x 181 table past the masks
+    for (uint64_t k = 0; k < t->info.light_count; k++) {
+        uint64_t w15 = t->lights[16 * k + 15];
+        if (w15 != 0 && (w15 - 1 < need_lights || w15 - 1 + 9 + 361ull * 181 > t->lights_len))
+            return fail(VH_ERR_SCENE, "IES profile out of the light table");
+    }
     if (t->emitters_len != 16ull * t->info.emitter_count || t->emit_cdf_len != t->info.emitter_count)
         return fail(VH_ERR_SCENE, "emitter table does not match the emitter count");
     uint64_t ew = t->info.env_width, eh = t->info.env_height;
@@ -703,11 +709,15 @@ int vh_accumulate(vh_renderer* r, vh_accumulation* a, uint32_t samples)
     bool restir = a->restirOn;
     bool split = a->lighting;
 
-    hipEvent_t e0, e1;
-    HIPTRY(hipEventCreate(&e0));
-    HIPTRY(hipEventCreate(&e1));
-    StageMarks marks;
+    for (hipEvent_t& e : a->span)
+        if (!e)
+            HIPTRY(hipEventCreate(&e));
+    hipEvent_t e0 = a->span[0], e1 = a->span[1];
+    StageMarks& marks = a->marks;
+    marks.reset();
     bool timing = wave && r->stageTiming;
+    if (timing) // the marks of one pass (about 3 per depth), created before the first launch
+        HIPTRY(marks.reserve((size_t)samples * (4 * kMaxDepth + 4)));
     int rc = VH_OK;
     HIPTRY(hipEventRecord(e0, s));
     if (wave) {
@@ -772,7 +782,7 @@ int vh_accumulate(vh_renderer* r, vh_accumulation* a, uint32_t samples)
         a->gpuSeconds += (double)ms * 1.0e-3;
         a->samples += samples;
         if (timing) {
-            for (size_t k = 1; k < marks.ev.size(); k++) {
+            for (size_t k = 1; k < marks.used; k++) {
                 if (marks.label[k] >= kStages)
                     continue;
                 float ms = 0.0f;
@@ -786,8 +796,6 @@ int vh_accumulate(vh_renderer* r, vh_accumulation* a, uint32_t samples)
             a->stageValid = rc == VH_OK;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return rc;
 }
 

@@ -136,21 +136,41 @@ constexpr uint32_t kCounterBanks = 64;      // banks of the traversal counters (
 // Stage timing of the wavefront launches: an event after each launch, labelled with the stage the
 // time since the previous event belongs to. kStageTrace: closest and shadow rays in one launch.
 enum Stage : uint8_t { kStageExtend, kStageShade, kStageShadow, kStageResolve, kStageTrace, kStages };
+// A pool of timing events reused from call to call: a call's first marks create events, later
+// calls only record them, so no event is created between the launches of a timed pass.
 struct StageMarks {
     std::vector<hipEvent_t> ev;
     std::vector<uint8_t> label; // label[k]: stage of ev[k-1] -> ev[k]; kStages: no interval ends at ev[k]
+    size_t used = 0;            // events recorded since reset()
     StageMarks() = default;
     StageMarks(const StageMarks&) = delete;
     StageMarks& operator=(const StageMarks&) = delete;
+    void reset()
+    {
+        used = 0;
+        label.clear();
+    }
+    // Events a call will record, created ahead of its launches.
+    hipError_t reserve(size_t n)
+    {
+        while (ev.size() < n) {
+            hipEvent_t e;
+            hipError_t err = hipEventCreate(&e);
+            if (err != hipSuccess)
+                return err;
+            ev.push_back(e);
+        }
+        return hipSuccess;
+    }
     hipError_t mark(hipStream_t s, uint8_t stage)
     {
-        hipEvent_t e;
-        hipError_t err = hipEventCreate(&e);
-        if (err != hipSuccess)
-            return err;
-        ev.push_back(e);
+        if (used == ev.size()) {
+            hipError_t err = reserve(used + 1);
+            if (err != hipSuccess)
+                return err;
+        }
         label.push_back(stage);
-        return hipEventRecord(e, s);
+        return hipEventRecord(ev[used++], s);
     }
     ~StageMarks()
     {
@@ -276,6 +296,8 @@ struct vh_accumulation {
     vh::DevBuf shRec, shVis, shQueue, shNRec, shOut, shPixel, shTotal; // deferred shadow rays
     vh::DevBuf staged; // the megakernel's per-sample radiance when samples run in parallel
     vh::DevBuf persistRec; // the persistent megakernel's shadow records (per resident lane)
+    vh::StageMarks marks;  // stage-timing events, reused across calls
+    hipEvent_t span[2] = {nullptr, nullptr}; // start and end of a call
     uint32_t shK = 0;
     uint32_t samples = 0;
     double gpuSeconds = 0.0;
@@ -288,6 +310,12 @@ struct vh_accumulation {
     bool wfReady = false;
     uint64_t wfBytes = 0;
     uint64_t pixels() const { return (uint64_t)width * height; }
+    ~vh_accumulation()
+    {
+        for (hipEvent_t e : span)
+            if (e)
+                (void)hipEventDestroy(e);
+    }
     // shadow records a deferred path may make: one per delta light, one emitter or environment
     // sample, and ReSTIR DI's winner
     uint32_t shadowRecords() const { return r->info.light_count + 1 + (restirOn && restir.di ? 1u : 0u); }

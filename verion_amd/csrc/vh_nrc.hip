@@ -612,6 +612,29 @@ int train_steps(vh_nrc* nn, const float4* samples, const uint32_t* countPtr, uin
     return VH_OK;
 }
 
+// He-uniform initialisation (He et al., ICCV 2015: weights uniform in +-sqrt(6 / fan_in)) with
+// zero biases; the output layer is scaled by 0.1 and its 13 unused outputs start at zero. Weight
+// p (its index in the flat parameter vector) takes u = (pcg(pcg(seed ^ 0xA511E9B3) + p) >> 8) 2^-24,
+// the stream of Rng::next (pt_core.slang:79-83) keyed by the parameter, and w = lim (2u - 1).
+void init_params(uint32_t seed, float* theta)
+{
+    const uint32_t key = pcg(seed ^ 0xA511E9B3u);
+    for (int p = 0; p < kParams; p++)
+        theta[p] = 0.0f;
+    for (int l = 0; l < kNL; l++) {
+        const int K = layerK(l), N = layerN(l);
+        const bool last = l == kNL - 1;
+        const float lim = sqrtf(6.0f / (float)K) * (last ? 0.1f : 1.0f);
+        for (int e = 0; e < K * N; e++) {
+            const uint32_t p = (uint32_t)(wOff(l) + e);
+            if (last && e % N >= 3)
+                continue;
+            const float u = (float)(pcg(key + p) >> 8) * (1.0f / 16777216.0f);
+            theta[p] = lim * (2.0f * u - 1.0f);
+        }
+    }
+}
+
 int upload_batch(vh_nrc* nn, uint64_t n, const float* raw12, const float* target3, DevBuf& buf, hipStream_t s)
 {
     // sample records: (p, rough) (wo, 0) (n, 0) (target, 1)
@@ -668,7 +691,9 @@ int vh_nrc_create(vh_renderer* r, const vh_nrc_config* cfg, vh_nrc** out)
     HIPTRY(nn->groupSums.alloc(4ull * kParams * ((kTrainBlocks + kReduceGroup - 1) / kReduceGroup)));
     HIPTRY(nn->lossPart.alloc(4ull * kTrainBlocks));
     HIPTRY(nn->loss.alloc(16));
-    HIPTRY(hipMemsetAsync(nn->theta.p, 0, 4ull * kParams, s));
+    std::vector<float> theta0(kParams);
+    init_params(0u, theta0.data()); // seed 0; vh_nrc_init draws another
+    HIPTRY(hipMemcpyAsync(nn->theta.p, theta0.data(), 4ull * kParams, hipMemcpyHostToDevice, s));
     HIPTRY(hipMemsetAsync(nn->m.p, 0, 4ull * kParams, s));
     HIPTRY(hipMemsetAsync(nn->v.p, 0, 4ull * kParams, s));
     k_zero_bf16<<<(kWb + 255) / 256, 256, 0, s>>>(nn->wb.as<__bf16>(), kWb); // padding columns stay zero
@@ -723,6 +748,23 @@ int vh_nrc_set_params(vh_nrc* nn, const float* theta, uint64_t n)
     HIPTRY(hipGetLastError());
     HIPTRY(hipStreamSynchronize(s));
     return VH_OK;
+}
+
+int vh_nrc_init_params(uint32_t seed, float* params, uint64_t n)
+{
+    if (!params || n != (uint64_t)kParams)
+        return fail(VH_ERR_INVALID_ARGUMENT, "expected " + std::to_string(kParams) + " parameters");
+    init_params(seed, params);
+    return VH_OK;
+}
+
+int vh_nrc_init(vh_nrc* nn, uint32_t seed)
+{
+    if (!nn)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    std::vector<float> theta(kParams);
+    init_params(seed, theta.data());
+    return vh_nrc_set_params(nn, theta.data(), theta.size()); // synchronises before theta goes
 }
 
 int vh_nrc_get_params(vh_nrc* nn, float* theta, uint64_t n)

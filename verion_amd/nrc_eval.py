@@ -31,8 +31,6 @@ def evaluate(scene: str = "cornell", width: int = 1920, height: int = 1080, ref_
              log=lambda *a: None) -> dict:
     """Renders the reference, the plain ladder, then one NRC ladder per variant (a dict of
     `Nrc` arguments over DEFAULT_VARIANT), each from freshly initialised parameters."""
-    from oracle import nrc as nrc_init  # parameter initialisation only (He-uniform, seeded)
-
     variants = variants or [{}]
     ctx = Context()
     s = scenes.cornell() if scene == "cornell" else scenes.procedural()
@@ -71,7 +69,7 @@ def evaluate(scene: str = "cornell", width: int = 1920, height: int = 1080, ref_
     for v in variants:
         cfg = dict(DEFAULT_VARIANT, **v)
         cache = Nrc(r, **cfg)
-        cache.set_params(nrc_init.init(seed))
+        cache.init(seed)
         acc = r.new_accumulation(width, height)
 
         def nrc_step(k):

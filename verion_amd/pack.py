@@ -16,6 +16,7 @@ from typing import List, Optional, Tuple
 
 import numpy as np
 
+from . import ies
 from . import scene as sc
 
 MAX_DEPTH = 8            # pack.rs:13
@@ -339,10 +340,12 @@ def virtual_light(s: sc.Scene):
 
 
 def pack_lights(s: sc.Scene):
-    """pack.rs:582-695: delta lights, ranked slots, the virtual main light, albedo tables, link masks."""
+    """pack.rs:582-695: delta lights, ranked slots, the virtual main light, albedo tables, link masks,
+    then the IES profiles of point and spot lights (their frame and table; word 15 = offset + 1)."""
     sky = np.zeros(3, np.float32)
     lights: List[List[int]] = []
     strengths = []
+    profiles = []  # (row in the light table, the light) of every light with an IES profile
     for light in s.lights:
         cone = [0.0, 0.0, 0.0, 0.0]
         offset = 0.0
@@ -373,6 +376,8 @@ def pack_lights(s: sc.Scene):
         words[8:12] = [_f32bits(v) for v in cone]
         words[12] = _f32bits(offset)
         words[14] = light.link_mask & 0xFFFFFFFF
+        if getattr(light, "ies", None) is not None and kind != 1:
+            profiles.append((len(lights), light))
         lights.append(words)
         strengths.append(float(luminance(color) * F32(intensity)))
     ranked = sorted(range(len(strengths)), key=lambda k: -strengths[k])  # stable, strongest first
@@ -396,7 +401,21 @@ def pack_lights(s: sc.Scene):
         masks[2 * i] = s.instance_light_mask(i)
         masks[2 * i + 1] = s.instance_emit_mask(i)
     table.append(masks)
-    return np.concatenate(table).astype(np.uint32), count, sky, vl is not None
+    out = np.concatenate(table).astype(np.uint32)
+    if profiles:
+        parts = [out]
+        size = out.size
+        for row, light in profiles:
+            profile, frame = light.ies
+            frame = np.asarray(frame, np.float32).reshape(9)
+            data = np.asarray(profile.table, np.float32).reshape(-1)
+            if data.size != ies.HORIZONTAL * ies.VERTICAL:
+                raise ValueError(f"an IES table holds {ies.HORIZONTAL} x {ies.VERTICAL} values, not {data.size}")
+            out[row * LIGHT_WORDS + 15] = size + 1
+            parts += [frame.view(np.uint32), data.view(np.uint32)]
+            size += frame.size + data.size
+        out = np.concatenate(parts).astype(np.uint32)
+    return out, count, sky, vl is not None
 
 
 def pack(s: sc.Scene) -> Packed:

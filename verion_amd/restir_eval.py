@@ -28,7 +28,10 @@ from .backend import WAVEFRONT, Context, Renderer, Restir
 
 
 def evaluate(width: int = 1920, height: int = 1080, frames: int = 64, ref_spp: int = 1024, candidates: int = 32,
-             history: int = 4, neighbours: int = 5, seed: int = 1, log=lambda *a: None) -> dict:
+             history: int = 4, neighbours: int = 5, seed: int = 1, plain_frames: int = 0, methods=None,
+             log=lambda *a: None) -> dict:
+    """`plain_frames` (default 16 x frames) bounds the plain ladder the methods' errors are timed
+    on; `methods` picks among restir_di_2_neighbours, restir_di and restir_di_gi (default all)."""
     ctx = Context()
     r = Renderer(ctx, scenes.many_lights())
     r.set_integrator(WAVEFRONT)
@@ -60,16 +63,18 @@ def evaluate(width: int = 1920, height: int = 1080, frames: int = 64, ref_spp: i
         r.set_restir(None)
         return pts
 
-    plain = run(None, frames * 16)
+    plain = run(None, plain_frames or frames * 16)
     log("plain", [(f, round(t, 4), e) for f, t, e in plain])
     at = {f: (t, e) for f, t, e in plain}
     out = {"plain": {"ms_per_frame": 1e3 * at[frames][0] / frames, "relmse": at[frames][1],
                      "ladder": [[f, round(t, 5), e] for f, t, e in plain]}}
     curve = [(t, e) for _, t, e in plain]
-    methods = (("restir_di_2_neighbours", Restir(di=True, gi=False, candidates=candidates, history=history)),
+    all_methods = (("restir_di_2_neighbours", Restir(di=True, gi=False, candidates=candidates, history=history)),
                ("restir_di", Restir(di=True, gi=False, candidates=candidates, history=history, neighbours=neighbours)),
                ("restir_di_gi", Restir(di=True, gi=True, candidates=candidates, history=history, neighbours=neighbours)))
-    for name, restir in methods:
+    for name, restir in all_methods:
+        if methods is not None and name not in methods:
+            continue
         pts = run(restir, frames)
         log(name, [(f, round(t, 4), e) for f, t, e in pts])
         t, e = pts[-1][1], pts[-1][2]

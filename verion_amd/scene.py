@@ -91,6 +91,9 @@ class PointLight:
     color: Tuple[float, float, float] = (1.0, 1.0, 1.0)
     intensity: float = 1.0
     link_mask: int = 0xFFFFFFFF
+    # LightExtra::ies (scene.rs:181-183): an IES profile (verion_amd.ies.IesProfile) and the light's
+    # world frame, rows x, y, z; vertical angle 0 points along -z, horizontal angle 0 along +x
+    ies: Optional[Tuple[object, Tuple[Tuple[float, float, float], ...]]] = None
 
 
 @dataclass
@@ -110,6 +113,7 @@ class SpotLight:
     inner_cone: float = 0.0
     outer_cone: float = 0.7853981633974483
     link_mask: int = 0xFFFFFFFF
+    ies: Optional[Tuple[object, Tuple[Tuple[float, float, float], ...]]] = None  # as PointLight.ies
 
 
 @dataclass
