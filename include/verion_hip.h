@@ -257,6 +257,11 @@ int vh_render_aovs(vh_renderer* r, uint32_t width, uint32_t height, float* albed
 int vh_primary_hits(vh_renderer* r, uint32_t width, uint32_t height, uint32_t sample, uint32_t* hits);
 /* Closest hits of `n` arbitrary rays (org xyz, dir xyz per ray; 4 words out per ray). */
 int vh_trace_rays(vh_renderer* r, uint64_t n, const float* rays, float tmax, uint32_t* hits);
+/* Host-only (no device work): the traversal's limits: stack entries per lane, of which the first
+ * ones live in LDS, the deepest LBVH (levels of 4-wide nodes) a renderer accepts (deeper trees are
+ * VH_ERR_SCENE at vh_renderer_create / vh_renderer_update, since a traversal of a tree of depth D may
+ * hold 3 (D - 1) + 4 entries), and the most triangles per leaf. */
+int vh_trace_limits(uint32_t out[4]);
 /* LBVH shape: nodes, leaves, maximum depth, triangles. */
 int vh_bvh_stats(const vh_renderer* r, uint64_t out[4]);
 /* Traversal counting for the roofline's algorithmic bytes: while on, every ray query adds to

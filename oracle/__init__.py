@@ -58,6 +58,8 @@ def lib():
         for name in ("or_atan2", "or_pow", "or_G1", "or_D", "or_fresnel"):
             getattr(L, name).restype = C.c_float
             getattr(L, name).argtypes = [C.c_float, C.c_float]
+        L.or_diff_prod.restype = C.c_float
+        L.or_diff_prod.argtypes = [C.c_float] * 4
         L.or_sheen_core.restype = C.c_float
         L.or_sheen_core.argtypes = [C.c_float] * 4
         L.or_pcg.restype = C.c_uint32

@@ -135,6 +135,15 @@ static f3 transform_point(const uint32_t* ins, uint32_t base, f3 p)
 
 /* Woop/Benthin/Wald watertight test, no culling. Returns 1 and t, b1, b2 (weights of vertices 1
  * and 2) for 0 < t < tmax. */
+/* a*b - c*d within 2 ulp (Cornea, Harrison and Tang 2003): the products split exactly by fmaf, so
+ * zero stays zero, the sign is right and swapping the products negates the result exactly */
+static float diff_prod(float a, float b, float c, float d)
+{
+    float p1 = a * b, p2 = c * d;
+    float e1 = fmaf(a, b, -p1), e2 = fmaf(c, d, -p2);
+    return (p1 - p2) + (e1 - e2);
+}
+
 static int intersect_tri(const float* v, f3 o, f3 d, float tmax, float* tout, float* b1, float* b2)
 {
     float dv[3] = {d.x, d.y, d.z};
@@ -163,10 +172,10 @@ static int intersect_tri(const float* v, f3 o, f3 d, float tmax, float* tout, fl
     float U = cx * by - cy * bx;
     float V = ax * cy - ay * cx;
     float W = bx * ay - by * ax;
-    if (U == 0.0f || V == 0.0f || W == 0.0f) {
-        U = (float)((double)cx * (double)by - (double)cy * (double)bx);
-        V = (float)((double)ax * (double)cy - (double)ay * (double)cx);
-        W = (float)((double)bx * (double)ay - (double)by * (double)ax);
+    if (U == 0.0f || V == 0.0f || W == 0.0f) { /* decide an edge through the ray exactly */
+        U = diff_prod(cx, by, cy, bx);
+        V = diff_prod(ax, cy, ay, cx);
+        W = diff_prod(bx, ay, by, ax);
     }
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f))
         return 0;
@@ -2059,6 +2068,8 @@ float or_log2(float x) { return or_log2f(x); }
 float or_exp2(float x) { return or_exp2f(x); }
 float or_pow(float x, float y) { return or_powf(x, y); }
 float or_rng_next(uint32_t* state) { return rng_next(state); }
+float or_diff_prod(float a, float b, float c, float d) { return diff_prod(a, b, c, d); }
+
 int or_intersect(const float* tri9, const float* org, const float* dir, float tmax, float* out3)
 {
     float t, b1, b2;

@@ -188,6 +188,38 @@ def test_shared_edges_are_watertight():
         assert _isect(a, o, d)[0] or _isect(b, o, d)[0]
 
 
+def test_difference_of_products_is_exact_in_sign_and_antisymmetric():
+    # the watertight test's fallback for an edge function that rounds to 0: a*b - c*d within 2 ulp
+    # (Cornea, Harrison and Tang), against exact rational arithmetic
+    from fractions import Fraction
+
+    rng = np.random.default_rng(11)
+    vals = rng.normal(size=(4000, 4)).astype(np.float32)
+    vals[:1000, 2] = vals[:1000, 0]  # near-cancelling pairs: a*b close to c*d
+    vals[:1000, 3] = (vals[:1000, 1] * (1 + rng.normal(size=1000) * 1e-6)).astype(np.float32)
+    vals[1000:1100, 2:] = vals[1000:1100, :2]  # exactly cancelling
+    for a, b, c, d in vals:
+        got = L.or_diff_prod(float(a), float(b), float(c), float(d))
+        exact = Fraction(float(a)) * Fraction(float(b)) - Fraction(float(c)) * Fraction(float(d))
+        assert (got > 0) == (exact > 0) and (got < 0) == (exact < 0) and (got == 0) == (exact == 0)
+        if exact != 0:
+            assert abs(Fraction(got) - exact) <= 2 * 2.0 ** -24 * abs(exact)
+        assert L.or_diff_prod(float(c), float(d), float(a), float(b)) == -got
+
+
+def test_edges_through_the_ray_exactly_are_hit_from_both_sides():
+    # rays straight down through dyadic points of a quad's shared diagonal: the single-precision
+    # edge function is exactly 0, so both triangles take the exact fallback and both report the hit
+    a = [0, 0, 0, 1, 0, 0, 1, 1, 0]
+    b = [0, 0, 0, 1, 1, 0, 0, 1, 0]
+    for k in range(1, 64):
+        t = k / 64.0
+        assert _isect(a, [t, t, 1.0], [0, 0, -1])[0] and _isect(b, [t, t, 1.0], [0, 0, -1])[0]
+        # one float off the diagonal: exactly one side
+        off = float(np.nextafter(np.float32(t), np.float32(2.0)))
+        assert _isect(a, [off, t, 1.0], [0, 0, -1])[0] != _isect(b, [off, t, 1.0], [0, 0, -1])[0]
+
+
 def _soup(n, seed):
     from verion_amd import scene as sc
     from verion_amd import pack as pk

@@ -119,6 +119,7 @@ def library() -> C.CDLL:
         "vh_nrc_create": (i32, [vp, C.POINTER(abi.NrcConfig), C.POINTER(vp)]),
         "vh_nrc_destroy": (None, [vp]),
         "vh_nrc_init_params": (i32, [u32, f32p, u64]),
+        "vh_trace_limits": (i32, [u32p]),
         "vh_nrc_init": (i32, [vp, u32]),
         "vh_nrc_set_params": (i32, [vp, f32p, u64]),
         "vh_nrc_get_params": (i32, [vp, f32p, u64]),

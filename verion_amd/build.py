@@ -67,5 +67,5 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose="-v" in sys.argv)
     sys.exit(0)

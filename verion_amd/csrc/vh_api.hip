@@ -322,7 +322,7 @@ int upload_tables(vh_renderer* r, const vh_scene_tables* t)
     DeviceTables dt{r->positions.as<float>(), r->indices.as<uint32_t>(), r->instances.as<uint32_t>(), r->instStart.as<uint32_t>(),
                     r->instanceCount, (uint32_t)ntri};
     TRY(build_bvh(r->bvh, dt, c->stream));
-    if (3 * (r->bvh.depth + 1) + 4 > (uint32_t)kStack)
+    if (r->bvh.depth > kMaxTreeDepth)
         return fail(VH_ERR_SCENE, "LBVH deeper than the traversal stack holds (depth " + std::to_string(r->bvh.depth) + ")");
     return VH_OK;
 }
@@ -369,6 +369,17 @@ using namespace vh;
 extern "C" {
 
 const char* vh_last_error(void) { return g_error.c_str(); }
+
+int vh_trace_limits(uint32_t out[4])
+{
+    if (!out)
+        return fail(VH_ERR_INVALID_ARGUMENT, "null argument");
+    out[0] = (uint32_t)kStack;
+    out[1] = (uint32_t)kShort;
+    out[2] = kMaxTreeDepth;
+    out[3] = kLeafMax;
+    return VH_OK;
+}
 
 int vh_cdf_guide(const float* cdf, uint32_t n, uint32_t bits, uint32_t* guide)
 {
@@ -518,7 +529,7 @@ static int update_tables(vh_renderer* r, const vh_scene_tables* t, const vh_upda
         if (ratio > kRefitLimit) {
             TRY(build_bvh(r->bvh, dt, c->stream));
             rep.bvh_work = 2;
-            if (3 * (r->bvh.depth + 1) + 4 > (uint32_t)kStack)
+            if (r->bvh.depth > kMaxTreeDepth)
                 return fail(VH_ERR_SCENE, "LBVH deeper than the traversal stack holds (depth " + std::to_string(r->bvh.depth) + ")");
         }
         HIPTRY(hipStreamSynchronize(c->stream));

@@ -578,10 +578,11 @@ __global__ __launch_bounds__(256) void k_refit(const float4* __restrict__ tris, 
 }
 
 // surface-area cost of the tree: per block, the sum over the slots of its nodes of the slot box
-// area, times the triangle count for a leaf slot (in FP64; the host adds the blocks in order)
-__global__ __launch_bounds__(256) void k_sah(const float4* __restrict__ nodes, uint32_t n, double* __restrict__ partial)
+// area, times the triangle count for a leaf slot (in FP32, in a fixed order; the host adds the
+// blocks in order in FP64)
+__global__ __launch_bounds__(256) void k_sah(const float4* __restrict__ nodes, uint32_t n, float* __restrict__ partial)
 {
-    double acc = 0.0;
+    float acc = 0.0f;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float4* nd = nodes + 8 * (size_t)i;
         float4 r6 = nd[6], r7 = nd[7];
@@ -593,11 +594,11 @@ __global__ __launch_bounds__(256) void k_sah(const float4* __restrict__ nodes, u
         for (int k = 0; k < 4; k++) {
             if (ref[k] == kEmptyChild)
                 continue;
-            double dx = (double)hx[k] - (double)lx[k], dy = (double)hy[k] - (double)ly[k], dz = (double)hz[k] - (double)lz[k];
-            acc += (dx * dy + dy * dz + dz * dx) * (cnt[k] > 0 ? (double)cnt[k] : 1.0);
+            float dx = hx[k] - lx[k], dy = hy[k] - ly[k], dz = hz[k] - lz[k];
+            acc += (dx * dy + dy * dz + dz * dx) * (cnt[k] > 0 ? (float)cnt[k] : 1.0f);
         }
     }
-    __shared__ double s[256];
+    __shared__ float s[256];
     s[threadIdx.x] = acc;
     __syncthreads();
     for (uint32_t st = 128; st > 0; st >>= 1) {
@@ -642,17 +643,17 @@ static int tree_cost(const BvhBuild& b, hipStream_t s, double& cost)
 {
     uint32_t blocks = std::min<uint32_t>(grid_for(b.nodes, 256), 1024u);
     DevBuf partial;
-    VH_CHECK(partial.alloc(sizeof(double) * blocks));
-    k_sah<<<blocks, 256, 0, s>>>(b.nodes_buf.as<float4>(), b.nodes, partial.as<double>());
+    VH_CHECK(partial.alloc(sizeof(float) * blocks));
+    k_sah<<<blocks, 256, 0, s>>>(b.nodes_buf.as<float4>(), b.nodes, partial.as<float>());
     VH_CHECK(hipGetLastError());
-    std::vector<double> host(blocks);
+    std::vector<float> host(blocks);
     float4 root[6];
-    VH_CHECK(hipMemcpyAsync(host.data(), partial.p, sizeof(double) * blocks, hipMemcpyDeviceToHost, s));
+    VH_CHECK(hipMemcpyAsync(host.data(), partial.p, sizeof(float) * blocks, hipMemcpyDeviceToHost, s));
     VH_CHECK(hipMemcpyAsync(root, b.nodes_buf.p, sizeof(root), hipMemcpyDeviceToHost, s));
     VH_CHECK(hipStreamSynchronize(s));
     double sum = 0.0;
-    for (double v : host)
-        sum += v;
+    for (float v : host)
+        sum += (double)v;
     double ext[3];
     for (int a = 0; a < 3; a++) {
         float4 l = root[2 * a], h = root[2 * a + 1];

@@ -864,6 +864,7 @@ int vh_nrc_accumulate(vh_nrc* nn, vh_accumulation* a, uint32_t frames)
         T.counts = ts.counts.as<uint32_t>();
         T.n = nn->cfg.train_paths;
         T.words = 17;
+        T.traceWaves = nn->r->ctx->cus * kTraceWavesPerCu; // short queues spread over the resident waves
         if (r->info.light_count + 1 <= kMaxShadow) { // shadow rays deferred to the cooperative shadow kernel
             T.shK = kMaxShadow;
             T.shRec = ts.shRec.as<float>();

@@ -618,7 +618,7 @@ __global__ void k_nrc_pick(uint32_t* trainPix, uint32_t nTrain, uint32_t npix, u
     }
 }
 
-__global__ __launch_bounds__(kTraceBlock) void k_nrc_t_extend(Scene S, Params P, Wavefront W, const uint32_t* trainPix, uint32_t d)
+__global__ __launch_bounds__(kTraceBlock, VH_EXTEND_WAVES) void k_nrc_t_extend(Scene S, Params P, Wavefront W, const uint32_t* trainPix, uint32_t d)
 {
     uint32_t count = W.counts[d];
     const uint32_t* st = W.state[d & 1u];

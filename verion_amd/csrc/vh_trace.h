@@ -61,6 +61,18 @@ VH_HD RayPre ray_pre(V3 d)
 
 VH_HD float pick(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
+// a*b - c*d with a relative error of at most 2 ulp (Cornea, Harrison and Tang 2003; bound by
+// Jeannerod, Louvet and Muller 2013): both products are split exactly by FMA, so an exact zero
+// stays 0, any other value keeps its sign, and diffProd(c, d, a, b) is exactly -diffProd(a, b, c, d)
+// (the same edge seen from the triangle on its other side). The watertight test takes it where an
+// edge function rounds to zero in single precision (the case Woop et al. recompute in double).
+VH_HD float diffProd(float a, float b, float c, float d)
+{
+    float p1 = a * b, p2 = c * d;
+    float e1 = fmaf(a, b, -p1), e2 = fmaf(c, d, -p2);
+    return (p1 - p2) + (e1 - e2);
+}
+
 template <bool BARY = true>
 VH_HD bool intersect(const RayPre& r, V3 o, V3 p0, V3 p1, V3 p2, float tmax, float& tout, float& b1, float& b2)
 {
@@ -72,10 +84,10 @@ VH_HD bool intersect(const RayPre& r, V3 o, V3 p0, V3 p1, V3 p2, float tmax, flo
     float U = cx * by - cy * bx;
     float V = ax * cy - ay * cx;
     float W = bx * ay - by * ax;
-    if (U == 0.0f || V == 0.0f || W == 0.0f) {
-        U = (float)((double)cx * (double)by - (double)cy * (double)bx);
-        V = (float)((double)ax * (double)cy - (double)ay * (double)cx);
-        W = (float)((double)bx * (double)ay - (double)by * (double)ax);
+    if (U == 0.0f || V == 0.0f || W == 0.0f) { // an edge through the ray within rounding: decide it exactly
+        U = diffProd(cx, by, cy, bx);
+        V = diffProd(ax, cy, ay, cx);
+        W = diffProd(bx, ay, by, ax);
     }
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f))
         return false;
@@ -105,6 +117,13 @@ VH_HD bool intersect(const RayPre& r, V3 o, V3 p0, V3 p1, V3 p2, float tmax, flo
 __device__ __forceinline__ V3 xyz(float4 a) { return v3(a.x, a.y, a.z); }
 
 constexpr int kStack = 96;
+// Stack entries a traversal may need in a tree of `depth` levels of 4-wide nodes (the root on level
+// 1): a node on level L is reached with at most 3 pending siblings per ancestor on the stack and
+// pushes at most 4 children, so at most 3 (depth - 1) + 4; the renderer keeps a margin of 6.
+__host__ __device__ constexpr uint32_t stackNeed(uint32_t depth) { return 3 * (depth + 1) + 4; }
+// The deepest tree the renderer accepts: stackNeed(kMaxTreeDepth) <= kStack < stackNeed(kMaxTreeDepth + 1).
+constexpr uint32_t kMaxTreeDepth = ((uint32_t)kStack - 4) / 3 - 1;
+static_assert(stackNeed(kMaxTreeDepth) <= (uint32_t)kStack && stackNeed(kMaxTreeDepth + 1) > (uint32_t)kStack, "depth bound");
 // Leaves hold at most kLeafMax triangles (the build makes a leaf of every subtree that small); a
 // node's pending leaf triangles are kept as 4 groups of kLeafGroup bits, one group per child slot.
 // 1080p procedural scene: leaves of up to 8 visit 8% fewer nodes but test 85% more triangles
@@ -177,6 +196,7 @@ struct TraceState {
     float ix, iy, iz, ox, oy, oz;
     uint32_t near; // per axis (bit a): the inverse direction is negative (slab4)
     RayPre pre;
+    uint32_t axes;    // kx | ky << 2 | kz << 4 (the cooperative traversal hands the ray to other lanes)
     float tmax, best; // the ray's limit, and the closest accepted t (or tmax)
     Hit h;
     uint32_t node;
@@ -192,6 +212,7 @@ __device__ __forceinline__ void traceBegin(TraceState& s, V3 o, V3 d, float tmax
     s.h.b1 = s.h.b2 = 0.0f;
     s.o = o;
     s.pre = ray_pre(d);
+    s.axes = (uint32_t)s.pre.kx | ((uint32_t)s.pre.ky << 2) | ((uint32_t)s.pre.kz << 4);
     s.ix = safe_inv(d.x);
     s.iy = safe_inv(d.y);
     s.iz = safe_inv(d.z);
@@ -312,26 +333,35 @@ __device__ __forceinline__ bool traceStep(const Bvh& bvh, TraceState& s, uint32_
 // in the node (up to 16) while most lanes wait: ~0.45 tests per lane and node step against a
 // maximum of 4 to 8. traceWave has the wave's 64 lanes step their rays together and share the
 // triangle tests of each step through LDS: the lanes list their pending (ray, triangle) tests, every
-// lane takes the next 64 of them (usually all), and each ray keeps the smallest (t, id) found,
-// through a 64-bit LDS atomic minimum. A ray's tests within one step use the limit of the step's
-// start instead of one tightened after each test, which only adds tests whose hits lose: the hit,
-// the smallest t with ties to the smallest id, is the same. Lanes whose ray is done keep testing
-// the other lanes' triangles until the wave's last ray ends. All 64 lanes of a wave must call it
-// together (the wave-uniform loops of the wavefront kernels). Its callers need the hit's ids and
-// distance only (shading recomputes the barycentrics from the hit point), so it returns no
-// barycentric weights (b1 = b2 = 0).
+// lane takes the next 64 of them (usually all), reads the owner's ray from the owner's registers
+// (ds_bpermute) and keeps the smallest (t, id) per ray through a 64-bit LDS atomic minimum. A ray's
+// tests within one step use the limit of the step's start instead of one tightened after each
+// test, which only adds tests whose hits lose: the hit, the smallest t with ties to the smallest id,
+// is the same. Lanes whose ray is done keep testing the other lanes' triangles until the wave's last
+// ray ends. All 64 lanes of a wave must call it together (the wave-uniform loops of the wavefront
+// kernels). Its callers need the hit's ids and distance only (shading recomputes the barycentrics
+// from the hit point), so it returns no barycentric weights (b1 = b2 = 0).
+//
+// Short stack: the first kShort entries of each lane's traversal stack live in LDS (entry i of
+// lane l at [i][l], so a wave's accesses never conflict), the rest in the lane's overflow array.
+// On the 1080p procedural scene 0.06% of closest-hit steps hold more than 12 entries (0.5% on the
+// 10,000-emitter scene); the deepest stacks seen were 23 and 29 entries. A closest-hit entry keeps
+// its box entry distance rounded down to bfloat16 (its upper 16 bits): an entry is skipped when
+// that lower bound lies beyond the current hit, so rounding only keeps an entry the exact distance
+// would have skipped, never skips one that could hold the hit.
 
 constexpr uint32_t kWaveTests = 64 * 4 * kLeafGroup; // 64 lanes x 4 leaf children x kLeafMax triangles at most
+constexpr int kShort = 12;                           // stack entries per lane in LDS
+constexpr int kSpill = kStack - kShort;              // the rest, in the lane's overflow array
 
 struct WaveTris {
-    float4 ray0[64];  // origin, and the largest t a triangle can have and still matter this step
-    float4 ray1[64];  // shear constants sx, sy, sz of the watertight test, tmax
-    uint4 base[64];   // first triangle (leaf order) of each child slot of the node
-    uint32_t axes[64]; // kx | ky << 2 | kz << 4
-    unsigned long long key[64]; // (t bits << 32) | global id: the best hit of the step
-    uint2 win[64];    // instance, primitive of that hit
-    uint16_t list[kWaveTests]; // pending tests: owner lane << (2 + kLeafBits) | child slot << kLeafBits | triangle
+    unsigned long long key[64];  // (t bits << 32) | global id: the best hit of the step
+    uint2 win[64];               // instance, primitive of that hit
+    uint32_t stackN[kShort][64]; // the first kShort stack entries of each lane: node
+    uint16_t stackT[kShort][64]; // and (closest hit) the upper half of its entry distance's bits
+    uint16_t list[kWaveTests];   // pending tests: owner lane << (2 + kLeafBits) | child slot << kLeafBits | triangle
 };
+static_assert(sizeof(WaveTris) == 7680, "LDS of one wave's traversal");
 
 __device__ __forceinline__ void waveSync()
 {
@@ -346,20 +376,15 @@ __device__ __forceinline__ uint32_t lanesBelow(uint64_t m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// The ray of this lane, for the lanes that test its triangles.
-__device__ __forceinline__ void wavePublish(const TraceState& s, WaveTris& sh)
-{
-    const uint32_t lane = threadIdx.x & 63u;
-    sh.ray0[lane] = make_float4(s.o.x, s.o.y, s.o.z, s.tmax);
-    sh.ray1[lane] = make_float4(s.pre.sx, s.pre.sy, s.pre.sz, s.tmax);
-    sh.axes[lane] = (uint32_t)s.pre.kx | ((uint32_t)s.pre.ky << 2) | ((uint32_t)s.pre.kz << 4);
-}
+// v of lane `src` (every lane of the wave executes it: an inactive source lane would read as 0)
+__device__ __forceinline__ uint32_t fromLane(uint32_t src, uint32_t v) { return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v); }
+__device__ __forceinline__ float fromLane(uint32_t src, float v) { return asf(fromLane(src, asu(v))); }
 
 // One traversal step of the wave's rays (every lane calls it; `live` lanes have a ray): each live
 // ray visits one node, the wave tests all their leaf triangles together, then each ray pushes its
 // inner children and pops the next node; `live` turns false when the ray is done.
 template <bool ANY>
-__device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& live, WaveTris& sh, uint32_t* stackN, float* stackT)
+__device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& live, WaveTris& sh, uint32_t* spillN, float* spillT)
 {
     const uint32_t lane = threadIdx.x & 63u;
     Hit& h = s.h;
@@ -367,12 +392,15 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
     uint32_t tris = 0;
     float key[4] = {kInf, kInf, kInf, kInf};
     uint32_t val[4] = {0u, 0u, 0u, 0u};
+    uint4 child = make_uint4(0u, 0u, 0u, 0u);
+    // the largest t a triangle can have and still matter this step (ties kept for the id comparison)
+    const float lim = h.hit ? asf(asu(h.t) + 1u) : s.tmax;
     if (live) {
         const float4* n = bvh.nodes + 8 * (size_t)s.node;
         s.visited++;
         float tn[4];
         uint32_t mask = slab4(n, s.near, s.ix, s.iy, s.iz, s.ox, s.oy, s.oz, s.best, tn);
-        uint4 child = *reinterpret_cast<const uint4*>(n + 6);
+        child = *reinterpret_cast<const uint4*>(n + 6);
         uint4 count = *reinterpret_cast<const uint4*>(n + 7);
 #pragma unroll
         for (uint32_t c = 0; c < 4; c++) {
@@ -383,10 +411,32 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
             key[c] = hitc && cnt == 0 ? tn[c] : kInf;
             val[c] = sel4(child, c);
         }
-        if (tris != 0) {
-            sh.base[lane] = child;
+        if (tris != 0)
             sh.key[lane] = ~0ull;
-            sh.ray0[lane].w = h.hit ? asf(asu(h.t) + 1u) : s.tmax; // ties are kept for the id comparison
+        if (!ANY) {
+            cmpx(key[0], val[0], key[1], val[1]);
+            cmpx(key[2], val[2], key[3], val[3]);
+            cmpx(key[0], val[0], key[2], val[2]);
+            cmpx(key[1], val[1], key[3], val[3]);
+            cmpx(key[1], val[1], key[2], val[2]);
+        }
+        // the inner children hit, nearest first (closest hit), are pushed farthest first; before the
+        // triangle tests, which leaves their registers free (the pop after the tests skips what the
+        // tests' hits put out of reach, and an any-hit ray that hits ends with its pushes unread)
+#pragma unroll
+        for (int i = 3; i >= 0; i--) {
+            if (key[i] < kInf && s.sp < kStack) { // the build bounds the depth: the stack cannot fill
+                if (s.sp < kShort) {
+                    sh.stackN[s.sp][lane] = val[i];
+                    if (!ANY) // an any-hit ray ends at its first hit: entry distances are never compared
+                        sh.stackT[s.sp][lane] = (uint16_t)(asu(key[i]) >> 16);
+                } else {
+                    spillN[s.sp - kShort] = val[i];
+                    if (!ANY)
+                        spillT[s.sp - kShort] = key[i];
+                }
+                s.sp++;
+            }
         }
     }
     // exclusive prefix of the per-lane test counts (at most 4 kLeafMax) over the wave
@@ -404,29 +454,32 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
         s.tested += cnt;
         waveSync();
         for (uint32_t r = 0; r < total; r += 64) {
-            uint32_t q = r + lane;
+            const uint32_t q = r + lane;
+            const uint32_t e = q < total ? (uint32_t)sh.list[q] : 0u;
+            const uint32_t owner = e >> (2 + kLeafBits);
+            const uint32_t c = (e >> kLeafBits) & 3u, k = e & (kLeafGroup - 1u);
+            // the owner's ray, limit and child references from its registers (all 64 lanes here)
+            const V3 o = v3(fromLane(owner, s.o.x), fromLane(owner, s.o.y), fromLane(owner, s.o.z));
+            const float olim = fromLane(owner, lim), otmax = fromLane(owner, s.tmax);
+            RayPre rp;
+            rp.sx = fromLane(owner, s.pre.sx);
+            rp.sy = fromLane(owner, s.pre.sy);
+            rp.sz = fromLane(owner, s.pre.sz);
+            const uint32_t ax = fromLane(owner, s.axes);
+            const uint32_t c0 = fromLane(owner, child.x), c1 = fromLane(owner, child.y), c2 = fromLane(owner, child.z),
+                           c3 = fromLane(owner, child.w);
             bool found = false;
-            uint32_t owner = 0;
             unsigned long long mine = 0;
             uint2 w = make_uint2(0u, 0u);
             if (q < total) {
-                uint32_t e = sh.list[q];
-                owner = e >> (2 + kLeafBits);
-                uint32_t c = (e >> kLeafBits) & 3u, k = e & (kLeafGroup - 1u);
-                float4 r0 = sh.ray0[owner], r1 = sh.ray1[owner];
-                uint32_t ax = sh.axes[owner];
-                RayPre rp;
                 rp.kx = (int)(ax & 3u);
                 rp.ky = (int)((ax >> 2) & 3u);
                 rp.kz = (int)((ax >> 4) & 3u);
-                rp.sx = r1.x;
-                rp.sy = r1.y;
-                rp.sz = r1.z;
-                const uint32_t first = reinterpret_cast<const uint32_t*>(sh.base)[4 * owner + c];
+                const uint32_t first = c == 0 ? c0 : (c == 1 ? c1 : (c == 2 ? c2 : c3));
                 const float4* tr = bvh.tris + 3 * (size_t)(first + k);
                 float4 a = tr[0], bb = tr[1], cc = tr[2];
                 float t, b1, b2;
-                if (intersect<false>(rp, v3(r0.x, r0.y, r0.z), xyz(a), xyz(bb), xyz(cc), r0.w, t, b1, b2) && t < r1.w) {
+                if (intersect<false>(rp, o, xyz(a), xyz(bb), xyz(cc), olim, t, b1, b2) && t < otmax) {
                     found = true;
                     mine = ((unsigned long long)asu(t) << 32) | asu(a.w); // t > 0: its bits order like the value
                     w = make_uint2(asu(bb.w), asu(cc.w));
@@ -460,28 +513,24 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
         if (ANY && h.hit) {
             live = false;
         } else {
-            if (!ANY) {
-                cmpx(key[0], val[0], key[1], val[1]);
-                cmpx(key[2], val[2], key[3], val[3]);
-                cmpx(key[0], val[0], key[2], val[2]);
-                cmpx(key[1], val[1], key[3], val[3]);
-                cmpx(key[1], val[1], key[2], val[2]);
-            }
-#pragma unroll
-            for (int i = 3; i >= 0; i--) {
-                if (key[i] < kInf && s.sp < kStack) {
-                    stackN[s.sp] = val[i];
-                    if (!ANY)
-                        stackT[s.sp] = key[i];
-                    s.sp++;
-                }
-            }
+            // pop the next node that can still hold a closer hit
             live = false;
             while (s.sp > 0) {
                 s.sp--;
-                if (!ANY && h.hit && stackT[s.sp] > h.t)
+                uint32_t node;
+                float t = 0.0f;
+                if (s.sp < kShort) {
+                    node = sh.stackN[s.sp][lane];
+                    if (!ANY)
+                        t = asf((uint32_t)sh.stackT[s.sp][lane] << 16);
+                } else {
+                    node = spillN[s.sp - kShort];
+                    if (!ANY)
+                        t = spillT[s.sp - kShort];
+                }
+                if (!ANY && h.hit && t > h.t)
                     continue;
-                s.node = stackN[s.sp];
+                s.node = node;
                 live = true;
                 break;
             }
@@ -496,11 +545,10 @@ __device__ Hit traceWave(const Bvh& bvh, V3 o, V3 d, float tmax, bool active, Wa
     TraceState s;
     traceBegin(s, o, d, tmax);
     bool live = active && bvh.ntri != 0;
-    wavePublish(s, sh);
-    uint32_t stackN[kStack];
-    float stackT[kStack];
+    uint32_t spillN[kSpill];
+    float spillT[kSpill];
     while (__ballot(live) != 0)
-        waveStep<ANY>(bvh, s, live, sh, stackN, stackT);
+        waveStep<ANY>(bvh, s, live, sh, spillN, spillT);
     if (active)
         count_traversal<ANY>(bvh, s.visited, s.tested, s.h.hit);
     return s.h;

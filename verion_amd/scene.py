@@ -102,6 +102,7 @@ class DirectionalLight:
     color: Tuple[float, float, float] = (1.0, 1.0, 1.0)
     intensity: float = 1.0
     link_mask: int = 0xFFFFFFFF
+    ies: Optional[Tuple[object, Tuple[Tuple[float, float, float], ...]]] = None  # kept, not packed (pack.rs:628)
 
 
 @dataclass
