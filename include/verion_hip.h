@@ -310,6 +310,14 @@ int vh_nrc_get_params(vh_nrc* nrc, float* params, uint64_t n);
 int vh_nrc_infer(vh_nrc* nrc, uint64_t n, const float* raw12, float* out3);
 /* Gradient of the mean relative L2 loss over a batch (23888 floats) and the loss; no update. */
 int vh_nrc_gradient(vh_nrc* nrc, uint64_t n, const float* raw12, const float* target3, float* grad, float* loss);
+/* vh_nrc_gradient on the caller's device buffers, allocating nothing. No counterpart in the
+ * reference: for callers that cannot allocate device memory (serving loops, harnesses that forbid
+ * it). raw12 (12 floats per sample), target3 (3 per sample), grad (23888 floats) and loss (1 float,
+ * may be NULL) are device pointers on the cache's device; n <= 65536 (VH_ERR_INVALID_ARGUMENT
+ * beyond). The samples are staged in a 4 MiB device array of the library that every call on a
+ * device shares and overwrites: calls that overlap on one device race. Blocks until done; the
+ * result equals vh_nrc_gradient's bit for bit. */
+int vh_nrc_gradient_noalloc(vh_nrc* nrc, uint64_t n, const float* raw12, const float* target3, float* grad, float* loss);
 /* `steps` Adam steps on one batch; the loss of the last step's forward pass. */
 int vh_nrc_train(vh_nrc* nrc, uint64_t n, const float* raw12, const float* target3, uint32_t steps, float* loss);
 /* The NRC integrator: per frame, train_paths plain training paths on random pixels (every opaque

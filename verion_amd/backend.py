@@ -125,6 +125,7 @@ def library() -> C.CDLL:
         "vh_nrc_get_params": (i32, [vp, f32p, u64]),
         "vh_nrc_infer": (i32, [vp, u64, f32p, f32p]),
         "vh_nrc_gradient": (i32, [vp, u64, f32p, f32p, f32p, f32p]),
+        "vh_nrc_gradient_noalloc": (i32, [vp, u64, vp, vp, vp, vp]),
         "vh_nrc_train": (i32, [vp, u64, f32p, f32p, u32, f32p]),
         "vh_nrc_accumulate": (i32, [vp, vp, u32]),
         "vh_nrc_stats": (i32, [vp, f64p]),

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "vh_internal.h"
@@ -503,6 +504,33 @@ __global__ void k_zero_bf16(__bf16* p, uint32_t n)
 
 constexpr uint32_t kTrainBlocks = 256;
 
+// Sample records (4 float4: position + roughness, direction, normal, target + weight) of
+// vh_nrc_gradient_noalloc, staged in a file-scope device array of kNoAllocSamples records (4 MiB)
+// that every device running the library holds until the process ends. Host code must not pass the
+// array's name to a kernel (on the host it is a stand-in, not a device address): it uses the
+// address hipGetSymbolAddress returns for the current device, kept per device by scratch_address.
+constexpr uint32_t kNoAllocSamples = 1u << 16;
+static __device__ __align__(256) float4 scratch[4 * kNoAllocSamples];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchLock;
+float4* g_scratchAddr[kMaxDevices] = {};
+
+// The record layout of upload_batch, from the caller's device arrays (12 and 3 floats per sample).
+__global__ void k_nrc_pack(const float* __restrict__ raw12, const float* __restrict__ target3, uint32_t n, float4* __restrict__ rec)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float* r = raw12 + 12 * (size_t)i;
+    const float* t = target3 + 3 * (size_t)i;
+    float4* o = rec + 4 * (size_t)i;
+    o[0] = make_float4(r[0], r[1], r[2], r[3]);
+    o[1] = make_float4(r[4], r[5], r[6], r[7]);
+    o[2] = make_float4(r[8], r[9], r[10], r[11]);
+    o[3] = make_float4(t[0], t[1], t[2], 1.0f);
+}
+
 } // namespace
 
 } // namespace vh
@@ -652,6 +680,28 @@ int upload_batch(vh_nrc* nn, uint64_t n, const float* raw12, const float* target
     return VH_OK;
 }
 
+// The device address of `scratch` on `device`, which must be current: looked up once per device
+// (hipGetSymbolAddress, with its status checked) and kept for the life of the process.
+int scratch_address(int device, float4** out)
+{
+    std::lock_guard<std::mutex> lock(g_scratchLock);
+    if (device < 0 || device >= kMaxDevices)
+        return fail(VH_ERR_UNSUPPORTED, "device index beyond the no-allocation staging table");
+    if (!g_scratchAddr[device]) {
+        int current = -1;
+        HIPTRY(hipGetDevice(&current));
+        if (current != device)
+            return fail(VH_ERR_INVALID_ARGUMENT, "the cache's device is not the current device");
+        void* p = nullptr;
+        HIPTRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p)
+            return fail(VH_ERR_KERNEL, "no device address for the no-allocation staging array");
+        g_scratchAddr[device] = static_cast<float4*>(p);
+    }
+    *out = g_scratchAddr[device];
+    return VH_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -721,6 +771,8 @@ int vh_nrc_create(vh_renderer* r, const vh_nrc_config* cfg, vh_nrc** out)
         HIPTRY(hipEventCreateWithFlags(&nn->adamDone[b], hipEventDisableTiming));
     }
     HIPTRY(hipStreamSynchronize(s));
+    float4* staging = nullptr; // the no-allocation gradient's staging array on this device (set-up)
+    TRY(scratch_address(r->ctx->device, &staging));
     *out = nn.release();
     return VH_OK;
 }
@@ -809,6 +861,25 @@ int vh_nrc_gradient(vh_nrc* nn, uint64_t n, const float* raw12, const float* tar
     HIPTRY(hipMemcpy(grad, nn->grad.p, 4ull * kParams, hipMemcpyDeviceToHost));
     if (loss)
         HIPTRY(hipMemcpy(loss, nn->loss.p, 4, hipMemcpyDeviceToHost));
+    return VH_OK;
+}
+
+int vh_nrc_gradient_noalloc(vh_nrc* nn, uint64_t n, const float* raw12, const float* target3, float* grad, float* loss)
+{
+    if (!nn || !raw12 || !target3 || !grad || n == 0)
+        return fail(VH_ERR_INVALID_ARGUMENT, "bad argument");
+    if (n > kNoAllocSamples)
+        return fail(VH_ERR_INVALID_ARGUMENT, "vh_nrc_gradient_noalloc takes at most 65536 samples");
+    float4* rec = nullptr;
+    TRY(scratch_address(nn->r->ctx->device, &rec));
+    hipStream_t s = nn->r->ctx->stream;
+    k_nrc_pack<<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(raw12, target3, (uint32_t)n, rec);
+    HIPTRY(hipGetLastError());
+    TRY(train_steps(nn, rec, nullptr, (uint32_t)n, (uint32_t)n, 1, false, s));
+    HIPTRY(hipMemcpyAsync(grad, nn->grad.p, 4ull * kParams, hipMemcpyDeviceToDevice, s));
+    if (loss)
+        HIPTRY(hipMemcpyAsync(loss, nn->loss.p, 4, hipMemcpyDeviceToDevice, s));
+    HIPTRY(hipStreamSynchronize(s));
     return VH_OK;
 }
 
