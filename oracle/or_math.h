@@ -9,8 +9,9 @@
  * :171-172, :301-305, :658-659; acos/atan2 at :204-205, :252-253; pow at :452; log2 at :338) go
  * through these functions on both sides.
  *
- * Pinned by tests/test_oracle_math.py against numpy's float64 functions (error bounds in ulps) and
- * against closed values (sin(pi/6), acos(0), log2(2^k), exp2(k)).
+ * Pinned by tests/test_oracle_units.py (test_sin_cos_against_float64, test_acos_atan2_against_float64,
+ * test_log2_exp2_pow) against numpy's float64 functions (error bounds in ulps) and against closed
+ * values (sin(pi/6), acos(0), log2(2^k), exp2(k)).
  */
 #ifndef OR_MATH_H
 #define OR_MATH_H

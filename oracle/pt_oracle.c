@@ -24,8 +24,13 @@
  *
  * Parity pins: tests/test_oracle_*.py (closed forms of the reference's own GPU tests, the albedo
  * tables, brute force vs BVH, golden PCG vectors, ReSTIR GI's solid-angle Jacobian, first-frame
- * identity and convergence). Parity unpinned: ReSTIR GI's split of first-bounce emission into the
- * first hit's own light (pt_core.slang:1162-1168) changes no tested image beyond its noise.
+ * identity and convergence); tests/test_oracle_reference_pins.py (the reference's assertions on
+ * environment maps and importance sampling, dispersion, the thin lens, vertex normals and normal
+ * maps, area-light form factors, textures and mips, light slots and the virtual main light, and an
+ * isolating scene for ReSTIR GI's first-bounce emission split, pt_core.slang:1162-1168, whose
+ * removal a planted copy shows the test catches); tests/test_ies.py (IES profiles against
+ * per-sample closed forms); tests/test_oracle_units.py (the exact-sign difference of products of
+ * the watertight test against rational arithmetic). Parity unpinned: none.
  */
 #include <math.h>
 #include <stdint.h>
