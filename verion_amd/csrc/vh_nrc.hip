@@ -392,45 +392,17 @@ __global__ void k_nrc_reduce_groups(const float* partials, uint32_t blocks, floa
     uint32_t g = blockIdx.y;
     if (p >= (uint32_t)kParams)
         return;
-    uint32_t b0 = g * kReduceGroup, b1 = min(blocks, b0 + kReduceGroup);
-    float s = 0.0f;
-    for (uint32_t b = b0; b < b1; b++)
-        s += partials[(size_t)b * kParams + p];
-    groupSums[(size_t)g * kParams + p] = s;
-}
-
-// grad[p] = sum of the group sums in group order; the loss = the blocks' loss partials summed in a
-// fixed order.
-__global__ void k_nrc_reduce(const float* groupSums, uint32_t groups, float* grad, const float* lossPartials, uint32_t blocks,
-                             float* loss, const uint32_t* countPtr, uint32_t countImm, uint32_t cap, uint32_t slice, uint32_t slices)
-{
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < (uint32_t)kParams) {
-        float s = 0.0f;
-        for (uint32_t g = 0; g < groups; g++)
-            s += groupSums[(size_t)g * kParams + p];
-        grad[p] = s;
-    }
-    if (blockIdx.x == 0 && threadIdx.x < 64) {
-        // the loss partials by one wave in a fixed order: lane sums b = lane, lane + 64, ...;
-        // then a fixed butterfly over the lanes (lane 0's result is deterministic)
-        float s = 0.0f;
-        for (uint32_t b = threadIdx.x; b < blocks; b += 64)
-            s += lossPartials[b];
+    const uint32_t b0 = g * kReduceGroup;
+    float part[kReduceGroup];
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
-            s += __shfl_xor(s, off, 64);
-        if (threadIdx.x == 0) {
-            uint32_t total = countPtr ? *countPtr : countImm;
-            if (total > cap)
-                total = cap;
-            uint32_t lo, hi;
-            sliceRange(total, slice, slices, lo, hi);
-            uint32_t n = hi - lo;
-            loss[0] = s / (float)(n > 0 ? n : 1);
-            loss[1] = (float)n;
-        }
-    }
+    for (uint32_t k = 0; k < kReduceGroup; k++) // every load in flight at once
+        part[k] = b0 + k < blocks ? partials[(size_t)(b0 + k) * kParams + p] : 0.0f;
+    float s = 0.0f;
+#pragma unroll
+    for (uint32_t k = 0; k < kReduceGroup; k++)
+        if (b0 + k < blocks)
+            s += part[k];
+    groupSums[(size_t)g * kParams + p] = s;
 }
 
 // Parameter p -> its bf16 copies (forward [n][k] and, when wb is given, backward [k][npad]);
@@ -461,25 +433,69 @@ struct NrcAverage {
     float correction; // 1 - decay^step
 };
 
-__global__ void k_nrc_adam(float* theta, float* m, float* v, const float* grad, float lr, float b1, float b2, float c1, float c2,
-                           float eps, __bf16* wt, __bf16* wb, NrcAverage A)
+constexpr uint32_t kTrainBlocks = 256;
+constexpr uint32_t kReduceGroups = (kTrainBlocks + kReduceGroup - 1) / kReduceGroup;
+
+struct AdamStep {
+    float lr, b1, b2, c1, c2, eps; // c1, c2: the bias corrections 1 - b1^step, 1 - b2^step
+};
+
+// The second stage of the gradient sum and (ADAM) the Adam step, one thread per parameter:
+// grad[p] = the group sums added in group order (a fixed count, so all its loads are in flight at
+// once; the order of a loop over them, hence its bits); the loss = the blocks' loss partials in a
+// fixed order; then Adam (beta 0.9 / 0.99, bias-corrected) on the FP32 master parameter, its bf16
+// copies and the moving average the integrator infers with.
+template <bool ADAM>
+__global__ void k_nrc_reduce_step(const float* groupSums, float* grad, const float* lossPartials, float* loss, const uint32_t* countPtr,
+                                  uint32_t countImm, uint32_t cap, uint32_t slice, uint32_t slices, float* theta, float* m, float* v,
+                                  AdamStep a, __bf16* wt, __bf16* wb, NrcAverage A)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (uint32_t)kParams)
-        return;
-    float g = grad[p];
-    float mm = b1 * m[p] + (1.0f - b1) * g;
-    float vv = b2 * v[p] + (1.0f - b2) * g * g;
-    m[p] = mm;
-    v[p] = vv;
-    float th = theta[p] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
-    theta[p] = th;
-    writeCopies(p, th, wt, wb);
-    float a = A.decay * A.avg[p] + (1.0f - A.decay) * th;
-    A.avg[p] = a;
-    float e = A.decay > 0.0f ? a / A.correction : th;
-    A.eval[p] = e;
-    writeCopies(p, e, A.ewt, nullptr);
+    if (p < (uint32_t)kParams) {
+        float part[kReduceGroups];
+#pragma unroll
+        for (uint32_t g = 0; g < kReduceGroups; g++)
+            part[g] = groupSums[(size_t)g * kParams + p];
+        float gsum = 0.0f;
+#pragma unroll
+        for (uint32_t g = 0; g < kReduceGroups; g++)
+            gsum += part[g];
+        grad[p] = gsum;
+        if (ADAM) {
+            float mm = a.b1 * m[p] + (1.0f - a.b1) * gsum;
+            float vv = a.b2 * v[p] + (1.0f - a.b2) * gsum * gsum;
+            m[p] = mm;
+            v[p] = vv;
+            float th = theta[p] - a.lr * (mm / a.c1) / (sqrtf(vv / a.c2) + a.eps);
+            theta[p] = th;
+            writeCopies(p, th, wt, wb);
+            float av = A.decay * A.avg[p] + (1.0f - A.decay) * th;
+            A.avg[p] = av;
+            float e = A.decay > 0.0f ? av / A.correction : th;
+            A.eval[p] = e;
+            writeCopies(p, e, A.ewt, nullptr);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 64) {
+        // the loss partials by one wave in a fixed order: lane sums b = lane, lane + 64, ...;
+        // then a fixed butterfly over the lanes (lane 0's result is deterministic)
+        float ls = 0.0f;
+        for (uint32_t b = threadIdx.x; b < kTrainBlocks; b += 64)
+            ls += lossPartials[b];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            ls += __shfl_xor(ls, off, 64);
+        if (threadIdx.x == 0) {
+            uint32_t total = countPtr ? *countPtr : countImm;
+            if (total > cap)
+                total = cap;
+            uint32_t lo, hi;
+            sliceRange(total, slice, slices, lo, hi);
+            uint32_t n = hi - lo;
+            loss[0] = ls / (float)(n > 0 ? n : 1);
+            loss[1] = (float)n;
+        }
+    }
 }
 
 // theta -> its bf16 copies; the average restarts at theta (its running sum at zero).
@@ -501,8 +517,6 @@ __global__ void k_zero_bf16(__bf16* p, uint32_t n)
     if (i < n)
         p[i] = (__bf16)0.0f;
 }
-
-constexpr uint32_t kTrainBlocks = 256;
 
 // Sample records (4 float4: position + roughness, direction, normal, target + weight) of
 // vh_nrc_gradient_noalloc, staged in a file-scope device array of kNoAllocSamples records (4 MiB)
@@ -620,18 +634,21 @@ int train_steps(vh_nrc* nn, const float4* samples, const uint32_t* countPtr, uin
         const uint32_t slice = slices > 1 ? k % slices : 0u, nsl = slices > 1 ? slices : 1u;
         k_nrc_train<<<kTrainBlocks, 256, 0, s>>>(net, samples, countPtr, countImm, cap, slice, nsl, nn->partials.as<float>(),
                                                  nn->lossPart.as<float>());
-        const uint32_t groups = (kTrainBlocks + kReduceGroup - 1) / kReduceGroup;
-        k_nrc_reduce_groups<<<dim3((kParams + 255) / 256, groups), 256, 0, s>>>(nn->partials.as<float>(), kTrainBlocks,
-                                                                                nn->groupSums.as<float>());
-        k_nrc_reduce<<<(kParams + 255) / 256, 256, 0, s>>>(nn->groupSums.as<float>(), groups, nn->grad.as<float>(),
-                                                           nn->lossPart.as<float>(), kTrainBlocks, nn->loss.as<float>(), countPtr,
-                                                           countImm, cap, slice, nsl);
+        k_nrc_reduce_groups<<<dim3((kParams + 255) / 256, kReduceGroups), 256, 0, s>>>(nn->partials.as<float>(), kTrainBlocks,
+                                                                                       nn->groupSums.as<float>());
+        AdamStep st{nn->cfg.learning_rate, 0.9f, 0.99f, 1.0f, 1.0f, 1.0e-8f};
         if (adam) {
             nn->step++;
-            float c1 = 1.0f - powf(0.9f, (float)nn->step), c2 = 1.0f - powf(0.99f, (float)nn->step);
-            k_nrc_adam<<<(kParams + 255) / 256, 256, 0, s>>>(nn->theta.as<float>(), nn->m.as<float>(), nn->v.as<float>(),
-                                                             nn->grad.as<float>(), nn->cfg.learning_rate, 0.9f, 0.99f, c1, c2, 1.0e-8f,
-                                                             nn->wt.as<__bf16>(), nn->wb.as<__bf16>(), nn->average());
+            st.c1 = 1.0f - powf(0.9f, (float)nn->step);
+            st.c2 = 1.0f - powf(0.99f, (float)nn->step);
+            k_nrc_reduce_step<true><<<(kParams + 255) / 256, 256, 0, s>>>(
+                nn->groupSums.as<float>(), nn->grad.as<float>(), nn->lossPart.as<float>(), nn->loss.as<float>(), countPtr, countImm, cap,
+                slice, nsl, nn->theta.as<float>(), nn->m.as<float>(), nn->v.as<float>(), st, nn->wt.as<__bf16>(), nn->wb.as<__bf16>(),
+                nn->average());
+        } else {
+            k_nrc_reduce_step<false><<<(kParams + 255) / 256, 256, 0, s>>>(
+                nn->groupSums.as<float>(), nn->grad.as<float>(), nn->lossPart.as<float>(), nn->loss.as<float>(), countPtr, countImm, cap,
+                slice, nsl, nullptr, nullptr, nullptr, st, nullptr, nullptr, NrcAverage{});
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess)
