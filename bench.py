@@ -14,7 +14,9 @@ GPU seconds to relMSE 1e-3 of the neural radiance cache and of the plain tracer 
 at 1080p, 1 spp per frame, against a plain reference render (16,384 spp, the workload's, by default); `batch`:
 p50/p99 job latency and aggregate Msamples/s of 32 uneven jobs in one shared wavefront against
 one job at a time; and `restir_many_lights`: ms per frame and relMSE of plain, ReSTIR DI and DI + GI
-on the many-lights workload at 1080p.
+on the many-lights workload at 1080p; and `reference_benchmark`: the reference's own throughput
+benchmark (its cube scene and 5M-triangle scale scene, 1024x768, 1,024 spp) beside the GPU seconds
+it publishes for RTX 3080, M4 Pro and M5.
 
     python bench.py [--gpus 1] [--steps 64] [--warmup 4]
 """
@@ -131,6 +133,8 @@ def main():
     ap.add_argument("--no-nrc", action="store_true", help="skip the NRC vs plain time-to-relMSE measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the 32-job batch measurement")
     ap.add_argument("--no-restir", action="store_true", help="skip the many-lights ReSTIR measurement")
+    ap.add_argument("--no-reference-scenes", action="store_true",
+                    help="skip the reference's own benchmark scenes (cube and the 5M-triangle scale scene)")
     ap.add_argument("--no-cornell", action="store_true", help="skip the Cornell box measurement (for profiler runs)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (8 blocks per CU, 8x8 tiles)")
     ap.add_argument("--shape", type=str, default=None, help="fixed launch shape BLOCKS,TILE_W,TILE_H instead of tuning")
@@ -267,6 +271,8 @@ def main():
         line["restir_many_lights"] = restir_many_lights(W, H)
     if not args.no_batch:
         line["batch"] = batch_jobs()
+    if not args.no_reference_scenes:
+        line["reference_benchmark"] = reference_benchmark()
     print(json.dumps(line), flush=True)
 
 
@@ -325,6 +331,16 @@ def batch_jobs():
     return {"jobs": res["jobs"], "pixels_total": res["pixels_total"], "spp_range": res["spp_range"],
             "adaptive": res["adaptive"], "batched": {k: res["batched"][k] for k in keep + ("tuning",)},
             "one_at_a_time": {k: res["one_at_a_time"][k] for k in keep}, "films_checked_equal": res["films_checked_equal"]}
+
+
+def reference_benchmark():
+    """The reference's own throughput benchmark (scripts/bench.sh: benchmarks/cube.gltf and the
+    generated 5M-triangle scale scene, 1024x768, 1,024 spp) on this backend, beside the GPU seconds
+    the reference publishes for it (RTX 3080 Vulkan, Apple M4 Pro and M5)."""
+    from verion_amd import reference_bench
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    return reference_bench.evaluate(os.path.join(here, "tests", "golden", "cube.gltf"), log=log)
 
 
 def restir_many_lights(width, height):

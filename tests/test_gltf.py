@@ -216,3 +216,27 @@ def test_reference_cornell_box_imports_and_renders():
     st = o.render(pack.base_params(p.info, p.view, 24, 18, False, True), 2)
     img = oracle.mean(st, 2)
     assert np.isfinite(img).all() and img.mean() > 0.05
+
+
+def test_frame_camera_fits_the_bounds_like_the_cli():
+    # scene.rs:453-477: centre of the world bounds, backed off along +z until the bounding sphere
+    # fills the vertical field of view (0.8), 5 % margin; focus at that distance; a scene without
+    # instances keeps its camera
+    import math
+
+    from verion_amd import scene as sc
+    from verion_amd import scenes
+
+    s = sc.Scene()
+    m = s.add_mesh(scenes.cube_mesh(0.5))
+    s.add_instance(sc.Instance(m, s.add_material(sc.BaseBsdf()), ((2.0, 0.0, 0.0, 1.0), (0.0, 1.0, 0.0, -2.0), (0.0, 0.0, 1.0, 3.0))))
+    s.frame_camera()
+    r = math.sqrt(2.0 ** 2 + 1.0 + 1.0) / 2.0
+    dist = r / math.sin(0.4) * 1.05
+    t = s.camera.transform
+    assert abs(t[0][3] - 1.0) < 1e-6 and abs(t[1][3] + 2.0) < 1e-6 and abs(t[2][3] - (3.0 + dist)) < 1e-5
+    assert abs(s.camera.focus_distance - dist) < 1e-5 and s.camera.vertical_fov == 0.8
+    empty = sc.Scene()
+    before = empty.camera
+    empty.frame_camera()
+    assert empty.camera == before

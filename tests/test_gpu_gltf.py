@@ -73,3 +73,28 @@ def test_film_equals_the_oracle(cornell, integrator):
     p = cornell.packed
     want = oracle.OracleScene(p).render(pack.base_params(p.info, p.view, w, h, False, True), spp)["film"]
     assert np.array_equal(acc.film(), want)
+
+
+@pytest.mark.parametrize("name", ["cube", "scale"])
+@pytest.mark.parametrize("integrator", [MEGAKERNEL, WAVEFRONT])
+def test_reference_benchmark_scenes_match_the_oracle(ctx, name, integrator):
+    # the reference's throughput benchmark inputs (scripts/bench.sh): benchmarks/cube.gltf as is, and
+    # the scale scene's generator at a reduced size (its textures, normal map and metallic-roughness
+    # map on a 24,000-triangle sphere), cameras framed as the CLI frames them
+    from verion_amd import scenes
+
+    if name == "cube":
+        s, warnings = gltf.load_gltf(os.path.join(GOLDEN, "cube.gltf"))
+    else:
+        s, warnings = gltf.load_gltf_bytes(scenes.reference_scale_glb(lat=100, lon=120, tex=512))
+    assert not warnings
+    s.frame_camera()
+    r = Renderer(ctx, s)
+    r.set_integrator(integrator)
+    acc = r.new_accumulation(64, 48)
+    r.accumulate(acc, 8)
+    want = oracle.OracleScene(r.packed).render(pack.base_params(r.packed.info, r.packed.view, 64, 48, False, True), 8)["film"]
+    got = acc.film()
+    rel = np.abs(got - want) / np.maximum(np.abs(want), 1e-6)
+    assert rel.max() <= 1e-4, f"exact {np.mean(got == want):.4f}"
+    assert got.max() > 0

@@ -241,6 +241,33 @@ class Scene:
         self.changes.lights = True
         self.lights[i] = light
 
+    def frame_camera(self) -> None:
+        """scene.rs:453-477 (the CLI's default when the file has no camera, main.rs:294-296): the
+        default camera moved back along +z from the centre of the world-space bounds until a sphere
+        around them fills the vertical field of view, with 5 % margin; focus at that distance."""
+        f = np.float32
+        lo = np.full(3, np.finfo(np.float32).max, np.float32)
+        hi = np.full(3, np.finfo(np.float32).min, np.float32)
+        for inst in self.instances:
+            t = np.asarray(inst.transform, np.float32)
+            p = self.meshes[inst.mesh].positions.astype(np.float32)
+            for a in range(3):
+                v = ((t[a, 0] * p[:, 0] + t[a, 1] * p[:, 1]) + t[a, 2] * p[:, 2]) + t[a, 3]
+                if v.size:
+                    lo[a] = min(lo[a], v.min())
+                    hi[a] = max(hi[a], v.max())
+        if lo[0] > hi[0]:
+            return
+        c = [(lo[a] + hi[a]) / f(2.0) for a in range(3)]
+        r = f(np.sqrt(sum(f(hi[a] - lo[a]) ** 2 for a in range(3)))) / f(2.0)
+        cam = Camera()
+        dist = f(r / f(np.sin(f(cam.vertical_fov) / f(2.0)))) * f(1.05)
+        rows = [list(row) for row in cam.transform]
+        rows[0][3], rows[1][3], rows[2][3] = float(c[0]), float(c[1]), float(c[2] + dist)
+        cam.transform = tuple(tuple(row) for row in rows)
+        cam.focus_distance = float(dist)
+        self.set_camera(cam)
+
     def set_camera(self, camera: Camera) -> None:
         self.changes.camera = True
         self.camera = camera

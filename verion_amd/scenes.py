@@ -305,3 +305,81 @@ def many_lights(emitters: int = 10000, clutter_boxes: int = 41000, seed: int = 1
         s.add_instance(sc.Instance(m, mat))
     s.camera = sc.Camera(transform=((1.0, 0.0, 0.0, 0.0), (0.0, 0.9578, 0.2873, 9.0), (0.0, -0.2873, 0.9578, 32.0)))
     return s
+
+
+def reference_scale_glb(lat: int = 1250, lon: int = 2000, tex: int = 4096) -> bytes:
+    """The reference's scale benchmark scene (scripts/gen_scale_scene.py: REQ-PERF-003), built here
+    as one GLB document: a UV sphere of 2 lat lon triangles (5.0M) with normals and uvs over
+    [0, 8] x [0, 4] at (0, 1, 0) on a 12 m ground quad, four tex x tex PNG textures repeating a
+    256 x 256 tile (base colour, normal, metallic-roughness on the sphere; a checker on the ground)
+    and a point light of 600 at (3, 6, 4). Load it with verion_amd.gltf.load_gltf_bytes; with no
+    camera in the file the CLI frames one (Scene.frame_camera)."""
+    import io
+    import json
+    import struct
+
+    from PIL import Image as PilImage
+
+    def tile_png(fn):
+        y, x = np.mgrid[0:256, 0:256].astype(np.float64)
+        t = np.stack(fn(x, y), axis=-1)
+        full = np.tile(t, (tex // 256, tex // 256, 1)).astype(np.uint8)
+        buf = io.BytesIO()
+        PilImage.fromarray(full, "RGB").save(buf, "PNG", compress_level=1)
+        return buf.getvalue()
+
+    clamp = lambda v: np.clip(np.trunc(v), 0, 255)  # max(0, min(255, int(v)))
+    const = lambda c: lambda x, y: np.full_like(x, c)
+    images = [
+        tile_png(lambda x, y: (clamp(128 + 100 * np.sin(x / 9)), clamp(128 + 100 * np.sin(y / 7)), const(90)(x, y))),
+        tile_png(lambda x, y: (clamp(128 + 40 * np.sin(x / 5)), clamp(128 + 40 * np.sin(y / 5)), const(230)(x, y))),
+        tile_png(lambda x, y: (const(0)(x, y), clamp(140 + 80 * np.sin((x + y) / 11)), clamp(60 + 40 * np.sin(x / 13)))),
+        tile_png(lambda x, y: (np.where((x // 32 + y // 32) % 2 == 1, 200, 90),) * 3),
+    ]
+    i = np.arange(lat + 1, dtype=np.float64)[:, None]
+    j = np.arange(lon + 1, dtype=np.float64)[None, :]
+    th, ph = np.pi * i / lat, 2 * np.pi * j / lon
+    n = np.stack(np.broadcast_arrays(np.sin(th) * np.cos(ph), np.cos(th), np.sin(th) * np.sin(ph)), axis=-1).reshape(-1, 3)
+    pos = (n + np.array([0.0, 1.0, 0.0])).astype(np.float32)
+    nrm = n.astype(np.float32)
+    uv = np.stack(np.broadcast_arrays(j / lon * 8 + 0 * i, i / lat * 4 + 0 * j), axis=-1).reshape(-1, 2).astype(np.float32)
+    a = (np.arange(lat)[:, None] * (lon + 1) + np.arange(lon)[None, :]).reshape(-1).astype(np.uint32)
+    b = a + np.uint32(lon + 1)
+    idx = np.stack([a, b, a + 1, a + 1, b, b + 1], axis=-1).reshape(-1).astype(np.uint32)
+    gp = np.array([-6, 0, -6, -6, 0, 6, 6, 0, 6, 6, 0, -6], np.float32)
+    gn = np.array([0, 1, 0] * 4, np.float32)
+    gu = np.array([0, 0, 0, 6, 6, 6, 6, 0], np.float32)
+    gi = np.array([0, 1, 2, 0, 2, 3], np.uint32)
+    chunks = [c.tobytes() for c in (pos, nrm, uv, idx, gp, gn, gu, gi)] + images
+    views, blob = [], b""
+    for c in chunks:
+        views.append({"buffer": 0, "byteOffset": len(blob), "byteLength": len(c)})
+        blob += c + b"\0" * (-len(c) % 4)
+    nv = len(pos)
+    acc = lambda v, t, count, ct=5126, **kw: {"bufferView": v, "componentType": ct, "count": count, "type": t, **kw}
+    doc = {
+        "asset": {"version": "2.0"}, "scene": 0, "scenes": [{"nodes": [0, 1, 2]}],
+        "extensionsUsed": ["KHR_lights_punctual"],
+        "extensions": {"KHR_lights_punctual": {"lights": [{"type": "point", "color": [1, 1, 1], "intensity": 600}]}},
+        "nodes": [{"mesh": 0}, {"mesh": 1}, {"translation": [3, 6, 4], "extensions": {"KHR_lights_punctual": {"light": 0}}}],
+        "meshes": [
+            {"primitives": [{"attributes": {"POSITION": 0, "NORMAL": 1, "TEXCOORD_0": 2}, "indices": 3, "material": 0}]},
+            {"primitives": [{"attributes": {"POSITION": 4, "NORMAL": 5, "TEXCOORD_0": 6}, "indices": 7, "material": 1}]},
+        ],
+        "materials": [
+            {"pbrMetallicRoughness": {"baseColorTexture": {"index": 0}, "metallicRoughnessTexture": {"index": 2}},
+             "normalTexture": {"index": 1}},
+            {"pbrMetallicRoughness": {"baseColorTexture": {"index": 3}, "metallicFactor": 0, "roughnessFactor": 1}},
+        ],
+        "textures": [{"source": k} for k in range(4)],
+        "images": [{"bufferView": 8 + k, "mimeType": "image/png"} for k in range(4)],
+        "accessors": [acc(0, "VEC3", nv, min=[-1, 0, -1], max=[1, 2, 1]), acc(1, "VEC3", nv), acc(2, "VEC2", nv),
+                      acc(3, "SCALAR", len(idx), 5125), acc(4, "VEC3", 4, min=[-6, 0, -6], max=[6, 0, 6]), acc(5, "VEC3", 4),
+                      acc(6, "VEC2", 4), acc(7, "SCALAR", 6, 5125)],
+        "bufferViews": views, "buffers": [{"byteLength": len(blob)}],
+    }
+    js = json.dumps(doc).encode("utf-8")
+    js += b" " * (-len(js) % 4)
+    total = 12 + 8 + len(js) + 8 + len(blob)
+    return (struct.pack("<III", 0x46546C67, 2, total) + struct.pack("<II", len(js), 0x4E4F534A) + js
+            + struct.pack("<II", len(blob), 0x004E4942) + blob)
