@@ -30,7 +30,6 @@ constexpr int kIn = 96, kHid = 64, kOut = 16, kNL = 6;
 constexpr int kRows = 64;   // rows per block tile (4 waves x 16)
 constexpr int kLdX = 104;   // padded leading dimension of 96-wide rows in LDS
 constexpr int kLdH = 72;    // padded leading dimension of 64-wide rows in LDS
-constexpr int kLdT = 72;    // padded leading dimension of transposed tiles (64 rows of samples) in LDS
 
 __host__ __device__ constexpr int layerK(int l) { return l == 0 ? kIn : kHid; }
 __host__ __device__ constexpr int layerN(int l) { return l == kNL - 1 ? kOut : kHid; }
@@ -172,16 +171,22 @@ __global__ __launch_bounds__(256) void k_nrc_infer(NrcNet net, const float4* rec
     }
 }
 
-// One layer of the backward pass for a 64-row tile: dW (and db) accumulate into this wave's tiles,
+// The training kernel works on 32-row tiles (two 16-row MFMA tiles): its activations, their
+// transposed copies and the gradients of one tile take 71 KB of LDS, so two blocks share a CU.
+constexpr int kTrainRows = 32;
+constexpr int kLdS = 40; // padded leading dimension of transposed training tiles (32 samples) in LDS
+
+// One layer of the backward pass for a 32-row tile: dW (and db) accumulate into this wave's tiles,
 // then (l > 0) dZ of the layer below = (dZ W^T) * relu'(input).
 template <int L, int NT>
 __device__ __forceinline__ void backLayer(const NrcNet& net, const __bf16* hinT, const __bf16* dz, const __bf16* dzT, __bf16* dzNext,
                                           __bf16* dzNextT, f32x4_t (&acc)[NT], float& db)
 {
-    // Layer L's backward over one 64-row tile. hinT: the layer's input transposed ([k][row]); dz
-    // and dzT: its output gradient, row-major and transposed; writes the input's gradient (through
-    // the ReLU of the layer below) in both layouts. The forward pass stores every activation
-    // transposed as well, so the weight gradient hinT x dz reads MFMA operands straight from LDS.
+    // Layer L's backward over one tile. hinT: the layer's input transposed ([k][row]); dz and dzT:
+    // its output gradient, row-major and transposed; writes the input's gradient (through the ReLU
+    // of the layer below) in both layouts. The forward pass stores every activation transposed as
+    // well, so the weight gradient hinT x dz reads MFMA operands straight from LDS: one 32-deep
+    // MFMA per 16 x 16 output tile.
     constexpr int K = layerK(L), N = layerN(L), NP = layerNpad(L);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
     constexpr int NTILE = N / 16;
@@ -189,13 +194,13 @@ __device__ __forceinline__ void backLayer(const NrcNet& net, const __bf16* hinT,
     for (int j = 0; j < NT; j++) {
         int t = wave + 4 * j;
         int kt = t / NTILE, nt = t % NTILE;
-        acc[j] = mma16(hinT + kt * 16 * kLdT, kLdT, dzT + nt * 16 * kLdT, kLdT, kRows, acc[j]);
+        acc[j] = mma16(hinT + kt * 16 * kLdS, kLdS, dzT + nt * 16 * kLdS, kLdS, kTrainRows, acc[j]);
     }
     if ((int)threadIdx.x < N) { // the bias gradient: one contiguous row of dzT
-        const bf16x8_t* row = reinterpret_cast<const bf16x8_t*>(dzT + threadIdx.x * kLdT);
+        const bf16x8_t* row = reinterpret_cast<const bf16x8_t*>(dzT + threadIdx.x * kLdS);
         float s = 0.0f;
 #pragma unroll
-        for (int q = 0; q < kRows / 8; q++) {
+        for (int q = 0; q < kTrainRows / 8; q++) {
             bf16x8_t v = row[q];
 #pragma unroll
             for (int e = 0; e < 8; e++)
@@ -204,19 +209,21 @@ __device__ __forceinline__ void backLayer(const NrcNet& net, const __bf16* hinT,
         db += s;
     }
     if constexpr (L > 0) {
+        // the 2 row tiles x K/16 column tiles of the input gradient, 2 (K = 64) or 3 (K = 96) per wave
         const __bf16* W = net.wb + wbOff(L);
+        const int rt = wave & 1;
 #pragma unroll
-        for (int kt = 0; kt < K / 16; kt++) {
+        for (int kt = wave >> 1; kt < K / 16; kt += 2) {
             f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
-            c = mma16(dz + wave * 16 * kLdH, kLdH, W + kt * 16 * NP, NP, NP, c);
+            c = mma16(dz + rt * 16 * kLdH, kLdH, W + kt * 16 * NP, NP, NP, c);
             int col = kt * 16 + r;
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                int row = wave * 16 + g * 4 + t;
-                float h = (float)hinT[col * kLdT + row];
+                int row = rt * 16 + g * 4 + t;
+                float h = (float)hinT[col * kLdS + row];
                 __bf16 v = (__bf16)(h > 0.0f ? c[t] : 0.0f);
                 dzNext[row * kLdH + col] = v;
-                dzNextT[col * kLdT + row] = v;
+                dzNextT[col * kLdS + row] = v;
             }
         }
     }
@@ -248,20 +255,21 @@ __host__ __device__ __forceinline__ void sliceRange(uint32_t n, uint32_t slice, 
     hi = (uint32_t)((uint64_t)n * (slice + 1) / slices);
 }
 
-// Fused forward + backward over 64-row tiles of training samples (4 float4 each: 3 of inputs, the
+// Fused forward + backward over 32-row tiles of training samples (4 float4 each: 3 of inputs, the
 // RGB target), over slice `slice` of `slices` of them. Writes this block's gradient partial sums
-// (kParams floats) and loss sum.
-__global__ __launch_bounds__(256) void k_nrc_train(NrcNet net, const float4* samples, const uint32_t* countPtr, uint32_t countImm,
-                                                   uint32_t cap, uint32_t slice, uint32_t slices, float* partials, float* lossPartials)
+// (kParams floats) and loss sum. Every product is split over the 4 waves by 16-row tile and
+// column tile, with a block barrier between layers.
+__global__ __launch_bounds__(256, 2) void k_nrc_train(NrcNet net, const float4* samples, const uint32_t* countPtr, uint32_t countImm,
+                                                      uint32_t cap, uint32_t slice, uint32_t slices, float* partials, float* lossPartials)
 {
-    __shared__ __attribute__((aligned(16))) __bf16 s_x[kRows * kLdX];      // encoded inputs, row-major
-    __shared__ __attribute__((aligned(16))) __bf16 s_xT[kIn * kLdT];       // and transposed
-    __shared__ __attribute__((aligned(16))) __bf16 s_hr[2][kRows * kLdH];  // hidden activations, row-major (ping-pong)
-    __shared__ __attribute__((aligned(16))) __bf16 s_hT[5][kHid * kLdT];   // every hidden activation, transposed
-    __shared__ float s_y[kRows][kOut];
-    __shared__ float s_t[kRows][4];
-    __shared__ __attribute__((aligned(16))) __bf16 s_dz[2][kRows * kLdH];
-    __shared__ __attribute__((aligned(16))) __bf16 s_dzT[2][kHid * kLdT];
+    __shared__ __attribute__((aligned(16))) __bf16 s_x[kTrainRows * kLdX];     // encoded inputs, row-major
+    __shared__ __attribute__((aligned(16))) __bf16 s_xT[kIn * kLdS];           // and transposed
+    __shared__ __attribute__((aligned(16))) __bf16 s_hr[2][kTrainRows * kLdH]; // hidden activations, row-major (ping-pong)
+    __shared__ __attribute__((aligned(16))) __bf16 s_hT[5][kHid * kLdS];       // every hidden activation, transposed
+    __shared__ float s_y[kTrainRows][4];                                        // RGB outputs
+    __shared__ float s_t[kTrainRows][4];                                        // RGB targets, weight
+    __shared__ __attribute__((aligned(16))) __bf16 s_dz[2][kTrainRows * kLdH];
+    __shared__ __attribute__((aligned(16))) __bf16 s_dzT[2][kHid * kLdS];
     __shared__ float s_loss[256];
     uint32_t total = countPtr ? *countPtr : countImm;
     if (total > cap)
@@ -279,65 +287,63 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcNet net, const float4* sam
     for (auto& v : a4) v = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
     a5[0] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
     float lossAcc = 0.0f;
-    for (uint32_t tile = first + blockIdx.x * kRows; tile < n; tile += gridDim.x * kRows) {
-        // encode, row-major and transposed
-        const int row = wave * 16 + r;
-        uint32_t q = tile + row;
-        __bf16* xr = s_x + row * kLdX + g * 24;
-        if (q < n) {
-            const float4* s = samples + (size_t)q * 4;
-            encodePart(net, s[0], s[1], s[2], g, xr);
-            if (g == 0) {
-                float4 t = s[3];
-                s_t[row][0] = t.x;
-                s_t[row][1] = t.y;
-                s_t[row][2] = t.z;
-                s_t[row][3] = 1.0f;
+    const int rt = wave & 1; // this wave's 16-row tile in the forward products
+    for (uint32_t tile = first + blockIdx.x * kTrainRows; tile < n; tile += gridDim.x * kTrainRows) {
+        // encode (waves 0-1: one row and a quarter of its inputs per lane), row-major and transposed
+        if (wave < 2) {
+            const int row = wave * 16 + r;
+            uint32_t q = tile + row;
+            __bf16* xr = s_x + row * kLdX + g * 24;
+            if (q < n) {
+                const float4* sp = samples + (size_t)q * 4;
+                encodePart(net, sp[0], sp[1], sp[2], g, xr);
+                if (g == 0) {
+                    float4 t = sp[3];
+                    s_t[row][0] = t.x;
+                    s_t[row][1] = t.y;
+                    s_t[row][2] = t.z;
+                    s_t[row][3] = 1.0f;
+                }
+            } else {
+                for (int k = 0; k < 24; k++)
+                    xr[k] = (__bf16)0.0f;
+                if (g == 0)
+                    s_t[row][3] = 0.0f;
             }
-        } else {
             for (int k = 0; k < 24; k++)
-                xr[k] = (__bf16)0.0f;
-            if (g == 0)
-                s_t[row][3] = 0.0f;
+                s_xT[(g * 24 + k) * kLdS + row] = xr[k];
         }
-        for (int k = 0; k < 24; k++)
-            s_xT[(g * 24 + k) * kLdT + row] = xr[k];
-        // a wave's forward pass reads only its own 16 rows (row-major copies): a wave barrier
-        // until the output layer, whose loss step reads every row. The transposed copies are read
-        // by the backward pass, after that block barrier, and the next tile writes them only
-        // after the backward pass's last block barrier.
-        waveSync();
-        // forward; B operands stream from the global bf16 copy (L2-resident, 47 KB)
+        __syncthreads();
+        // forward: wave w takes row tile w & 1 and the column tiles w >> 1, (w >> 1) + 2 of each
+        // layer (the output layer's one column tile on waves 0-1); B operands stream from the global
+        // bf16 copy (L2-resident, 47 KB)
 #pragma unroll
         for (int l = 0; l < kNL; l++) {
             const int K = layerK(l), N = layerN(l);
             const __bf16* in = l == 0 ? s_x : s_hr[(l - 1) & 1];
             const int ldin = l == 0 ? kLdX : kLdH;
 #pragma unroll
-            for (int nt = 0; nt < N / 16; nt++) {
+            for (int nt = wave >> 1; nt < N / 16; nt += 2) {
                 f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
-                c = mma16(in + wave * 16 * ldin, ldin, net.wt + wtOff(l) + nt * 16 * K, K, K, c);
+                c = mma16(in + rt * 16 * ldin, ldin, net.wt + wtOff(l) + nt * 16 * K, K, K, c);
                 int col = nt * 16 + r;
                 float bias = net.theta[bOff(l) + col];
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
-                    int rw = wave * 16 + g * 4 + t;
+                    int rw = rt * 16 + g * 4 + t;
                     if (l < kNL - 1) {
                         __bf16 v = (__bf16)fmaxf(c[t] + bias, 0.0f);
                         s_hr[l & 1][rw * kLdH + col] = v;
-                        s_hT[l][col * kLdT + rw] = v;
-                    } else {
+                        s_hT[l][col * kLdS + rw] = v;
+                    } else if (col < 3) {
                         s_y[rw][col] = c[t] + bias;
                     }
                 }
             }
-            if (l == kNL - 1)
-                __syncthreads();
-            else
-                waveSync();
+            __syncthreads();
         }
         // relative L2 loss gradient, d(loss)/dy = 2 (y - t) / (y^2 + 0.01) / B for RGB
-        if (threadIdx.x < kRows) {
+        if (threadIdx.x < kTrainRows) {
             int i = threadIdx.x;
             bool ok = s_t[i][3] != 0.0f;
             for (int c = 0; c < 32; c++) {
@@ -350,7 +356,7 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcNet net, const float4* sam
                 }
                 s_dz[0][i * kLdH + c] = (__bf16)gv;
                 if (c < kOut)
-                    s_dzT[0][c * kLdT + i] = (__bf16)gv;
+                    s_dzT[0][c * kLdS + i] = (__bf16)gv;
             }
         }
         __syncthreads();
@@ -371,12 +377,12 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcNet net, const float4* sam
     s_loss[threadIdx.x] = lossAcc;
     __syncthreads();
     if (threadIdx.x < 64) { // fixed order: four per lane, then a butterfly (lane 0's is kept)
-        float s = ((s_loss[threadIdx.x] + s_loss[threadIdx.x + 64]) + s_loss[threadIdx.x + 128]) + s_loss[threadIdx.x + 192];
+        float sl = ((s_loss[threadIdx.x] + s_loss[threadIdx.x + 64]) + s_loss[threadIdx.x + 128]) + s_loss[threadIdx.x + 192];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1)
-            s += __shfl_xor(s, off, 64);
+            sl += __shfl_xor(sl, off, 64);
         if (threadIdx.x == 0)
-            lossPartials[blockIdx.x] = s;
+            lossPartials[blockIdx.x] = sl;
     }
 }
 
@@ -384,7 +390,7 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcNet net, const float4* sam
 // sample order): first groups of kReduceGroup consecutive blocks, in block order, into one sum per
 // group (enough threads to stream the partials at memory bandwidth), then the group sums in
 // group order.
-constexpr uint32_t kReduceGroup = 16;
+constexpr uint32_t kReduceGroup = 32;
 
 __global__ void k_nrc_reduce_groups(const float* partials, uint32_t blocks, float* groupSums)
 {
@@ -433,7 +439,7 @@ struct NrcAverage {
     float correction; // 1 - decay^step
 };
 
-constexpr uint32_t kTrainBlocks = 256;
+constexpr uint32_t kTrainBlocks = 512; // two blocks per CU
 constexpr uint32_t kReduceGroups = (kTrainBlocks + kReduceGroup - 1) / kReduceGroup;
 
 struct AdamStep {
