@@ -173,13 +173,13 @@ __device__ __forceinline__ float safe_inv(float d)
 }
 
 template <bool ANY>
-__device__ __forceinline__ void count_traversal(const Bvh& bvh, uint32_t visited, uint32_t tested, bool hit)
+__device__ __forceinline__ void count_traversal(const Bvh& bvh, uint32_t rays, uint32_t visited, uint32_t tested, bool hit)
 {
     if (bvh.counters) {
         // 64 banks of 8 counters, picked by block, keep the atomics from serialising on one address
         unsigned long long* bank = bvh.counters + 8 * (blockIdx.x & 63u);
         unsigned long long* c = bank + (ANY ? 3 : 0);
-        atomicAdd(c, 1ull);
+        atomicAdd(c, (unsigned long long)rays);
         atomicAdd(c + 1, (unsigned long long)visited);
         atomicAdd(c + 2, (unsigned long long)tested);
         if (hit && !ANY)
@@ -202,6 +202,7 @@ struct TraceState {
     uint32_t node;
     int sp;
     uint32_t visited, tested;
+    uint32_t ray; // wave traversal: the lane whose ray this lane's node belongs to
 };
 
 __device__ __forceinline__ void traceBegin(TraceState& s, V3 o, V3 d, float tmax)
@@ -333,14 +334,17 @@ __device__ __forceinline__ bool traceStep(const Bvh& bvh, TraceState& s, uint32_
 // in the node (up to 16) while most lanes wait: ~0.45 tests per lane and node step against a
 // maximum of 4 to 8. traceWave has the wave's 64 lanes step their rays together and share the
 // triangle tests of each step through LDS: the lanes list their pending (ray, triangle) tests, every
-// lane takes the next 64 of them (usually all), reads the owner's ray from the owner's registers
-// (ds_bpermute) and keeps the smallest (t, id) per ray through a 64-bit LDS atomic minimum. A ray's
-// tests within one step use the limit of the step's start instead of one tightened after each
-// test, which only adds tests whose hits lose: the hit, the smallest t with ties to the smallest id,
-// is the same. Lanes whose ray is done keep testing the other lanes' triangles until the wave's last
-// ray ends. All 64 lanes of a wave must call it together (the wave-uniform loops of the wavefront
-// kernels). Its callers need the hit's ids and distance only (shading recomputes the barycentrics
-// from the hit point), so it returns no barycentric weights (b1 = b2 = 0).
+// lane takes the next 64 of them (usually all), reads the listing lane's ray from that lane's
+// registers (ds_bpermute) and keeps the smallest (t, id) per ray through a 64-bit LDS atomic
+// minimum. A ray's tests within one step use the limit of the step's start instead of one tightened
+// after each test, which only adds tests whose hits lose: the hit, the smallest t with ties to the
+// smallest id, is the same. Lanes whose ray is done keep testing the other lanes' triangles, and
+// take over parts of the other rays' traversals (work stealing, below), until the wave's last ray
+// ends. Every node a ray reaches is visited once, by one lane, and every test that can hold the
+// hit is made whichever lane makes it, so the hit does not depend on who did which part. All 64
+// lanes of a wave must call it together (the wave-uniform loops of the wavefront kernels). Its
+// callers need the hit's ids and distance only (shading recomputes the barycentrics from the hit
+// point), so it returns no barycentric weights (b1 = b2 = 0).
 //
 // Short stack: the first kShort entries of each lane's traversal stack live in LDS (entry i of
 // lane l at [i][l], so a wave's accesses never conflict), the rest in the lane's overflow array.
@@ -355,11 +359,12 @@ constexpr int kShort = 12;                           // stack entries per lane i
 constexpr int kSpill = kStack - kShort;              // the rest, in the lane's overflow array
 
 struct WaveTris {
-    unsigned long long key[64];  // (t bits << 32) | global id: the best hit of the step
+    unsigned long long key[64];  // per ray (its owner lane): (t bits << 32) | global id of the best hit so far, ~0 for none
     uint2 win[64];               // instance, primitive of that hit
     uint32_t stackN[kShort][64]; // the first kShort stack entries of each lane: node
     uint16_t stackT[kShort][64]; // and (closest hit) the upper half of its entry distance's bits
-    uint16_t list[kWaveTests];   // pending tests: owner lane << (2 + kLeafBits) | child slot << kLeafBits | triangle
+    uint16_t list[kWaveTests];   // pending tests: listing lane << (2 + kLeafBits) | child slot << kLeafBits | triangle;
+                                 // while work is handed out, the offered entries: donor lane << 4 | stack index
 };
 static_assert(sizeof(WaveTris) == 7680, "LDS of one wave's traversal");
 
@@ -380,39 +385,61 @@ __device__ __forceinline__ uint32_t lanesBelow(uint64_t m)
 __device__ __forceinline__ uint32_t fromLane(uint32_t src, uint32_t v) { return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v); }
 __device__ __forceinline__ float fromLane(uint32_t src, float v) { return asf(fromLane(src, asu(v))); }
 
-// One traversal step of the wave's rays (every lane calls it; `live` lanes have a ray): each live
-// ray visits one node, the wave tests all their leaf triangles together, then each ray pushes its
-// inner children and pops the next node; `live` turns false when the ray is done.
+// Work stealing inside a wave: once at least kStealMinIdle lanes have no node, every lane with at
+// most kShort stack entries offers its top kOfferMax entries and the idle lanes take them in lane
+// order. 1080p procedural scene (wave steps x 64 over node visits): lane use 0.50 -> 0.83 for
+// closest-hit and 0.39 -> 0.81 for shadow rays, for 11% and 17% more node visits (a taken subtree
+// is searched before the owner's later hits could cull it); per trace launch 0.257 -> 0.175 ms.
+// Measured: 1, 4, 8, 16, 24 and 32 idle lanes and 1, 2, 4 and 7 entries per offer; 16 and 2 (or 1
+// or 4) within 1% of the best, 1 idle lane 1.5% and 32 2% slower.
+constexpr uint32_t kStealMinIdle = 16; // idle lanes a wave needs before it hands out work
+constexpr uint32_t kOfferMax = 2;      // stack entries one lane gives away per step
+constexpr uint32_t kOfferBits = 2;     // bits of an offer count (0..kOfferMax)
+static_assert(kOfferMax < (1u << kOfferBits) && kShort <= 16, "offers are counted in kOfferBits bits, entries named in 4");
+
+// best (t, id) of ray `r` so far (sh.key[r], ~0 for none) as (hit, t)
+__device__ __forceinline__ bool bestOf(const WaveTris& sh, uint32_t r, float& t)
+{
+    const unsigned long long k = sh.key[r];
+    t = asf((uint32_t)(k >> 32));
+    return k != ~0ull;
+}
+
+// One traversal step of the wave (every lane calls it): each live lane visits one node of the ray
+// it works for (s.ray: its own, or another lane's it took work from), the wave tests all their
+// leaf triangles together into that ray's best (t, id) in LDS, then each lane pushes its inner
+// children and pops the next node; `live` turns false when the lane has no node left. Lanes left
+// without a node then take the top stack entries of lanes that still have some (work stealing
+// inside the wave), with a copy of that ray, so a wave no longer runs as long as its longest ray.
 template <bool ANY>
 __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& live, WaveTris& sh, uint32_t* spillN, float* spillT)
 {
     const uint32_t lane = threadIdx.x & 63u;
-    Hit& h = s.h;
     const float kInf = __builtin_inff();
     uint32_t tris = 0;
     float key[4] = {kInf, kInf, kInf, kInf};
     uint32_t val[4] = {0u, 0u, 0u, 0u};
     uint4 child = make_uint4(0u, 0u, 0u, 0u);
+    float bt;
+    const bool bh = bestOf(sh, s.ray, bt);
     // the largest t a triangle can have and still matter this step (ties kept for the id comparison)
-    const float lim = h.hit ? asf(asu(h.t) + 1u) : s.tmax;
+    const float lim = bh ? asf(asu(bt) + 1u) : s.tmax;
     if (live) {
         const float4* n = bvh.nodes + 8 * (size_t)s.node;
         s.visited++;
         float tn[4];
-        uint32_t mask = slab4(n, s.near, s.ix, s.iy, s.iz, s.ox, s.oy, s.oz, s.best, tn);
+        uint32_t mask = slab4(n, s.near, s.ix, s.iy, s.iz, s.ox, s.oy, s.oz, bh ? bt : s.tmax, tn);
         child = *reinterpret_cast<const uint4*>(n + 6);
         uint4 count = *reinterpret_cast<const uint4*>(n + 7);
 #pragma unroll
         for (uint32_t c = 0; c < 4; c++) {
             bool hitc = (mask >> c & 1u) != 0 && sel4(child, c) != kEmptyChild;
             uint32_t cnt = sel4(count, c);
-            if (hitc && cnt != 0 && !(h.hit && tn[c] > h.t))
+            if (hitc && cnt != 0 && !(bh && tn[c] > bt))
                 tris |= ((1u << cnt) - 1u) << (kLeafGroup * c);
             key[c] = hitc && cnt == 0 ? tn[c] : kInf;
             val[c] = sel4(child, c);
         }
-        if (tris != 0)
-            sh.key[lane] = ~0ull;
         if (!ANY) {
             cmpx(key[0], val[0], key[1], val[1]);
             cmpx(key[2], val[2], key[3], val[3]);
@@ -458,9 +485,10 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
             const uint32_t e = q < total ? (uint32_t)sh.list[q] : 0u;
             const uint32_t owner = e >> (2 + kLeafBits);
             const uint32_t c = (e >> kLeafBits) & 3u, k = e & (kLeafGroup - 1u);
-            // the owner's ray, limit and child references from its registers (all 64 lanes here)
+            // the listing lane's ray, limit and child references from its registers (all 64 lanes here)
             const V3 o = v3(fromLane(owner, s.o.x), fromLane(owner, s.o.y), fromLane(owner, s.o.z));
             const float olim = fromLane(owner, lim), otmax = fromLane(owner, s.tmax);
+            const uint32_t oray = fromLane(owner, s.ray);
             RayPre rp;
             rp.sx = fromLane(owner, s.pre.sx);
             rp.sy = fromLane(owner, s.pre.sy);
@@ -483,34 +511,18 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
                     found = true;
                     mine = ((unsigned long long)asu(t) << 32) | asu(a.w); // t > 0: its bits order like the value
                     w = make_uint2(asu(bb.w), asu(cc.w));
-                    atomicMin(&sh.key[owner], mine);
+                    atomicMin(&sh.key[oray], mine);
                 }
             }
             waveSync();
-            if (found && sh.key[owner] == mine)
-                sh.win[owner] = w;
+            if (found && sh.key[oray] == mine)
+                sh.win[oray] = w;
             waveSync();
         }
-        if (tris != 0) {
-            unsigned long long kk = sh.key[lane];
-            if (kk != ~0ull) {
-                float t = asf((uint32_t)(kk >> 32));
-                uint32_t gid = (uint32_t)kk;
-                if (!h.hit || t < h.t || (t == h.t && gid < h.gid)) {
-                    uint2 w = sh.win[lane];
-                    h.hit = true;
-                    h.t = t;
-                    h.gid = gid;
-                    h.inst = w.x;
-                    h.prim = w.y;
-                    s.best = t;
-                }
-            }
-        }
-        waveSync();
     }
     if (live) {
-        if (ANY && h.hit) {
+        const bool nh = bestOf(sh, s.ray, bt);
+        if (ANY && nh) {
             live = false;
         } else {
             // pop the next node that can still hold a closer hit
@@ -528,7 +540,7 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
                     if (!ANY)
                         t = spillT[s.sp - kShort];
                 }
-                if (!ANY && h.hit && t > h.t)
+                if (!ANY && nh && t > bt)
                     continue;
                 s.node = node;
                 live = true;
@@ -536,21 +548,96 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
             }
         }
     }
+    // work stealing: lanes without a node take the top entries (all in LDS) of lanes that have some
+    const uint64_t idleM = __ballot(!live);
+    const uint32_t offer = live && s.sp <= kShort ? ((uint32_t)s.sp < kOfferMax ? (uint32_t)s.sp : kOfferMax) : 0u;
+    const uint64_t offM = __ballot(offer != 0u);
+    const uint32_t nIdle = (uint32_t)__builtin_popcountll(idleM);
+    if (offM != 0 && nIdle >= kStealMinIdle) {
+        uint32_t opre = 0, ototal = 0;
+#pragma unroll
+        for (uint32_t b = 0; b < kOfferBits; b++) {
+            uint64_t m = __ballot((offer >> b) & 1u);
+            opre += lanesBelow(m) << b;
+            ototal += (uint32_t)__builtin_popcountll(m) << b;
+        }
+        // the offers in lane order go to the idle lanes in lane order
+        const uint32_t take = opre >= nIdle ? 0u : (offer < nIdle - opre ? offer : nIdle - opre);
+        for (uint32_t i = 0; i < take; i++)
+            sh.list[opre + i] = (uint16_t)((lane << 4) | (uint32_t)(s.sp - 1 - (int)i));
+        s.sp -= (int)take;
+        waveSync();
+        const uint32_t rank = lanesBelow(idleM);
+        const bool gets = !live && rank < ototal;
+        const uint32_t e = gets ? (uint32_t)sh.list[rank] : 0u;
+        const uint32_t v = e >> 4, idx = e & 15u;
+        // the donor's ray (every lane executes the permutes)
+        const float vox = fromLane(v, s.o.x), voy = fromLane(v, s.o.y), voz = fromLane(v, s.o.z);
+        const float vix = fromLane(v, s.ix), viy = fromLane(v, s.iy), viz = fromLane(v, s.iz);
+        const float vsx = fromLane(v, s.pre.sx), vsy = fromLane(v, s.pre.sy), vsz = fromLane(v, s.pre.sz);
+        const uint32_t vax = fromLane(v, s.axes), vray = fromLane(v, s.ray);
+        const float vtmax = fromLane(v, s.tmax);
+        if (gets) {
+            const uint32_t node = sh.stackN[idx][v];
+            float et = 0.0f;
+            if (!ANY)
+                et = asf((uint32_t)sh.stackT[idx][v] << 16);
+            float vb;
+            const bool vh = bestOf(sh, vray, vb);
+            if (ANY ? !vh : (!vh || !(et > vb))) {
+                s.o = v3(vox, voy, voz);
+                s.ix = vix;
+                s.iy = viy;
+                s.iz = viz;
+                s.ox = -vox * vix;
+                s.oy = -voy * viy;
+                s.oz = -voz * viz;
+                s.near = (vix < 0.0f ? 1u : 0u) | (viy < 0.0f ? 2u : 0u) | (viz < 0.0f ? 4u : 0u);
+                s.pre.sx = vsx;
+                s.pre.sy = vsy;
+                s.pre.sz = vsz;
+                s.axes = vax;
+                s.ray = vray;
+                s.tmax = vtmax;
+                s.node = node;
+                s.sp = 0;
+                live = true;
+            }
+        }
+        waveSync();
+    }
 }
 
-// One ray per lane, traced by the whole wave together.
+// One ray per lane, traced by the whole wave together; a ray's best (t, id) lives in sh.key / sh.win
+// while the wave's lanes work on it.
 template <bool ANY>
 __device__ Hit traceWave(const Bvh& bvh, V3 o, V3 d, float tmax, bool active, WaveTris& sh)
 {
+    const uint32_t lane = threadIdx.x & 63u;
     TraceState s;
     traceBegin(s, o, d, tmax);
+    s.ray = lane;
     bool live = active && bvh.ntri != 0;
     uint32_t spillN[kSpill];
     float spillT[kSpill];
+    sh.key[lane] = ~0ull;
+    waveSync();
     while (__ballot(live) != 0)
         waveStep<ANY>(bvh, s, live, sh, spillN, spillT);
-    if (active)
-        count_traversal<ANY>(bvh, s.visited, s.tested, s.h.hit);
+    waveSync();
+    Hit& h = s.h;
+    const unsigned long long kk = sh.key[lane];
+    if (active && kk != ~0ull) {
+        const uint2 w = sh.win[lane];
+        h.hit = true;
+        h.t = asf((uint32_t)(kk >> 32));
+        h.gid = (uint32_t)kk;
+        h.inst = w.x;
+        h.prim = w.y;
+    }
+    waveSync();
+    if (active || s.visited != 0)
+        count_traversal<ANY>(bvh, active ? 1u : 0u, s.visited, s.tested, active && h.hit);
     return s.h;
 }
 
@@ -565,7 +652,7 @@ __device__ Hit trace(const Bvh& bvh, V3 o, V3 d, float tmax)
     float stackT[kStack];
     while (!traceStep<ANY>(bvh, s, stackN, stackT)) {
     }
-    count_traversal<ANY>(bvh, s.visited, s.tested, s.h.hit);
+    count_traversal<ANY>(bvh, 1u, s.visited, s.tested, s.h.hit);
     return s.h;
 }
 
