@@ -325,7 +325,9 @@ int vh_nrc_train(vh_nrc* nrc, uint64_t n, const float* raw12, const float* targe
  * cache at its first opaque vertex at terminate_depth, into `acc` (FP64 film). Not with the
  * lighting split, adaptive sampling or ReSTIR (VH_ERR_UNSUPPORTED). */
 int vh_nrc_accumulate(vh_nrc* nrc, vh_accumulation* acc, uint32_t frames);
-/* Last loss, last training-sample count, total training seconds (warm-up included), total rendering seconds. */
+/* Last loss, last training-sample count, total training seconds (the warm-up, and the time the
+ * render stream waits for a frame's Adam steps, which otherwise run beside its path tracing),
+ * total rendering seconds; the two add up to the frames' GPU time. */
 int vh_nrc_stats(const vh_nrc* nrc, double out[4]);
 
 /* --- batched jobs (no counterpart in the reference: its session renders one job at a time,

@@ -69,13 +69,16 @@ __device__ void encodePart(const NrcNet& net, float4 q0, float4 q1, float4 q2, i
     if (part < 3) {
         float p = part == 0 ? q0.x : (part == 1 ? q0.y : q0.z);
         p = fminf(fmaxf((p - net.lo[part]) * net.inv[part], 0.0f), 1.0f);
-        float base = kPi * p;
+        // sin, cos (2^k pi p) = sin, cos (2 pi frac(2^(k-1) p)): the scaling by a power of two and
+        // the fraction are exact, so the only rounding is the hardware sine's (v_sin_f32 and
+        // v_cos_f32 take their argument in revolutions), far below the bf16 rounding that follows
+        float x = 0.5f * p;
 #pragma unroll
         for (int k = 0; k < 12; k++) {
-            float s, c;
-            sincosf((float)(1u << k) * base, &s, &c);
-            dst[2 * k] = (__bf16)s;
-            dst[2 * k + 1] = (__bf16)c;
+            const float f = __builtin_amdgcn_fractf(x);
+            dst[2 * k] = (__bf16)__builtin_amdgcn_sinf(f);
+            dst[2 * k + 1] = (__bf16)__builtin_amdgcn_cosf(f);
+            x *= 2.0f;
         }
     } else {
         float x = q1.x, y = q1.y, z = q1.z;
@@ -126,7 +129,7 @@ __global__ __launch_bounds__(256) void k_nrc_infer(NrcNet net, const float4* rec
             for (int k = 0; k < 24; k++)
                 a0[k] = (__bf16)0.0f;
         }
-        __syncthreads();
+        waveSync(); // a wave's rows are its own from the encoding to the output: no block barrier
         int bo = 0;
 #pragma unroll
         for (int l = 0; l < kNL; l++) {
@@ -166,7 +169,7 @@ __global__ __launch_bounds__(256) void k_nrc_infer(NrcNet net, const float4* rec
                 }
             }
             bo += N;
-            __syncthreads();
+            waveSync();
         }
     }
 }
@@ -578,6 +581,11 @@ struct vh_nrc {
     DevBuf samples[kBufs], sampleCount[kBufs];
     hipEvent_t pathsDone[kBufs] = {}, adamDone[kBufs] = {};
     bool adamIssued[kBufs] = {};
+    // The Adam steps run on a stream of their own, beside the render's path tracing: only the
+    // render's cache inference reads the weights, so it alone waits for the frame's steps, and the
+    // next frame's steps wait for that inference (inferDone) before they overwrite the weights.
+    hipStream_t adamStream = nullptr;
+    hipEvent_t inferDone = nullptr;
     uint32_t step = 0;
     uint32_t frames = 0;
     uint32_t trainFrames = 0; // frames of training paths drawn so far (warm-up included)
@@ -608,15 +616,21 @@ struct vh_nrc {
         for (auto& t : sets)
             if (t.stream)
                 (void)hipStreamSynchronize(t.stream);
+        if (adamStream)
+            (void)hipStreamSynchronize(adamStream);
         for (uint32_t b = 0; b < kBufs; b++) {
             if (pathsDone[b])
                 (void)hipEventDestroy(pathsDone[b]);
             if (adamDone[b])
                 (void)hipEventDestroy(adamDone[b]);
         }
+        if (inferDone)
+            (void)hipEventDestroy(inferDone);
         for (auto& t : sets)
             if (t.stream)
                 (void)hipStreamDestroy(t.stream);
+        if (adamStream)
+            (void)hipStreamDestroy(adamStream);
     }
 };
 
@@ -793,6 +807,8 @@ int vh_nrc_create(vh_renderer* r, const vh_nrc_config* cfg, vh_nrc** out)
         HIPTRY(hipEventCreateWithFlags(&nn->pathsDone[b], hipEventDisableTiming));
         HIPTRY(hipEventCreateWithFlags(&nn->adamDone[b], hipEventDisableTiming));
     }
+    HIPTRY(hipStreamCreateWithFlags(&nn->adamStream, hipStreamNonBlocking));
+    HIPTRY(hipEventCreateWithFlags(&nn->inferDone, hipEventDisableTiming));
     HIPTRY(hipStreamSynchronize(s));
     float4* staging = nullptr; // the no-allocation gradient's staging array on this device (set-up)
     TRY(scratch_address(r->ctx->device, &staging));
@@ -977,24 +993,29 @@ int vh_nrc_accumulate(vh_nrc* nn, vh_accumulation* a, uint32_t frames)
         HIPTRY(hipEventRecord(nn->pathsDone[b], ts.stream));
         return VH_OK;
     };
+    bool inferIssued = false; // an inference of this call that the next Adam steps must wait for
     auto adam = [&](uint32_t tf) -> int {
         const uint32_t b = tf % vh_nrc::kBufs;
-        HIPTRY(hipStreamWaitEvent(s, nn->pathsDone[b], 0));
-        TRY(train_steps(nn, nn->samples[b].as<float4>(), nn->sampleCount[b].as<uint32_t>(), 0, sampleCap, nn->cfg.train_steps, true, s,
+        hipStream_t as = nn->adamStream;
+        HIPTRY(hipStreamWaitEvent(as, nn->pathsDone[b], 0));
+        if (inferIssued)
+            HIPTRY(hipStreamWaitEvent(as, nn->inferDone, 0));
+        TRY(train_steps(nn, nn->samples[b].as<float4>(), nn->sampleCount[b].as<uint32_t>(), 0, sampleCap, nn->cfg.train_steps, true, as,
                         nn->cfg.minibatch ? nn->cfg.train_steps : 1u));
-        HIPTRY(hipEventRecord(nn->adamDone[b], s));
+        HIPTRY(hipEventRecord(nn->adamDone[b], as));
         nn->adamIssued[b] = true;
         return VH_OK;
     };
-    // one sample of every pixel, ending in the cache at its first opaque vertex at termDepth
-    auto render = [&](uint32_t f) -> int {
-        const NrcNet net = nn->net(false, true);
+    // one sample of every pixel, ending in the cache at its first opaque vertex at termDepth: the
+    // paths up to the cache queries (no weights read), then, once the frame's Adam steps are done
+    // (`waitAdam`, which records the timing events around the wait), the inference and the rest
+    const bool merge = termDepth >= 1 && termDepth < maxDepth && W.shK > 0 && !S.translucent;
+    auto renderPaths = [&](uint32_t f) -> int {
         Params P = base;
         P.w[6] = a->samples + f;
         P.w[31] = (uint32_t)npix;
         TRY(launch_wf_reset(P, W, s));
         // opaque scenes: the cache depth's closest rays go with the previous depth's shadow rays
-        const bool merge = termDepth >= 1 && termDepth < maxDepth && W.shK > 0 && !S.translucent;
         for (uint32_t d = 0; d < std::min(termDepth, maxDepth); d++)
             TRY(launch_wf_depth(S, F, P, W, d, true, cfg, merge && d + 1 == termDepth));
         if (termDepth >= maxDepth)
@@ -1003,22 +1024,39 @@ int vh_nrc_accumulate(vh_nrc* nn, vh_accumulation* a, uint32_t frames)
             TRY(launch_wf_trace_resolve(S, F, P, W, termDepth, cfg));
         else
             TRY(launch_wf_depth(S, F, P, W, termDepth, false, cfg));
-        TRY(launch_nrc_query_shade(S, F, P, W, nn->queries.as<float4>(), termDepth, cfg));
+        return launch_nrc_query_shade(S, F, P, W, nn->queries.as<float4>(), termDepth, cfg);
+    };
+    auto renderCache = [&](uint32_t f) -> int {
+        if (termDepth >= maxDepth)
+            return VH_OK;
+        Params P = base;
+        P.w[6] = a->samples + f;
+        P.w[31] = (uint32_t)npix;
+        const NrcNet net = nn->net(false, true);
         uint32_t grid = std::min<uint32_t>((uint32_t)((npix + kRows - 1) / kRows), r->ctx->cus * 2);
         k_nrc_infer<1><<<grid, 256, 0, s>>>(net, nn->queries.as<float4>(), W.counts + termDepth, 0u, nullptr, F.film);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess)
             return fail(VH_ERR_KERNEL, std::string("NRC inference: ") + hipGetErrorString(e));
+        HIPTRY(hipEventRecord(nn->inferDone, s));
+        inferIssued = true;
+        // the depths past the cache carry only the paths it could not end (none on most scenes):
+        // small grids, no spreading, so their near-empty launches cost little
+        Wavefront WT = W;
+        WT.traceWaves = 0;
+        const LaunchCfg tail{s, std::min<uint32_t>(cfg.wfGrid ? cfg.wfGrid : 1024u, r->ctx->cus)};
         for (uint32_t d = termDepth + 1; d < maxDepth; d++)
-            TRY(launch_wf_depth(S, F, P, W, d, true, cfg));
+            TRY(launch_wf_depth(S, F, P, WT, d, true, tail));
         return VH_OK;
     };
     // Training frames of this call (the warm-up on the cache's first call, then one per frame); the
     // paths of training frame j + 2 run beside the Adam steps and render of frame j. Timing events
-    // on the render stream: ev[0] at the start, then per training frame (after Adam, after render).
+    // on the render stream: ev[0] at the start, then per training frame three: before and after
+    // the render stream waits for the frame's Adam steps, and at the frame's end. The wait is the
+    // training time the render stream sees; the rest is render time.
     const uint32_t warm = nn->frames == 0 ? nn->cfg.warmup_frames : 0;
     const uint32_t total = warm + frames;
-    std::vector<hipEvent_t> ev(1 + 2 * (size_t)total, nullptr);
+    std::vector<hipEvent_t> ev(1 + 3 * (size_t)total, nullptr);
     for (auto& e : ev)
         HIPTRY(hipEventCreate(&e));
     int rc = check_hip(hipEventRecord(ev[0], s), "hipEventRecord");
@@ -1026,37 +1064,47 @@ int vh_nrc_accumulate(vh_nrc* nn, vh_accumulation* a, uint32_t frames)
     for (auto& t : nn->sets)
         if (rc == VH_OK)
             rc = check_hip(hipStreamWaitEvent(t.stream, ev[0], 0), "hipStreamWaitEvent");
+    if (rc == VH_OK)
+        rc = check_hip(hipStreamWaitEvent(nn->adamStream, ev[0], 0), "hipStreamWaitEvent");
     const uint32_t tf0 = nn->trainFrames;
     for (uint32_t j = 0; j < std::min(total, 2u) && rc == VH_OK; j++)
         rc = issuePaths(tf0 + j);
     for (uint32_t j = 0; j < total && rc == VH_OK; j++) {
         rc = adam(tf0 + j);
-        if (rc == VH_OK)
-            rc = check_hip(hipEventRecord(ev[1 + 2 * j], s), "hipEventRecord");
         if (rc == VH_OK && j + 2 < total)
             rc = issuePaths(tf0 + j + 2);
         if (rc == VH_OK && j >= warm)
-            rc = render(j - warm);
+            rc = renderPaths(j - warm);
         if (rc == VH_OK)
-            rc = check_hip(hipEventRecord(ev[2 + 2 * j], s), "hipEventRecord");
+            rc = check_hip(hipEventRecord(ev[1 + 3 * j], s), "hipEventRecord");
+        if (rc == VH_OK)
+            rc = check_hip(hipStreamWaitEvent(s, nn->adamDone[(tf0 + j) % vh_nrc::kBufs], 0), "hipStreamWaitEvent");
+        if (rc == VH_OK)
+            rc = check_hip(hipEventRecord(ev[2 + 3 * j], s), "hipEventRecord");
+        if (rc == VH_OK && j >= warm)
+            rc = renderCache(j - warm);
+        if (rc == VH_OK)
+            rc = check_hip(hipEventRecord(ev[3 + 3 * j], s), "hipEventRecord");
     }
-    if (rc == VH_OK && hipEventSynchronize(ev[2 * total]) != hipSuccess)
+    if (rc == VH_OK && hipEventSynchronize(ev[3 * total]) != hipSuccess)
         rc = fail(VH_ERR_KERNEL, "NRC frames failed on the device");
     if (rc != VH_OK) {
         for (auto& t : nn->sets)
             (void)hipStreamSynchronize(t.stream);
+        (void)hipStreamSynchronize(nn->adamStream);
         (void)hipStreamSynchronize(s);
     }
     float trainMs = 0.0f, renderMs = 0.0f;
     for (uint32_t j = 0; j < total && rc == VH_OK; j++) {
-        float t1 = 0.0f, t2 = 0.0f;
-        (void)hipEventElapsedTime(&t1, ev[2 * j], ev[1 + 2 * j]);
-        (void)hipEventElapsedTime(&t2, ev[1 + 2 * j], ev[2 + 2 * j]);
+        float t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
+        (void)hipEventElapsedTime(&t1, ev[3 * j], ev[1 + 3 * j]);
+        (void)hipEventElapsedTime(&t2, ev[1 + 3 * j], ev[2 + 3 * j]);
+        (void)hipEventElapsedTime(&t3, ev[2 + 3 * j], ev[3 + 3 * j]);
         if (j < warm)
-            trainMs += t1 + t2;
+            trainMs += t1 + t2 + t3;
         else {
-            trainMs += t1;
-            renderMs += t2;
+            trainMs += t2;
+            renderMs += t1 + t3;
         }
     }
     for (auto e : ev)
