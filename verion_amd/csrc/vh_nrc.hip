@@ -20,6 +20,7 @@
 #include "vh_internal.h"
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 namespace vh {
@@ -27,7 +28,6 @@ namespace vh {
 namespace {
 
 constexpr int kIn = 96, kHid = 64, kOut = 16, kNL = 6;
-constexpr int kRows = 64;   // rows per block tile (4 waves x 16)
 constexpr int kLdX = 104;   // padded leading dimension of 96-wide rows in LDS
 constexpr int kLdH = 72;    // padded leading dimension of 64-wide rows in LDS
 
@@ -97,20 +97,38 @@ __device__ void encodePart(const NrcNet& net, float4 q0, float4 q1, float4 q2, i
     }
 }
 
+// Forward weights in the inference's LDS: layer l as [n][layerK(l) + 8] (rows padded by 16 bytes so
+// the 16 rows an MFMA operand read touches fall in different banks).
+__host__ __device__ constexpr int wpLd(int l) { return layerK(l) + 8; }
+__host__ __device__ constexpr int wpOff(int l) { return l == 0 ? 0 : wpOff(l - 1) + layerN(l - 1) * wpLd(l - 1); }
+constexpr int kWp = wpOff(kNL); // 26240 bf16
+
 // MODE 0: out[3 q + c] = prediction of raw query q (3 float4 per record).
 // MODE 1: render: query records of the terminating depth (5 float4: inputs, pixel, throughput,
 //         total); film[pixel] += total + throughput * max(prediction, 0).
+// Each wave runs 16 queries at a time through the network on its own: out^T = W^T act^T per layer,
+// the weights as the MFMA A operand (LDS) and the activations, row-major per query, as B, so a
+// lane's result is 4 consecutive features of one query, stored with one 8-byte write. One block of
+// kInferWaves waves per CU shares one copy of the weights (52 KB of the CU's 160 KB LDS; the
+// waves' activations take the rest), for 4 waves per SIMD.
+constexpr int kInferWaves = 16;
 template <int MODE>
-__global__ __launch_bounds__(256) void k_nrc_infer(NrcNet net, const float4* rec, const uint32_t* countPtr, uint32_t countImm,
-                                                   float* out, double* film)
+__global__ __launch_bounds__(64 * kInferWaves) void k_nrc_infer(NrcNet net, const float4* rec, const uint32_t* countPtr,
+                                                                uint32_t countImm, float* out, double* film)
 {
-    __shared__ __attribute__((aligned(16))) __bf16 s_w[kWt];
+    __shared__ __attribute__((aligned(16))) __bf16 s_w[kWp];
     __shared__ float s_b[kBias];
-    __shared__ __attribute__((aligned(16))) __bf16 s_act[4][2][16 * kLdX];
-    for (int i = threadIdx.x; i < kWt / 8; i += 256)
-        reinterpret_cast<bf16x8_t*>(s_w)[i] = reinterpret_cast<const bf16x8_t*>(net.wt)[i];
+    __shared__ __attribute__((aligned(16))) __bf16 s_act[kInferWaves][2][16 * kLdX];
+    for (int l = 0; l < kNL; l++) {
+        const int K = layerK(l), N = layerN(l), chunks = K / 8;
+        for (int i = threadIdx.x; i < N * chunks; i += 64 * kInferWaves) {
+            const int nn = i / chunks, c = i % chunks;
+            *reinterpret_cast<bf16x8_t*>(s_w + wpOff(l) + nn * wpLd(l) + 8 * c) =
+                *reinterpret_cast<const bf16x8_t*>(net.wt + wtOff(l) + nn * K + 8 * c);
+        }
+    }
     for (int l = 0, o = 0; l < kNL; l++) {
-        for (int i = threadIdx.x; i < layerN(l); i += 256)
+        for (int i = threadIdx.x; i < layerN(l); i += 64 * kInferWaves)
             s_b[o + i] = net.theta[bOff(l) + i];
         o += layerN(l);
     }
@@ -118,8 +136,9 @@ __global__ __launch_bounds__(256) void k_nrc_infer(NrcNet net, const float4* rec
     const uint32_t n = countPtr ? *countPtr : countImm;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
     const int stride = MODE == 0 ? 3 : (int)kNrcQueryWords;
-    for (uint32_t tile = blockIdx.x * kRows; tile < n; tile += gridDim.x * kRows) {
-        uint32_t q = tile + wave * 16 + r;
+    // wave-uniform loop over this wave's 16-query tiles
+    for (uint32_t tile = (blockIdx.x * kInferWaves + wave) * 16u; tile < n; tile += gridDim.x * kInferWaves * 16u) {
+        const uint32_t q = tile + r; // this lane's query in the B operand and the result
         bool valid = q < n && (MODE == 0 || rec[(size_t)q * stride + 4].w != 0.0f);
         __bf16* a0 = s_act[wave][0] + r * kLdX + g * 24;
         if (valid) {
@@ -129,41 +148,39 @@ __global__ __launch_bounds__(256) void k_nrc_infer(NrcNet net, const float4* rec
             for (int k = 0; k < 24; k++)
                 a0[k] = (__bf16)0.0f;
         }
-        waveSync(); // a wave's rows are its own from the encoding to the output: no block barrier
+        waveSync(); // a wave's queries are its own from the encoding to the output: no block barrier
         int bo = 0;
 #pragma unroll
         for (int l = 0; l < kNL; l++) {
             const int K = layerK(l), N = layerN(l);
             const __bf16* in = s_act[wave][l & 1];
             __bf16* nx = s_act[wave][(l + 1) & 1];
-            const int ldin = kLdX;
+#pragma unroll
             for (int nt = 0; nt < N / 16; nt++) {
                 f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
-                c = mma16(in, ldin, s_w + wtOff(l) + nt * 16 * K, K, K, c);
-                int col = nt * 16 + r;
-                float bias = s_b[bo + col];
+                c = mma16(s_w + wpOff(l) + nt * 16 * wpLd(l), wpLd(l), in, kLdX, K, c);
+                const int f0 = nt * 16 + g * 4; // the lane's 4 features
                 if (l < kNL - 1) {
+                    bf16x4_t v;
 #pragma unroll
                     for (int t = 0; t < 4; t++)
-                        nx[(g * 4 + t) * kLdX + col] = (__bf16)fmaxf(c[t] + bias, 0.0f);
-                } else if (r < 3) {
+                        v[t] = (__bf16)fmaxf(c[t] + s_b[bo + f0 + t], 0.0f);
+                    *reinterpret_cast<bf16x4_t*>(nx + r * kLdX + f0) = v;
+                } else if (g == 0 && q < n) { // features 0-2: the RGB prediction
+                    if (MODE == 0) {
 #pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        uint32_t qq = tile + wave * 16 + g * 4 + t;
-                        float v = c[t] + bias;
-                        if (MODE == 0) {
-                            if (qq < n)
-                                out[(size_t)qq * 3 + r] = v;
-                        } else if (qq < n) {
-                            const float4* qr = rec + (size_t)qq * stride;
-                            float4 q4 = qr[4];
-                            if (q4.w != 0.0f) {
-                                uint32_t pixel = asu(qr[1].w);
-                                float4 thr = qr[3];
-                                float tv = r == 0 ? thr.x : (r == 1 ? thr.y : thr.z);
-                                float tot = r == 0 ? q4.x : (r == 1 ? q4.y : q4.z);
-                                film[3 * (size_t)pixel + r] += (double)(tot + tv * fmaxf(v, 0.0f));
-                            }
+                        for (int ch = 0; ch < 3; ch++)
+                            out[(size_t)q * 3 + ch] = c[ch] + s_b[bo + ch];
+                    } else {
+                        const float4* qr = rec + (size_t)q * stride;
+                        float4 q4 = qr[4];
+                        if (q4.w != 0.0f) {
+                            uint32_t pixel = asu(qr[1].w);
+                            float4 thr = qr[3];
+                            const float tv[3] = {thr.x, thr.y, thr.z}, tot[3] = {q4.x, q4.y, q4.z};
+#pragma unroll
+                            for (int ch = 0; ch < 3; ch++)
+                                film[3 * (size_t)pixel + ch] += (double)(tot[ch] + tv[ch] * fmaxf(c[ch] + s_b[bo + ch], 0.0f));
                         }
                     }
                 }
@@ -395,23 +412,30 @@ __global__ __launch_bounds__(256, 2) void k_nrc_train(NrcNet net, const float4* 
 // group order.
 constexpr uint32_t kReduceGroup = 32;
 
+// Four parameters per thread (16-byte loads; kParams is a multiple of 4), each summed in block order.
+static_assert(kParams % 4 == 0, "float4 rows of the partials");
 __global__ void k_nrc_reduce_groups(const float* partials, uint32_t blocks, float* groupSums)
 {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t g = blockIdx.y;
-    if (p >= (uint32_t)kParams)
+    const uint32_t p4 = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t g = blockIdx.y;
+    if (p4 >= (uint32_t)kParams / 4)
         return;
     const uint32_t b0 = g * kReduceGroup;
-    float part[kReduceGroup];
+    const float4* src = reinterpret_cast<const float4*>(partials);
+    float4 part[kReduceGroup];
 #pragma unroll
     for (uint32_t k = 0; k < kReduceGroup; k++) // every load in flight at once
-        part[k] = b0 + k < blocks ? partials[(size_t)(b0 + k) * kParams + p] : 0.0f;
-    float s = 0.0f;
+        part[k] = b0 + k < blocks ? src[(size_t)(b0 + k) * (kParams / 4) + p4] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
     for (uint32_t k = 0; k < kReduceGroup; k++)
-        if (b0 + k < blocks)
-            s += part[k];
-    groupSums[(size_t)g * kParams + p] = s;
+        if (b0 + k < blocks) {
+            s.x += part[k].x;
+            s.y += part[k].y;
+            s.z += part[k].z;
+            s.w += part[k].w;
+        }
+    reinterpret_cast<float4*>(groupSums)[(size_t)g * (kParams / 4) + p4] = s;
 }
 
 // Parameter p -> its bf16 copies (forward [n][k] and, when wb is given, backward [k][npad]);
@@ -654,7 +678,7 @@ int train_steps(vh_nrc* nn, const float4* samples, const uint32_t* countPtr, uin
         const uint32_t slice = slices > 1 ? k % slices : 0u, nsl = slices > 1 ? slices : 1u;
         k_nrc_train<<<kTrainBlocks, 256, 0, s>>>(net, samples, countPtr, countImm, cap, slice, nsl, nn->partials.as<float>(),
                                                  nn->lossPart.as<float>());
-        k_nrc_reduce_groups<<<dim3((kParams + 255) / 256, kReduceGroups), 256, 0, s>>>(nn->partials.as<float>(), kTrainBlocks,
+        k_nrc_reduce_groups<<<dim3((kParams / 4 + 255) / 256, kReduceGroups), 256, 0, s>>>(nn->partials.as<float>(), kTrainBlocks,
                                                                                        nn->groupSums.as<float>());
         AdamStep st{nn->cfg.learning_rate, 0.9f, 0.99f, 1.0f, 1.0f, 1.0e-8f};
         if (adam) {
@@ -880,8 +904,8 @@ int vh_nrc_infer(vh_nrc* nn, uint64_t n, const float* raw12, float* out3)
     HIPTRY(in.alloc(4 * host.size()));
     HIPTRY(out.alloc(12 * n));
     HIPTRY(hipMemcpyAsync(in.p, host.data(), 4 * host.size(), hipMemcpyHostToDevice, s));
-    uint32_t grid = (uint32_t)std::min<uint64_t>((n + kRows - 1) / kRows, 4096);
-    k_nrc_infer<0><<<grid, 256, 0, s>>>(nn->net(true, true), in.as<float4>(), nullptr, (uint32_t)n, out.as<float>(), nullptr);
+    uint32_t grid = (uint32_t)std::min<uint64_t>((n + 16 * kInferWaves - 1) / (16 * kInferWaves), nn->r->ctx->cus);
+    k_nrc_infer<0><<<grid, 64 * kInferWaves, 0, s>>>(nn->net(true, true), in.as<float4>(), nullptr, (uint32_t)n, out.as<float>(), nullptr);
     HIPTRY(hipGetLastError());
     HIPTRY(hipStreamSynchronize(s));
     HIPTRY(hipMemcpy(out3, out.p, 12 * n, hipMemcpyDeviceToHost));
@@ -1033,8 +1057,8 @@ int vh_nrc_accumulate(vh_nrc* nn, vh_accumulation* a, uint32_t frames)
         P.w[6] = a->samples + f;
         P.w[31] = (uint32_t)npix;
         const NrcNet net = nn->net(false, true);
-        uint32_t grid = std::min<uint32_t>((uint32_t)((npix + kRows - 1) / kRows), r->ctx->cus * 2);
-        k_nrc_infer<1><<<grid, 256, 0, s>>>(net, nn->queries.as<float4>(), W.counts + termDepth, 0u, nullptr, F.film);
+        uint32_t grid = std::min<uint32_t>((uint32_t)((npix + 16 * kInferWaves - 1) / (16 * kInferWaves)), r->ctx->cus);
+        k_nrc_infer<1><<<grid, 64 * kInferWaves, 0, s>>>(net, nn->queries.as<float4>(), W.counts + termDepth, 0u, nullptr, F.film);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess)
             return fail(VH_ERR_KERNEL, std::string("NRC inference: ") + hipGetErrorString(e));
