@@ -199,6 +199,12 @@ int vh_set_wavefront_blocks(vh_renderer* r, uint32_t blocks);
  * (8 x 8 by default), or 0 x 0 for scanlines. No counterpart in the reference (it dispatches
  * pixels in 64-wide groups of a row, wavefront.rs:11). Images do not depend on it. */
 int vh_set_wavefront_tiles(vh_renderer* r, uint32_t tile_width, uint32_t tile_height);
+/* Sample passes of one vh_accumulate call that the plain wavefront (no ReSTIR, lighting split or
+ * adaptive sampling, stage timing off) keeps in flight at once, 1 to 4 (default 3; 1 = one pass at
+ * a time). Each pass in flight has its own stream, wavefront buffers and FP64 film, added to the
+ * accumulation's film in sample order, so the film's bits do not depend on it. No counterpart in
+ * the reference (its wavefront records one pass after another into one submission, vulkan renderer.rs:681-760). */
+int vh_set_wavefront_pipeline(vh_renderer* r, uint32_t passes);
 
 /* --- accumulation (new_accumulation / accumulate, lib.rs:47-52; trait Accumulation :74-93) ---- */
 

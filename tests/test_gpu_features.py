@@ -275,6 +275,42 @@ def test_depth0_order_and_grid_do_not_change_the_image(ctx, adaptive):
             assert np.array_equal(s, films[0][1])
 
 
+@pytest.mark.parametrize("scene", ["zoo", "cornell", "spheres", "lights"])
+def test_passes_in_flight_do_not_change_the_film(ctx, scene):
+    # 1 to 4 sample passes in flight, over calls of 3 and 4 samples: the same film bit for bit (each
+    # pass's film is added in sample order), on a translucent scene (separate extend and shadow
+    # launches), opaque ones (merged trace launches) and one that shades its shadow rays fused
+    from verion_amd import scene as sc
+
+    if scene == "lights":
+        s = scenes.floor_scene(sc.PointLight(position=(0.0, 0.0, 2.0), color=(1.0, 1.0, 1.0), intensity=2.0), occluder=True)
+        for k in range(8):
+            s.add_light(sc.PointLight(position=(0.3 * k - 1.0, 0.5, 1.5), color=(0.5, 0.6, 0.7), intensity=1.0))
+    else:
+        s = {"zoo": scenes.zoo, "cornell": scenes.cornell, "spheres": lambda: scenes.procedural(spheres=16)}[scene]()
+    r = Renderer(ctx, s)
+    r.set_integrator(WAVEFRONT)
+    films = []
+    for passes in (1, 2, 3, 4):
+        r.set_wavefront_pipeline(passes)
+        acc = r.new_accumulation(40, 28)
+        r.accumulate(acc, 3)
+        r.accumulate(acc, 4)
+        assert acc.samples() == 7
+        films.append(acc.film())
+        acc.close()
+    r.set_wavefront_pipeline(3)
+    for f in films[1:]:
+        assert np.array_equal(f, films[0])
+    want = oracle.OracleScene(r.packed).render(params(r.packed, 40, 28), 7)["film"]
+    assert rel(films[-1], want) <= TOL
+    with pytest.raises(BackendError) as e:
+        r.set_wavefront_pipeline(0)
+    assert e.value.status == INVALID_ARGUMENT
+    with pytest.raises(BackendError):
+        r.set_wavefront_pipeline(5)
+
+
 def test_many_delta_lights_shade_fused_and_match_the_oracle(ctx):
     # more than 7 delta lights: the wavefront shades with its shadow rays inline (no deferred records)
     from verion_amd import scene as sc
@@ -287,7 +323,8 @@ def test_many_delta_lights_shade_fused_and_match_the_oracle(ctx):
     r.set_integrator(WAVEFRONT)
     acc = r.new_accumulation(24, 24)
     r.accumulate(acc, 2)
-    assert acc.wavefront_bytes() == 2 * 4 * 17 * 576 + 16 * 576 + 4 * 576 + 128  # no shadow records
+    # no shadow records; the 2 samples run as 2 passes in flight, each with its buffers and FP64 film
+    assert acc.wavefront_bytes() == 2 * (2 * 4 * 17 * 576 + 16 * 576 + 4 * 576 + 128 + 24 * 576)
     st = oracle.OracleScene(r.packed).render(params(r.packed, 24, 24), 2)
     assert np.array_equal(acc.film(), st["film"])
     # the persistent megakernel likewise traces them in the shading (its non-deferred form)

@@ -95,6 +95,7 @@ def library() -> C.CDLL:
         "vh_set_seed": (i32, [vp, u32]),
         "vh_set_wavefront_blocks": (i32, [vp, u32]),
         "vh_set_wavefront_tiles": (i32, [vp, u32, u32]),
+        "vh_set_wavefront_pipeline": (i32, [vp, u32]),
         "vh_accumulation_create": (i32, [vp, u32, u32, i32, C.POINTER(vp)]),
         "vh_accumulation_destroy": (None, [vp]),
         "vh_accumulate": (i32, [vp, vp, u32]),
@@ -340,6 +341,10 @@ class Renderer(_Owned):
     def set_wavefront_tiles(self, tile_width: int, tile_height: int):
         """Depth-0 pixel order of the wavefront: tiles, or (0, 0) for scanlines."""
         _check(library().vh_set_wavefront_tiles(self.h, tile_width, tile_height))
+
+    def set_wavefront_pipeline(self, passes: int):
+        """Sample passes of one accumulate call the plain wavefront keeps in flight (1 to 4)."""
+        _check(library().vh_set_wavefront_pipeline(self.h, passes))
 
     def new_accumulation(self, width: int, height: int, lighting: bool = False) -> Accumulation:
         return Accumulation(self, width, height, lighting)

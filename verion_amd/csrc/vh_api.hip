@@ -327,9 +327,10 @@ int upload_tables(vh_renderer* r, const vh_scene_tables* t)
     return VH_OK;
 }
 
-int ensure_wavefront(vh_accumulation* a)
+int ensure_wavefront(vh_accumulation* a, uint32_t j)
 {
-    if (a->wfReady)
+    WfBuffers& b = a->wf[j];
+    if (b.ready)
         return VH_OK;
     uint64_t n = a->pixels();
     uint32_t words = 17 + (a->lighting ? 17 : 0);
@@ -341,24 +342,47 @@ int ensure_wavefront(vh_accumulation* a)
     uint64_t shBytes = defer ? (4ull * (kShadowWords + 2) * K * n + 4ull * 6 * n) : 0;
     uint64_t bytes = 2 * 4 * words * n + 16 * n + 4 * n + 4 * kCountWords + shBytes;
     TRY(check_fits(bytes));
-    HIPTRY(a->state0.alloc(4 * words * n));
-    HIPTRY(a->state1.alloc(4 * words * n));
-    HIPTRY(a->hits.alloc(16 * n));
-    HIPTRY(a->queue0.alloc(4 * n));
-    HIPTRY(a->counts.alloc(4 * kCountWords));
-    HIPTRY(hipMemsetAsync(a->counts.p, 0, 4 * kCountWords, a->r->ctx->stream));
+    HIPTRY(b.state0.alloc(4 * words * n));
+    HIPTRY(b.state1.alloc(4 * words * n));
+    HIPTRY(b.hits.alloc(16 * n));
+    HIPTRY(b.queue0.alloc(4 * n));
+    HIPTRY(b.counts.alloc(4 * kCountWords));
+    HIPTRY(hipMemsetAsync(b.counts.p, 0, 4 * kCountWords, a->r->ctx->stream));
     if (defer) {
-        HIPTRY(a->shRec.alloc(4ull * kShadowWords * K * n));
-        HIPTRY(a->shVis.alloc(4ull * K * n));
-        HIPTRY(a->shQueue.alloc(4ull * K * n));
-        HIPTRY(a->shNRec.alloc(4 * n));
-        HIPTRY(a->shOut.alloc(4 * n));
-        HIPTRY(a->shPixel.alloc(4 * n));
-        HIPTRY(a->shTotal.alloc(12 * n));
-        a->shK = K;
+        HIPTRY(b.shRec.alloc(4ull * kShadowWords * K * n));
+        HIPTRY(b.shVis.alloc(4ull * K * n));
+        HIPTRY(b.shQueue.alloc(4ull * K * n));
+        HIPTRY(b.shNRec.alloc(4 * n));
+        HIPTRY(b.shOut.alloc(4 * n));
+        HIPTRY(b.shPixel.alloc(4 * n));
+        HIPTRY(b.shTotal.alloc(12 * n));
+        b.shK = K;
     }
-    a->wfBytes = bytes;
-    a->wfReady = true;
+    b.bytes = bytes;
+    b.ready = true;
+    return VH_OK;
+}
+
+// Passes in flight j = 1 .. depth - 1 beside wf[0]: their wavefront buffers, the FP64 films of all
+// `depth` passes (zeroed once; the film add zeroes them again), the streams and the events.
+int ensure_pipeline(vh_accumulation* a, uint32_t depth)
+{
+    vh_context* ctx = a->r->ctx;
+    const uint64_t filmBytes = 24 * a->pixels();
+    for (uint32_t j = 0; j < depth; j++) {
+        if (j > 0)
+            TRY(ensure_wavefront(a, j));
+        if (!a->pipeFilm[j].p) {
+            TRY(check_fits(filmBytes));
+            HIPTRY(a->pipeFilm[j].alloc(filmBytes));
+            HIPTRY(hipMemsetAsync(a->pipeFilm[j].p, 0, filmBytes, ctx->stream));
+        }
+        if (!ctx->pipe[j])
+            HIPTRY(hipStreamCreateWithFlags(&ctx->pipe[j], hipStreamNonBlocking));
+    }
+    for (hipEvent_t& e : a->pipeEv)
+        if (!e)
+            HIPTRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return VH_OK;
 }
 
@@ -424,6 +448,9 @@ void vh_context_destroy(vh_context* ctx)
         return;
     if (ctx->stream)
         (void)hipStreamDestroy(ctx->stream);
+    for (hipStream_t p : ctx->pipe)
+        if (p)
+            (void)hipStreamDestroy(p);
     delete ctx;
 }
 
@@ -632,6 +659,14 @@ int vh_set_wavefront_blocks(vh_renderer* r, uint32_t blocks)
     return VH_OK;
 }
 
+int vh_set_wavefront_pipeline(vh_renderer* r, uint32_t passes)
+{
+    if (!r || passes < 1 || passes > kMaxPipeline)
+        return fail(VH_ERR_INVALID_ARGUMENT, "passes in flight must be 1 to " + std::to_string(kMaxPipeline));
+    r->pipeline = passes;
+    return VH_OK;
+}
+
 int vh_set_wavefront_tiles(vh_renderer* r, uint32_t tile_width, uint32_t tile_height)
 {
     if (!r)
@@ -731,7 +766,42 @@ int vh_accumulate(vh_renderer* r, vh_accumulation* a, uint32_t samples)
         HIPTRY(marks.reserve((size_t)samples * (4 * kMaxDepth + 4)));
     int rc = VH_OK;
     HIPTRY(hipEventRecord(e0, s));
-    if (wave) {
+    // the plain wavefront keeps several sample passes in flight (without stage timing, which
+    // brackets launches on one stream)
+    const uint32_t depth = std::min(r->pipeline, samples);
+    const bool piped = wave && depth > 1 && !restir && !split && !a->adaptiveOn && !timing;
+    if (piped)
+        TRY(ensure_pipeline(a, depth));
+    if (piped) {
+        // pass k runs on stream pipe[k % depth] into film j = k % depth; the library stream adds
+        // film j to the accumulation's film once pass k is done, in sample order, and pass k + depth
+        // waits for that add before it reuses buffers and film j
+        hipEvent_t fork = a->pipeEv[2 * kMaxPipeline];
+        HIPTRY(hipEventRecord(fork, s));
+        for (uint32_t j = 0; j < depth; j++)
+            HIPTRY(hipStreamWaitEvent(r->ctx->pipe[j], fork, 0));
+        for (uint32_t k = 0; k < samples && rc == VH_OK; k++) {
+            const uint32_t j = k % depth;
+            hipStream_t ps = r->ctx->pipe[j];
+            hipEvent_t done = a->pipeEv[j], added = a->pipeEv[kMaxPipeline + j];
+            if (k >= depth)
+                HIPTRY(hipStreamWaitEvent(ps, added, 0));
+            Frame Fj = F;
+            Fj.film = a->pipeFilm[j].as<double>();
+            Params P = base;
+            P.w[6] = a->samples + k;
+            P.w[31] = (uint32_t)n;
+            LaunchCfg cfg{ps, r->wfBlocks ? r->wfBlocks : std::max(256u, r->ctx->cus * 8)};
+            rc = launch_wf_sample(S, Fj, P, a->wavefront(false, j), false, false, cfg, nullptr);
+            if (rc != VH_OK)
+                break;
+            HIPTRY(hipEventRecord(done, ps));
+            HIPTRY(hipStreamWaitEvent(s, done, 0));
+            rc = launch_film_add_pass(F.film, Fj.film, 3 * n, s);
+            if (rc == VH_OK)
+                HIPTRY(hipEventRecord(added, s));
+        }
+    } else if (wave) {
         Wavefront W = a->wavefront(split);
         LaunchCfg cfg{s, r->wfBlocks ? r->wfBlocks : std::max(256u, r->ctx->cus * 8)};
         for (uint32_t k = 0; k < samples && rc == VH_OK; k++) {
@@ -835,7 +905,7 @@ int vh_acc_kernel_seconds(const vh_accumulation* a, double out[5])
     return VH_OK;
 }
 
-uint64_t vh_acc_wavefront_bytes(const vh_accumulation* a) { return a ? a->wfBytes : 0; }
+uint64_t vh_acc_wavefront_bytes(const vh_accumulation* a) { return a ? a->wavefrontBytes() : 0; }
 uint64_t vh_acc_restir_bytes(const vh_accumulation* a) { return a && a->restirOn ? 4ull * kRestirStride * a->pixels() : 0; }
 
 int vh_acc_read_film(const vh_accumulation* a, double* out)

@@ -118,6 +118,20 @@ struct Wavefront {
     float* shTotal = nullptr;    // 3 x n: path total before its shadow contributions
 };
 
+// The device buffers behind one Wavefront: the state of one sample pass in flight.
+struct WfBuffers {
+    DevBuf state0, state1, hits, queue0, counts;
+    DevBuf shRec, shVis, shQueue, shNRec, shOut, shPixel, shTotal; // deferred shadow rays
+    uint32_t shK = 0;
+    bool ready = false;
+    uint64_t bytes = 0;
+};
+// Sample passes of one accumulate call that the plain wavefront keeps in flight at once, each on
+// its own stream with its own wavefront buffers and FP64 film, added to the accumulation's film in
+// sample order (vh_set_wavefront_pipeline). A pass's deepest depths hold a few thousand paths
+// whose launches last as long as their slowest chain of dependent fetches; the next passes fill
+// the GPU meanwhile.
+constexpr uint32_t kMaxPipeline = 4;
 // counts[] layout (kCountWords words): live paths per depth at 0..kMaxDepth, shadow records per
 // depth at kShadowCounts + d
 constexpr uint32_t kShadowCounts = 16;
@@ -201,6 +215,10 @@ int launch_megakernel_samples(const Scene& S, const Frame& F, const Params& P, u
 constexpr uint64_t kParallelPaths = 1u << 21;
 int launch_wf_sample(const Scene& S, const Frame& F, const Params& P, const Wavefront& W, bool restir, bool split, const LaunchCfg& cfg,
                      StageMarks* marks);
+// film[i] += pass[i] and pass[i] = 0 over n doubles: a pass in flight's FP64 film (each pixel's
+// 0 + radiance, exact) added in the order the passes were issued, which gives the film's bits of
+// passes run one at a time.
+int launch_film_add_pass(double* film, double* pass, uint64_t n, hipStream_t s);
 int launch_primary_hits(const Scene& S, const Params& P, uint4* out, hipStream_t s);
 int launch_nrc_train_paths(const Scene& S, const Params& P, const Wavefront& W, uint32_t* trainPix, uint32_t nTrain, uint32_t npix,
                            uint32_t frame, float4* vertices, float4* samples, uint32_t* sampleCount, uint32_t sampleCap,
@@ -228,6 +246,7 @@ int launch_aovs(const Scene& S, const Params& P, float4* albedoDepth, uint32_t* 
 struct vh_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t pipe[vh::kMaxPipeline] = {}; // the wavefront's passes in flight (created on first use)
     uint32_t cus = 0;
     uint64_t uploaded = 0;
 };
@@ -253,6 +272,7 @@ struct vh_renderer {
     uint32_t seed = 0;
     uint32_t wfBlocks = 0;
     uint32_t tileW = 8, tileH = 8; // depth-0 pixel order of the wavefront
+    uint32_t pipeline = 3;         // plain wavefront: sample passes in flight (1 = one at a time)
     vh::DevBuf counters; // traversal counter banks while counting is on
     // what a partial update must keep: the table lengths of positions, indices, uvs, vertex
     // normals, instances and meshes; hashes of the texture and environment tables
@@ -292,13 +312,14 @@ struct vh_accumulation {
     vh_renderer* r = nullptr;
     uint32_t width = 0, height = 0;
     bool lighting = false;
-    vh::DevBuf film, stats, light, restirBuf, state0, state1, hits, queue0, counts;
-    vh::DevBuf shRec, shVis, shQueue, shNRec, shOut, shPixel, shTotal; // deferred shadow rays
+    vh::DevBuf film, stats, light, restirBuf;
+    vh::WfBuffers wf[vh::kMaxPipeline];      // wf[0]: every wavefront pass; wf[1..]: passes in flight beside it
+    vh::DevBuf pipeFilm[vh::kMaxPipeline];   // FP64 film of each pass in flight, zero between passes
+    hipEvent_t pipeEv[2 * vh::kMaxPipeline + 1] = {}; // pass j done, film j added, and the fork
     vh::DevBuf staged; // the megakernel's per-sample radiance when samples run in parallel
     vh::DevBuf persistRec; // the persistent megakernel's shadow records (per resident lane)
     vh::StageMarks marks;  // stage-timing events, reused across calls
     hipEvent_t span[2] = {nullptr, nullptr}; // start and end of a call
-    uint32_t shK = 0;
     uint32_t samples = 0;
     double gpuSeconds = 0.0;
     double stage[vh::kStages] = {}; // GPU seconds per vh::Stage
@@ -307,12 +328,21 @@ struct vh_accumulation {
     vh_adaptive adaptive{};
     bool restirOn = false;
     vh_restir restir{};
-    bool wfReady = false;
-    uint64_t wfBytes = 0;
     uint64_t pixels() const { return (uint64_t)width * height; }
+    // device bytes of the wavefront passes: every set of buffers allocated, and the films of passes in flight
+    uint64_t wavefrontBytes() const
+    {
+        uint64_t b = 0;
+        for (uint32_t j = 0; j < vh::kMaxPipeline; j++)
+            b += (wf[j].ready ? wf[j].bytes : 0) + pipeFilm[j].bytes;
+        return b;
+    }
     ~vh_accumulation()
     {
         for (hipEvent_t e : span)
+            if (e)
+                (void)hipEventDestroy(e);
+        for (hipEvent_t e : pipeEv)
             if (e)
                 (void)hipEventDestroy(e);
     }
@@ -320,28 +350,29 @@ struct vh_accumulation {
     // sample, and ReSTIR DI's winner
     uint32_t shadowRecords() const { return r->info.light_count + 1 + (restirOn && restir.di ? 1u : 0u); }
     uint32_t traceWaves() const { return r->ctx->cus * vh::kTraceWavesPerCu; }
-    vh::Wavefront wavefront(bool split) const
+    vh::Wavefront wavefront(bool split, uint32_t j = 0) const
     {
+        const vh::WfBuffers& b = wf[j];
         vh::Wavefront W;
-        W.state[0] = state0.as<uint32_t>();
-        W.state[1] = state1.as<uint32_t>();
-        W.hits = hits.as<uint4>();
-        W.queue0 = queue0.as<uint32_t>();
-        W.counts = counts.as<uint32_t>();
+        W.state[0] = b.state0.as<uint32_t>();
+        W.state[1] = b.state1.as<uint32_t>();
+        W.hits = b.hits.as<uint4>();
+        W.queue0 = b.queue0.as<uint32_t>();
+        W.counts = b.counts.as<uint32_t>();
         W.n = (uint32_t)pixels();
         W.words = 17 + (split ? 17 : 0);
         W.tileW = r->tileW;
         W.tileH = r->tileH;
         W.traceWaves = traceWaves();
-        if (!split && shK > 0 && shadowRecords() <= shK) { // (a scene update may add lights)
-            W.shK = shK;
-            W.shRec = shRec.as<float>();
-            W.shVis = shVis.as<uint32_t>();
-            W.shQueue = shQueue.as<uint32_t>();
-            W.shNRec = shNRec.as<uint32_t>();
-            W.shOut = shOut.as<uint32_t>();
-            W.shPixel = shPixel.as<uint32_t>();
-            W.shTotal = shTotal.as<float>();
+        if (!split && b.shK > 0 && shadowRecords() <= b.shK) { // (a scene update may add lights)
+            W.shK = b.shK;
+            W.shRec = b.shRec.as<float>();
+            W.shVis = b.shVis.as<uint32_t>();
+            W.shQueue = b.shQueue.as<uint32_t>();
+            W.shNRec = b.shNRec.as<uint32_t>();
+            W.shOut = b.shOut.as<uint32_t>();
+            W.shPixel = b.shPixel.as<uint32_t>();
+            W.shTotal = b.shTotal.as<float>();
         }
         return W;
     }
@@ -351,7 +382,7 @@ namespace vh {
 // shared host helpers (vh_api.hip)
 int check_hip(hipError_t e, const char* what);
 int check_fits(uint64_t needed);
-int ensure_wavefront(vh_accumulation* a);
+int ensure_wavefront(vh_accumulation* a, uint32_t j = 0);
 Params base_params(const vh_renderer* r, uint32_t w, uint32_t h, bool lighting, bool adaptiveOn, const vh_adaptive& ad, bool restirOn,
                    const vh_restir& re);
 } // namespace vh

@@ -194,6 +194,25 @@ __global__ __launch_bounds__(256) void k_film_add_samples(double* __restrict__ f
     }
 }
 
+// film += pass, pass = 0, two doubles per thread and step (both buffers are 256-byte aligned)
+__global__ __launch_bounds__(256) void k_film_add_pass(double* __restrict__ film, double* __restrict__ pass, uint64_t n)
+{
+    const uint64_t pairs = n / 2;
+    double2* f2 = reinterpret_cast<double2*>(film);
+    double2* p2 = reinterpret_cast<double2*>(pass);
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < pairs; i += (uint64_t)gridDim.x * blockDim.x) {
+        double2 a = f2[i], b = p2[i];
+        a.x += b.x;
+        a.y += b.y;
+        f2[i] = a;
+        p2[i] = make_double2(0.0, 0.0);
+    }
+    if ((n & 1u) != 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+        film[n - 1] += pass[n - 1];
+        pass[n - 1] = 0.0;
+    }
+}
+
 // wf_args at depth 0 (wf_args.slang:12-17, and its clear of counts[0] at :21-22): the depth-0
 // count (every pixel, or none yet when wf_select fills the queue) and empty later queues
 __global__ void k_wf_reset(uint32_t* counts, uint32_t n, uint32_t adaptive, uint32_t maxDepth)
@@ -1007,6 +1026,13 @@ int launch_wf_sample(const Scene& S, const Frame& F, const Params& P0, const Wav
         return fail(VH_ERR_KERNEL, std::string("stage timing: ") + hipGetErrorString(me));
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("wavefront launch: ") + hipGetErrorString(e));
+}
+
+int launch_film_add_pass(double* film, double* pass, uint64_t n, hipStream_t s)
+{
+    k_film_add_pass<<<blocks_for(n / 2 + 1, 256, 2048), 256, 0, s>>>(film, pass, n);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("film add: ") + hipGetErrorString(e));
 }
 
 int launch_nrc_train_paths(const Scene& S, const Params& P, const Wavefront& W, uint32_t* trainPix, uint32_t nTrain, uint32_t npix,
