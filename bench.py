@@ -164,6 +164,7 @@ def main():
     r = Renderer(ctx, packed)
     log(f"scene + LBVH ready in {time.perf_counter() - t0:.2f} s: {r.bvh_stats()}")
     r.set_integrator(WAVEFRONT if args.integrator == "wavefront" else MEGAKERNEL)
+    r.set_wavefront_pipeline(args.pipeline)
 
     spp = args.spp_per_step
     warm = r.new_accumulation(W, H)

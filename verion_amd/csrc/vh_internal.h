@@ -208,7 +208,10 @@ constexpr uint32_t kPersistWavesPerCu = 16; // 4 per SIMD, the kernel's occupanc
 // Waves per CU a trace kernel spreads a short queue over (Wavefront::traceWaves): at 1080p on the
 // procedural scene 8, 16 and 20 per CU trace within 1% of each other, 4 per CU and 32 per CU are
 // 2% and 7% slower, and 64 rays to every wave (no spreading) 10% slower.
-constexpr uint32_t kTraceWavesPerCu = 16;
+#ifndef VH_TRACE_WAVES_PER_CU
+#define VH_TRACE_WAVES_PER_CU 16
+#endif
+constexpr uint32_t kTraceWavesPerCu = VH_TRACE_WAVES_PER_CU;
 int launch_megakernel_samples(const Scene& S, const Frame& F, const Params& P, uint32_t samples, float* staged, uint32_t* next,
                               float* shRec, uint32_t shK, uint32_t cus, hipStream_t s);
 // Paths in flight that fill the GPU: smaller images take several samples per dispatch in parallel.

@@ -191,30 +191,89 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_nrc_infer(NrcNet net, cons
     }
 }
 
-// The training kernel works on 32-row tiles (two 16-row MFMA tiles): its activations, their
-// transposed copies and the gradients of one tile take 71 KB of LDS, so two blocks share a CU.
+// The training kernel works on 32-row tiles (two 16-row MFMA tiles) with 8 waves per block: each
+// wave computes one 16 x 16 tile of every product of a layer, so a layer's chain of dependent MFMAs
+// is one tile long. Every wave keeps the weight operands of its tiles (forward and backward, 88
+// VGPRs) in registers for the whole launch: no global load waits inside the tile loop. The
+// activations, their transposed copies and the gradients of one tile take 71 KB of LDS; one block
+// per CU (2 waves per SIMD, 256 registers each).
 constexpr int kTrainRows = 32;
+constexpr int kTrainWaves = 8;
 constexpr int kLdS = 40; // padded leading dimension of transposed training tiles (32 samples) in LDS
 
-// One layer of the backward pass for a 32-row tile: dW (and db) accumulate into this wave's tiles,
-// then (l > 0) dZ of the layer below = (dZ W^T) * relu'(input).
-template <int L, int NT>
-__device__ __forceinline__ void backLayer(const NrcNet& net, const __bf16* hinT, const __bf16* dz, const __bf16* dzT, __bf16* dzNext,
-                                          __bf16* dzNextT, f32x4_t (&acc)[NT], float& db)
+// c += A B over K (A: 16 rows at a with leading dimension lda, read from LDS; B: this lane's operand
+// fragments of KS 32-deep steps, held in registers)
+template <int KS>
+__device__ __forceinline__ f32x4_t mmaReg(const __bf16* a, int lda, const bf16x8_t (&b)[KS], f32x4_t c)
 {
-    // Layer L's backward over one tile. hinT: the layer's input transposed ([k][row]); dz and dzT:
-    // its output gradient, row-major and transposed; writes the input's gradient (through the ReLU
-    // of the layer below) in both layouts. The forward pass stores every activation transposed as
-    // well, so the weight gradient hinT x dz reads MFMA operands straight from LDS: one 32-deep
-    // MFMA per 16 x 16 output tile.
-    constexpr int K = layerK(L), N = layerN(L), NP = layerNpad(L);
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const __bf16* pa = a + r * lda + g * 8;
+#pragma unroll
+    for (int k = 0; k < KS; k++)
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8_t*>(pa + 32 * k), b[k], c, 0, 0, 0);
+    return c;
+}
+
+// this lane's B fragments of the column tile of a matrix stored [col][k] at b (leading dimension ldb)
+template <int KS>
+__device__ __forceinline__ void loadFrag(const __bf16* b, int ldb, bf16x8_t (&f)[KS])
+{
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int k = 0; k < KS; k++)
+        f[k] = *reinterpret_cast<const bf16x8_t*>(b + r * ldb + g * 8 + 32 * k);
+}
+
+// Forward layer L of a tile: wave w computes row tile w & 1, column tile w >> 1 (waves past the
+// layer's column tiles idle); ReLU outputs go row-major to hout and transposed to hT, the output
+// layer's RGB to y.
+template <int L, int KS>
+__device__ __forceinline__ void fwdLayer(const __bf16* in, int ldin, const bf16x8_t (&w)[KS], float bias, __bf16* hout, __bf16* hT,
+                                         float (*y)[4])
+{
+    constexpr int N = layerN(L);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
-    constexpr int NTILE = N / 16;
+    const int rt = wave & 1, nt = wave >> 1;
+    if (nt < N / 16) {
+        f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
+        c = mmaReg<KS>(in + rt * 16 * ldin, ldin, w, c);
+        const int col = nt * 16 + r;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int rw = rt * 16 + g * 4 + t;
+            if (L < kNL - 1) {
+                __bf16 v = (__bf16)fmaxf(c[t] + bias, 0.0f);
+                hout[rw * kLdH + col] = v;
+                hT[col * kLdS + rw] = v;
+            } else if (col < 3) {
+                y[rw][col] = c[t] + bias;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// One layer of the backward pass for a tile: dW (and db) accumulate into this wave's tiles
+// (t = wave + 8 j), then (L > 0) dZ of the layer below = (dZ W^T) * relu'(input), one tile per wave
+// (row tile w & 1, column tile w >> 1) with W^T's fragments from registers. hinT: the layer's input
+// transposed ([k][row]); dz and dzT: its output gradient, row-major and transposed; writes the
+// input's gradient in both layouts. The forward pass stores every activation transposed as well,
+// so the weight gradient hinT x dz reads MFMA operands straight from LDS: one 32-deep MFMA per
+// 16 x 16 output tile.
+template <int L, int NT, int KS>
+__device__ __forceinline__ void backLayer(const __bf16* hinT, const __bf16* dz, const __bf16* dzT, __bf16* dzNext, __bf16* dzNextT,
+                                          const bf16x8_t (&wT)[KS], f32x4_t (&acc)[NT], float& db)
+{
+    constexpr int K = layerK(L), N = layerN(L);
+    constexpr int NTILE = N / 16, TILES = (K / 16) * NTILE;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
 #pragma unroll
     for (int j = 0; j < NT; j++) {
-        int t = wave + 4 * j;
-        int kt = t / NTILE, nt = t % NTILE;
-        acc[j] = mma16(hinT + kt * 16 * kLdS, kLdS, dzT + nt * 16 * kLdS, kLdS, kTrainRows, acc[j]);
+        const int t = wave + kTrainWaves * j;
+        if (t < TILES) {
+            const int kt = t / NTILE, nt = t % NTILE;
+            acc[j] = mma16(hinT + kt * 16 * kLdS, kLdS, dzT + nt * 16 * kLdS, kLdS, kTrainRows, acc[j]);
+        }
     }
     if ((int)threadIdx.x < N) { // the bias gradient: one contiguous row of dzT
         const bf16x8_t* row = reinterpret_cast<const bf16x8_t*>(dzT + threadIdx.x * kLdS);
@@ -229,22 +288,18 @@ __device__ __forceinline__ void backLayer(const NrcNet& net, const __bf16* hinT,
         db += s;
     }
     if constexpr (L > 0) {
-        // the 2 row tiles x K/16 column tiles of the input gradient, 2 (K = 64) or 3 (K = 96) per wave
-        const __bf16* W = net.wb + wbOff(L);
-        const int rt = wave & 1;
+        static_assert(K / 16 * 2 == kTrainWaves, "one input-gradient tile per wave");
+        const int rt = wave & 1, kt = wave >> 1;
+        f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
+        c = mmaReg<KS>(dz + rt * 16 * kLdH, kLdH, wT, c);
+        const int col = kt * 16 + r;
 #pragma unroll
-        for (int kt = wave >> 1; kt < K / 16; kt += 2) {
-            f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
-            c = mma16(dz + rt * 16 * kLdH, kLdH, W + kt * 16 * NP, NP, NP, c);
-            int col = kt * 16 + r;
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                int row = rt * 16 + g * 4 + t;
-                float h = (float)hinT[col * kLdS + row];
-                __bf16 v = (__bf16)(h > 0.0f ? c[t] : 0.0f);
-                dzNext[row * kLdH + col] = v;
-                dzNextT[col * kLdS + row] = v;
-            }
+        for (int t = 0; t < 4; t++) {
+            const int row = rt * 16 + g * 4 + t;
+            float h = (float)hinT[col * kLdS + row];
+            __bf16 v = (__bf16)(h > 0.0f ? c[t] : 0.0f);
+            dzNext[row * kLdH + col] = v;
+            dzNextT[col * kLdS + row] = v;
         }
     }
     __syncthreads();
@@ -253,16 +308,18 @@ __device__ __forceinline__ void backLayer(const NrcNet& net, const __bf16* hinT,
 template <int L, int NT>
 __device__ __forceinline__ void storeLayer(float* part, const f32x4_t (&acc)[NT], float db)
 {
-    constexpr int N = layerN(L);
-    constexpr int NTILE = N / 16;
+    constexpr int K = layerK(L), N = layerN(L);
+    constexpr int NTILE = N / 16, TILES = (K / 16) * NTILE;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
 #pragma unroll
     for (int j = 0; j < NT; j++) {
-        int t = wave + 4 * j;
-        int kt = t / NTILE, nt = t % NTILE;
+        const int t = wave + kTrainWaves * j;
+        if (t < TILES) {
+            const int kt = t / NTILE, nt = t % NTILE;
 #pragma unroll
-        for (int e = 0; e < 4; e++)
-            part[wOff(L) + (kt * 16 + g * 4 + e) * N + nt * 16 + r] = acc[j][e];
+            for (int e = 0; e < 4; e++)
+                part[wOff(L) + (kt * 16 + g * 4 + e) * N + nt * 16 + r] = acc[j][e];
+        }
     }
     if ((int)threadIdx.x < N)
         part[bOff(L) + threadIdx.x] = db;
@@ -277,10 +334,11 @@ __host__ __device__ __forceinline__ void sliceRange(uint32_t n, uint32_t slice, 
 
 // Fused forward + backward over 32-row tiles of training samples (4 float4 each: 3 of inputs, the
 // RGB target), over slice `slice` of `slices` of them. Writes this block's gradient partial sums
-// (kParams floats) and loss sum. Every product is split over the 4 waves by 16-row tile and
+// (kParams floats) and loss sum. Every product is split over the 8 waves by 16-row tile and
 // column tile, with a block barrier between layers.
-__global__ __launch_bounds__(256, 2) void k_nrc_train(NrcNet net, const float4* samples, const uint32_t* countPtr, uint32_t countImm,
-                                                      uint32_t cap, uint32_t slice, uint32_t slices, float* partials, float* lossPartials)
+__global__ __launch_bounds__(64 * kTrainWaves, 1) void k_nrc_train(NrcNet net, const float4* samples, const uint32_t* countPtr,
+                                                                   uint32_t countImm, uint32_t cap, uint32_t slice, uint32_t slices,
+                                                                   float* partials, float* lossPartials)
 {
     __shared__ __attribute__((aligned(16))) __bf16 s_x[kTrainRows * kLdX];     // encoded inputs, row-major
     __shared__ __attribute__((aligned(16))) __bf16 s_xT[kIn * kLdS];           // and transposed
@@ -290,7 +348,7 @@ __global__ __launch_bounds__(256, 2) void k_nrc_train(NrcNet net, const float4* 
     __shared__ float s_t[kTrainRows][4];                                        // RGB targets, weight
     __shared__ __attribute__((aligned(16))) __bf16 s_dz[2][kTrainRows * kLdH];
     __shared__ __attribute__((aligned(16))) __bf16 s_dzT[2][kHid * kLdS];
-    __shared__ float s_loss[256];
+    __shared__ float s_loss[64 * kTrainWaves];
     uint32_t total = countPtr ? *countPtr : countImm;
     if (total > cap)
         total = cap;
@@ -298,7 +356,27 @@ __global__ __launch_bounds__(256, 2) void k_nrc_train(NrcNet net, const float4* 
     sliceRange(total, slice, slices, first, n); // n: one past the slice's last sample
     const float invB = 1.0f / (float)(n > first ? n - first : 1);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
-    f32x4_t a0[6], a1[4], a2[4], a3[4], a4[4], a5[1];
+    const int nt = wave >> 1; // this wave's column tile of the forward products, and (kt) of the input gradients
+    // this wave's weight fragments: forward B = W^T [n][k] (column tile nt of each layer), backward
+    // B = W [k][npad] (column tile kt = nt of each input gradient); biases of column nt 16 + r
+    bf16x8_t f0[3], f1[2], f2[2], f3[2], f4[2], f5[2] = {}, b1[2], b2[2], b3[2], b4[2], b5[1];
+    loadFrag<3>(net.wt + wtOff(0) + nt * 16 * layerK(0), layerK(0), f0);
+    loadFrag<2>(net.wt + wtOff(1) + nt * 16 * kHid, kHid, f1);
+    loadFrag<2>(net.wt + wtOff(2) + nt * 16 * kHid, kHid, f2);
+    loadFrag<2>(net.wt + wtOff(3) + nt * 16 * kHid, kHid, f3);
+    loadFrag<2>(net.wt + wtOff(4) + nt * 16 * kHid, kHid, f4);
+    if (nt == 0)
+        loadFrag<2>(net.wt + wtOff(5), kHid, f5);
+    loadFrag<2>(net.wb + wbOff(1) + nt * 16 * layerNpad(1), layerNpad(1), b1);
+    loadFrag<2>(net.wb + wbOff(2) + nt * 16 * layerNpad(2), layerNpad(2), b2);
+    loadFrag<2>(net.wb + wbOff(3) + nt * 16 * layerNpad(3), layerNpad(3), b3);
+    loadFrag<2>(net.wb + wbOff(4) + nt * 16 * layerNpad(4), layerNpad(4), b4);
+    loadFrag<1>(net.wb + wbOff(5) + nt * 16 * layerNpad(5), layerNpad(5), b5);
+    float bias[kNL];
+#pragma unroll
+    for (int l = 0; l < kNL; l++)
+        bias[l] = nt * 16 < layerN(l) ? net.theta[bOff(l) + nt * 16 + r] : 0.0f;
+    f32x4_t a0[3], a1[2], a2[2], a3[2], a4[2], a5[1];
     float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f, d3 = 0.0f, d4 = 0.0f, d5 = 0.0f;
     for (auto& v : a0) v = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
     for (auto& v : a1) v = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
@@ -307,7 +385,6 @@ __global__ __launch_bounds__(256, 2) void k_nrc_train(NrcNet net, const float4* 
     for (auto& v : a4) v = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
     a5[0] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
     float lossAcc = 0.0f;
-    const int rt = wave & 1; // this wave's 16-row tile in the forward products
     for (uint32_t tile = first + blockIdx.x * kTrainRows; tile < n; tile += gridDim.x * kTrainRows) {
         // encode (waves 0-1: one row and a quarter of its inputs per lane), row-major and transposed
         if (wave < 2) {
@@ -334,34 +411,12 @@ __global__ __launch_bounds__(256, 2) void k_nrc_train(NrcNet net, const float4* 
                 s_xT[(g * 24 + k) * kLdS + row] = xr[k];
         }
         __syncthreads();
-        // forward: wave w takes row tile w & 1 and the column tiles w >> 1, (w >> 1) + 2 of each
-        // layer (the output layer's one column tile on waves 0-1); B operands stream from the global
-        // bf16 copy (L2-resident, 47 KB)
-#pragma unroll
-        for (int l = 0; l < kNL; l++) {
-            const int K = layerK(l), N = layerN(l);
-            const __bf16* in = l == 0 ? s_x : s_hr[(l - 1) & 1];
-            const int ldin = l == 0 ? kLdX : kLdH;
-#pragma unroll
-            for (int nt = wave >> 1; nt < N / 16; nt += 2) {
-                f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
-                c = mma16(in + rt * 16 * ldin, ldin, net.wt + wtOff(l) + nt * 16 * K, K, K, c);
-                int col = nt * 16 + r;
-                float bias = net.theta[bOff(l) + col];
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    int rw = rt * 16 + g * 4 + t;
-                    if (l < kNL - 1) {
-                        __bf16 v = (__bf16)fmaxf(c[t] + bias, 0.0f);
-                        s_hr[l & 1][rw * kLdH + col] = v;
-                        s_hT[l][col * kLdS + rw] = v;
-                    } else if (col < 3) {
-                        s_y[rw][col] = c[t] + bias;
-                    }
-                }
-            }
-            __syncthreads();
-        }
+        fwdLayer<0, 3>(s_x, kLdX, f0, bias[0], s_hr[0], s_hT[0], s_y);
+        fwdLayer<1, 2>(s_hr[0], kLdH, f1, bias[1], s_hr[1], s_hT[1], s_y);
+        fwdLayer<2, 2>(s_hr[1], kLdH, f2, bias[2], s_hr[0], s_hT[2], s_y);
+        fwdLayer<3, 2>(s_hr[0], kLdH, f3, bias[3], s_hr[1], s_hT[3], s_y);
+        fwdLayer<4, 2>(s_hr[1], kLdH, f4, bias[4], s_hr[0], s_hT[4], s_y);
+        fwdLayer<5, 2>(s_hr[0], kLdH, f5, bias[5], nullptr, nullptr, s_y);
         // relative L2 loss gradient, d(loss)/dy = 2 (y - t) / (y^2 + 0.01) / B for RGB
         if (threadIdx.x < kTrainRows) {
             int i = threadIdx.x;
@@ -380,24 +435,27 @@ __global__ __launch_bounds__(256, 2) void k_nrc_train(NrcNet net, const float4* 
             }
         }
         __syncthreads();
-        backLayer<5, 1>(net, s_hT[4], s_dz[0], s_dzT[0], s_dz[1], s_dzT[1], a5, d5);
-        backLayer<4, 4>(net, s_hT[3], s_dz[1], s_dzT[1], s_dz[0], s_dzT[0], a4, d4);
-        backLayer<3, 4>(net, s_hT[2], s_dz[0], s_dzT[0], s_dz[1], s_dzT[1], a3, d3);
-        backLayer<2, 4>(net, s_hT[1], s_dz[1], s_dzT[1], s_dz[0], s_dzT[0], a2, d2);
-        backLayer<1, 4>(net, s_hT[0], s_dz[0], s_dzT[0], s_dz[1], s_dzT[1], a1, d1);
-        backLayer<0, 6>(net, s_xT, s_dz[1], s_dzT[1], s_dz[0], s_dzT[0], a0, d0);
+        backLayer<5, 1, 1>(s_hT[4], s_dz[0], s_dzT[0], s_dz[1], s_dzT[1], b5, a5, d5);
+        backLayer<4, 2, 2>(s_hT[3], s_dz[1], s_dzT[1], s_dz[0], s_dzT[0], b4, a4, d4);
+        backLayer<3, 2, 2>(s_hT[2], s_dz[0], s_dzT[0], s_dz[1], s_dzT[1], b3, a3, d3);
+        backLayer<2, 2, 2>(s_hT[1], s_dz[1], s_dzT[1], s_dz[0], s_dzT[0], b2, a2, d2);
+        backLayer<1, 2, 2>(s_hT[0], s_dz[0], s_dzT[0], s_dz[1], s_dzT[1], b1, a1, d1);
+        backLayer<0, 3, 1>(s_xT, s_dz[1], s_dzT[1], s_dz[0], s_dzT[0], b5, a0, d0); // (no input gradient: b5 unused)
     }
     float* part = partials + (size_t)blockIdx.x * kParams;
-    storeLayer<0, 6>(part, a0, d0);
-    storeLayer<1, 4>(part, a1, d1);
-    storeLayer<2, 4>(part, a2, d2);
-    storeLayer<3, 4>(part, a3, d3);
-    storeLayer<4, 4>(part, a4, d4);
+    storeLayer<0, 3>(part, a0, d0);
+    storeLayer<1, 2>(part, a1, d1);
+    storeLayer<2, 2>(part, a2, d2);
+    storeLayer<3, 2>(part, a3, d3);
+    storeLayer<4, 2>(part, a4, d4);
     storeLayer<5, 1>(part, a5, d5);
     s_loss[threadIdx.x] = lossAcc;
     __syncthreads();
-    if (threadIdx.x < 64) { // fixed order: four per lane, then a butterfly (lane 0's is kept)
-        float sl = ((s_loss[threadIdx.x] + s_loss[threadIdx.x + 64]) + s_loss[threadIdx.x + 128]) + s_loss[threadIdx.x + 192];
+    if (threadIdx.x < 64) { // fixed order: kTrainWaves per lane, then a butterfly (lane 0's is kept)
+        float sl = 0.0f;
+#pragma unroll
+        for (int w = 0; w < kTrainWaves; w++)
+            sl += s_loss[threadIdx.x + 64 * w];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1)
             sl += __shfl_xor(sl, off, 64);
@@ -466,7 +524,10 @@ struct NrcAverage {
     float correction; // 1 - decay^step
 };
 
-constexpr uint32_t kTrainBlocks = 512; // two blocks per CU
+#ifndef VH_NRC_TRAIN_BLOCKS
+#define VH_NRC_TRAIN_BLOCKS 256
+#endif
+constexpr uint32_t kTrainBlocks = VH_NRC_TRAIN_BLOCKS; // one 8-wave block per CU
 constexpr uint32_t kReduceGroups = (kTrainBlocks + kReduceGroup - 1) / kReduceGroup;
 
 struct AdamStep {
@@ -676,7 +737,7 @@ int train_steps(vh_nrc* nn, const float4* samples, const uint32_t* countPtr, uin
     NrcNet net = nn->net(countPtr == nullptr);
     for (uint32_t k = 0; k < steps; k++) {
         const uint32_t slice = slices > 1 ? k % slices : 0u, nsl = slices > 1 ? slices : 1u;
-        k_nrc_train<<<kTrainBlocks, 256, 0, s>>>(net, samples, countPtr, countImm, cap, slice, nsl, nn->partials.as<float>(),
+        k_nrc_train<<<kTrainBlocks, 64 * kTrainWaves, 0, s>>>(net, samples, countPtr, countImm, cap, slice, nsl, nn->partials.as<float>(),
                                                  nn->lossPart.as<float>());
         k_nrc_reduce_groups<<<dim3((kParams / 4 + 255) / 256, kReduceGroups), 256, 0, s>>>(nn->partials.as<float>(), kTrainBlocks,
                                                                                        nn->groupSums.as<float>());
@@ -1064,8 +1125,12 @@ int vh_nrc_accumulate(vh_nrc* nn, vh_accumulation* a, uint32_t frames)
             return fail(VH_ERR_KERNEL, std::string("NRC inference: ") + hipGetErrorString(e));
         HIPTRY(hipEventRecord(nn->inferDone, s));
         inferIssued = true;
-        // the depths past the cache carry only the paths it could not end (none on most scenes):
-        // small grids, no spreading, so their near-empty launches cost little
+        // the depths past the cache carry only the paths it could not end: those that passed a
+        // surface of opacity < 1 (none in a scene without such materials, where every path that
+        // reaches the cache depth alive ends there); small grids, no spreading, so their near-empty
+        // launches cost little
+        if (!S.translucent)
+            return VH_OK;
         Wavefront WT = W;
         WT.traceWaves = 0;
         const LaunchCfg tail{s, std::min<uint32_t>(cfg.wfGrid ? cfg.wfGrid : 1024u, r->ctx->cus)};
