@@ -140,17 +140,41 @@ constexpr uint32_t kLeafGroup = 1u << kLeafBits;
 // test would take for an infinite box, so the traversal skips it by this reference.
 constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
 
-// Box test of the four children: returns a 4-bit mask of hits and their entry distances. `near`
-// holds, per axis, which of the node's lo / hi rows the ray enters through (bit a: the ray's
-// inverse direction on axis a is negative, so it enters through hi), so each axis takes its entry
-// and exit planes by address instead of a min and a max per child: the same distances as
-// min(lo, hi) / max(lo, hi) of the two slab distances for every box (lo <= hi), and an empty slot
-// (lo = +inf, hi = -inf) never hits.
-__device__ __forceinline__ uint32_t slab4(const float4* n, uint32_t near, float ix, float iy, float iz, float ox, float oy, float oz,
-                                          float tmax, float tn[4])
+// One node's eight 16-byte words, the box rows ordered by the ray's entry side: per axis the row
+// of the planes the ray enters through (e) and leaves through (f). `near` holds, per axis, which of
+// the node's lo / hi rows the ray enters through (bit a: the ray's inverse direction on axis a is
+// negative, so it enters through hi); taking each axis's planes by address instead of a min and a
+// max per child gives the same distances as min(lo, hi) / max(lo, hi) of the two slab distances for
+// every box (lo <= hi), and an empty slot (lo = +inf, hi = -inf) never hits. All eight loads are
+// issued before any of them is waited for (the scheduling barrier keeps the compiler from
+// interleaving the first uses, which would wait for the child words before the last box rows are
+// requested: two memory round trips per node instead of one).
+struct NodeRows {
+    float4 ex, fx, ey, fy, ez, fz;
+    uint4 child, count;
+};
+
+__device__ __forceinline__ NodeRows loadNode(const float4* n, uint32_t near)
 {
     const uint32_t sx = near & 1u, sy = (near >> 1) & 1u, sz = near >> 2;
-    float4 ex = n[sx], fx = n[1u - sx], ey = n[2u + sy], fy = n[3u - sy], ez = n[4u + sz], fz = n[5u - sz];
+    NodeRows r;
+    r.child = *reinterpret_cast<const uint4*>(n + 6);
+    r.count = *reinterpret_cast<const uint4*>(n + 7);
+    r.ex = n[sx];
+    r.fx = n[1u - sx];
+    r.ey = n[2u + sy];
+    r.fy = n[3u - sy];
+    r.ez = n[4u + sz];
+    r.fz = n[5u - sz];
+    __builtin_amdgcn_sched_barrier(0);
+    return r;
+}
+
+// Box test of the four children: returns a 4-bit mask of hits and their entry distances.
+__device__ __forceinline__ uint32_t slab4(const NodeRows& nr, float ix, float iy, float iz, float ox, float oy, float oz, float tmax,
+                                          float tn[4])
+{
+    const float4 ex = nr.ex, fx = nr.fx, ey = nr.ey, fy = nr.fy, ez = nr.ez, fz = nr.fz;
     float a[6][4] = {{ex.x, ex.y, ex.z, ex.w}, {fx.x, fx.y, fx.z, fx.w}, {ey.x, ey.y, ey.z, ey.w},
                      {fy.x, fy.y, fy.z, fy.w}, {ez.x, ez.y, ez.z, ez.w}, {fz.x, fz.y, fz.z, fz.w}};
     uint32_t mask = 0;
@@ -255,10 +279,10 @@ __device__ __forceinline__ bool traceStep(const Bvh& bvh, TraceState& s, uint32_
     Hit& h = s.h;
     const float4* n = bvh.nodes + 8 * (size_t)s.node;
     s.visited++;
+    const NodeRows nr = loadNode(n, s.near);
+    const uint4 child = nr.child, count = nr.count;
     float tn[4];
-    uint32_t mask = slab4(n, s.near, s.ix, s.iy, s.iz, s.ox, s.oy, s.oz, s.best, tn);
-    uint4 child = *reinterpret_cast<const uint4*>(n + 6);
-    uint4 count = *reinterpret_cast<const uint4*>(n + 7);
+    uint32_t mask = slab4(nr, s.ix, s.iy, s.iz, s.ox, s.oy, s.oz, s.best, tn);
     const float kInf = __builtin_inff();
     uint32_t tris = 0;
     float key[4];
@@ -427,10 +451,11 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
     if (live) {
         const float4* n = bvh.nodes + 8 * (size_t)s.node;
         s.visited++;
+        const NodeRows nr = loadNode(n, s.near);
+        child = nr.child;
+        const uint4 count = nr.count;
         float tn[4];
-        uint32_t mask = slab4(n, s.near, s.ix, s.iy, s.iz, s.ox, s.oy, s.oz, bh ? bt : s.tmax, tn);
-        child = *reinterpret_cast<const uint4*>(n + 6);
-        uint4 count = *reinterpret_cast<const uint4*>(n + 7);
+        uint32_t mask = slab4(nr, s.ix, s.iy, s.iz, s.ox, s.oy, s.oz, bh ? bt : s.tmax, tn);
 #pragma unroll
         for (uint32_t c = 0; c < 4; c++) {
             bool hitc = (mask >> c & 1u) != 0 && sel4(child, c) != kEmptyChild;
