@@ -510,7 +510,17 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
             const uint32_t e = q < total ? (uint32_t)sh.list[q] : 0u;
             const uint32_t owner = e >> (2 + kLeafBits);
             const uint32_t c = (e >> kLeafBits) & 3u, k = e & (kLeafGroup - 1u);
-            // the listing lane's ray, limit and child references from its registers (all 64 lanes here)
+            // from the listing lane's registers (all 64 lanes here): its child references first, so
+            // the triangle's fetch waits on them alone, then its ray and limit, which arrive while
+            // the fetch is in flight
+            const uint32_t c0 = fromLane(owner, child.x), c1 = fromLane(owner, child.y), c2 = fromLane(owner, child.z),
+                           c3 = fromLane(owner, child.w);
+            const uint32_t first = (c & 2u) != 0 ? ((c & 1u) != 0 ? c3 : c2) : ((c & 1u) != 0 ? c1 : c0);
+            // every lane fetches (lanes past the list take triangle 0, which exists whenever a test
+            // was listed), so no branch joins the loaded words and the first wait is at their use
+            const float4* tr = bvh.tris + 3 * (size_t)(q < total ? first + k : 0u);
+            const float4 a = tr[0], bb = tr[1], cc = tr[2];
+            __builtin_amdgcn_sched_barrier(0); // the ray's permutes stay behind the fetch
             const V3 o = v3(fromLane(owner, s.o.x), fromLane(owner, s.o.y), fromLane(owner, s.o.z));
             const float olim = fromLane(owner, lim), otmax = fromLane(owner, s.tmax);
             const uint32_t oray = fromLane(owner, s.ray);
@@ -519,8 +529,6 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
             rp.sy = fromLane(owner, s.pre.sy);
             rp.sz = fromLane(owner, s.pre.sz);
             const uint32_t ax = fromLane(owner, s.axes);
-            const uint32_t c0 = fromLane(owner, child.x), c1 = fromLane(owner, child.y), c2 = fromLane(owner, child.z),
-                           c3 = fromLane(owner, child.w);
             bool found = false;
             unsigned long long mine = 0;
             uint2 w = make_uint2(0u, 0u);
@@ -528,9 +536,6 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
                 rp.kx = (int)(ax & 3u);
                 rp.ky = (int)((ax >> 2) & 3u);
                 rp.kz = (int)((ax >> 4) & 3u);
-                const uint32_t first = c == 0 ? c0 : (c == 1 ? c1 : (c == 2 ? c2 : c3));
-                const float4* tr = bvh.tris + 3 * (size_t)(first + k);
-                float4 a = tr[0], bb = tr[1], cc = tr[2];
                 float t, b1, b2;
                 if (intersect<false>(rp, o, xyz(a), xyz(bb), xyz(cc), olim, t, b1, b2) && t < otmax) {
                     found = true;
