@@ -114,6 +114,27 @@ __device__ __forceinline__ V3 transformPoint(const uint32_t* I, uint32_t base, V
     return v3(r[0], r[1], r[2]);
 }
 
+// The same two transforms on a row-major 3 x 4 matrix already in registers (same operations, same
+// order, same bits).
+__device__ __forceinline__ V3 transformPoint(const float (&X)[12], V3 p)
+{
+    float r[3];
+#pragma unroll
+    for (uint32_t row = 0; row < 3; row++)
+        r[row] = X[4 * row] * p.x + X[4 * row + 1] * p.y + X[4 * row + 2] * p.z + X[4 * row + 3];
+    return v3(r[0], r[1], r[2]);
+}
+
+__device__ __forceinline__ V3 transformNormal(const float (&X)[12], V3 n)
+{
+    V3 r0 = v3(X[0], X[1], X[2]);
+    V3 r1 = v3(X[4], X[5], X[6]);
+    V3 r2 = v3(X[8], X[9], X[10]);
+    V3 c0 = cross(r1, r2);
+    float s = dot(r0, c0) < 0.0f ? -1.0f : 1.0f;
+    return s * v3(dot(c0, n), dot(cross(r2, r0), n), dot(cross(r0, r1), n));
+}
+
 __device__ __forceinline__ V3 transformNormal(const uint32_t* I, uint32_t base, V3 n)
 {
     V3 r0 = v3(asf(I[base]), asf(I[base + 1]), asf(I[base + 2]));
@@ -1096,13 +1117,26 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
             alive = false;
             break;
         }
-        uint32_t inst = hq.inst * 16;
-        uint32_t tri = S.instances[inst + 1] + 3 * hq.prim;
-        uint32_t vo = S.instances[inst];
-        uint32_t i0 = S.indices[tri] + vo, i1 = S.indices[tri + 1] + vo, i2 = S.indices[tri + 2] + vo;
-        V3 p0 = transformPoint(S.instances, inst + 4, vtx(S.positions, i0));
-        V3 p1 = transformPoint(S.instances, inst + 4, vtx(S.positions, i1));
-        V3 p2 = transformPoint(S.instances, inst + 4, vtx(S.positions, i2));
+        // the instance record (vertex offset, first index, material, emitter base, then the 3 x 4
+        // transform), then the triangle's indices, then its vertices and vertex normals: each
+        // group's loads are issued together (a scheduling barrier after them), so the chain is three
+        // memory round trips, not one per index and vertex
+        const uint32_t inst = hq.inst * 16;
+        const uint4* IR = reinterpret_cast<const uint4*>(S.instances + inst);
+        const uint4 I0 = IR[0], I1 = IR[1], I2 = IR[2], I3 = IR[3];
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t tri = I0.y + 3 * hq.prim, vo = I0.x;
+        const uint32_t j0 = S.indices[tri], j1 = S.indices[tri + 1], j2 = S.indices[tri + 2];
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t i0 = j0 + vo, i1 = j1 + vo, i2 = j2 + vo;
+        const V3 q0 = vtx(S.positions, i0), q1 = vtx(S.positions, i1), q2 = vtx(S.positions, i2);
+        const V3 vn0 = vtx(S.vnormals, i0), vn1 = vtx(S.vnormals, i1), vn2 = vtx(S.vnormals, i2);
+        __builtin_amdgcn_sched_barrier(0);
+        const float X[12] = {asf(I1.x), asf(I1.y), asf(I1.z), asf(I1.w), asf(I2.x), asf(I2.y),
+                             asf(I2.z), asf(I2.w), asf(I3.x), asf(I3.y), asf(I3.z), asf(I3.w)};
+        V3 p0 = transformPoint(X, q0);
+        V3 p1 = transformPoint(X, q1);
+        V3 p2 = transformPoint(X, q2);
         V3 nOut = normalize(cross(p1 - p0, p2 - p0));
         V3 wo = -dir;
         bool outside = dot(nOut, wo) > 0.0f;
@@ -1116,7 +1150,8 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
         float b1 = (d22 * dot(e3, e1) - d12 * dot(e3, e2)) / den;
         float b2 = (d11 * dot(e3, e2) - d12 * dot(e3, e1)) / den;
 
-        float opacity = opacityAt(c, hq.inst, hq.prim, b1, b2);
+        // (every opacity is 1 in a scene without translucent materials)
+        float opacity = S.translucent ? opacityAt(c, hq.inst, hq.prim, b1, b2) : 1.0f;
         if (opacity < 1.0f && rnext(rng) >= opacity) {
             org = hit + dir * 1.0e-4f;
             break;
@@ -1134,16 +1169,16 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
             c.F.restir[restirScratch(pixel) + 22] = asf(1u);
         }
 
-        V3 nsObj = (1.0f - b1 - b2) * vtx(S.vnormals, i0) + b1 * vtx(S.vnormals, i1) + b2 * vtx(S.vnormals, i2);
+        V3 nsObj = (1.0f - b1 - b2) * vn0 + b1 * vn1 + b2 * vn2;
         V3 ns = nOut;
         if (dot(nsObj, nsObj) > 1.0e-12f) {
-            ns = normalize(transformNormal(S.instances, inst + 4, nsObj));
+            ns = normalize(transformNormal(X, nsObj));
             if (dot(ns, nOut) < 0.0f)
                 ns = -ns;
         }
         V3 n = outside ? ns : -ns;
 
-        const float* M = S.materials + S.instances[inst + 2] * 24;
+        const float* M = S.materials + I0.z * 24;
         V3 albedo = v3(M[0], M[1], M[2]);
         uint32_t texBase = f2u(M[10]), texMR = f2u(M[11]), texEm = f2u(M[12]), texNm = f2u(M[13]);
         float texRMx = 1.0f, texRMy = 1.0f;
@@ -1191,7 +1226,7 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
             le = v3(0.0f);
         if (le.x > 0.0f || le.y > 0.0f || le.z > 0.0f) {
             float wB = 1.0f;
-            uint32_t eb = S.instances[inst + 3];
+            uint32_t eb = I0.w;
             if (st.misPrev && eb != 0) {
                 const float* E = S.emitters + (size_t)(eb - 1 + hq.prim) * 16;
                 float cosl = fmaxf(fabsf(dot(nOut, dir)), 1.0e-4f);
