@@ -78,6 +78,13 @@ __global__ __launch_bounds__(256) void k_world(const float* __restrict__ positio
         worldTriangle(positions, indices, instances, instStart, ninst, g, out + 3 * (size_t)g);
 }
 
+// the (instance, primitive) of every global triangle id, from the world triangles' w words
+__global__ __launch_bounds__(256) void k_ids(const float4* __restrict__ world, uint32_t ntri, uint2* __restrict__ ids)
+{
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ntri; g += gridDim.x * blockDim.x)
+        ids[g] = make_uint2(__float_as_uint(world[3 * (size_t)g + 1].w), __float_as_uint(world[3 * (size_t)g + 2].w));
+}
+
 // refit: world triangles straight into leaf order, leaf slot k holding triangle perm[k]
 __global__ __launch_bounds__(256) void k_world_leaf(const float* __restrict__ positions, const uint32_t* __restrict__ indices,
                                                     const uint32_t* __restrict__ instances, const uint32_t* __restrict__ instStart,
@@ -835,6 +842,9 @@ int build_bvh(BvhBuild& out, const DeviceTables& T, hipStream_t s)
     VH_CHECK(world.alloc(sizeof(float4) * 3 * (size_t)n));
     VH_CHECK(out.tris.alloc(sizeof(float4) * 3 * (size_t)n));
     k_world<<<grid_for(n, 256), 256, 0, s>>>(T.positions, T.indices, T.instances, T.instStart, ninst, n, world.as<float4>());
+    VH_CHECK(hipGetLastError());
+    VH_CHECK(out.ids.alloc(sizeof(uint2) * (size_t)n));
+    k_ids<<<grid_for(n, 256), 256, 0, s>>>(world.as<float4>(), n, out.ids.as<uint2>());
     VH_CHECK(hipGetLastError());
     uint32_t host_bnd[8];
     TRY(scene_bounds(out, world.as<float4>(), n, s, bnd, host_bnd));

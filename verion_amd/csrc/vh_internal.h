@@ -76,6 +76,7 @@ struct BvhBuild {
     DevBuf nodes_buf;
     DevBuf tris;
     DevBuf perm;                  // leaf slot -> global triangle id
+    DevBuf ids;                   // global triangle id -> (instance, primitive)
     std::vector<uint32_t> levels; // wide node ids of level j: levels[j] .. levels[j + 1] - 1
     double sah = 0.0;             // surface-area cost of the tree as built
     uint32_t nodes = 0;
@@ -303,6 +304,7 @@ struct vh_renderer {
         s.texInfo = texInfo.as<uint32_t>();
         s.bvh.nodes = bvh.nodes_buf.as<float4>();
         s.bvh.tris = bvh.tris.as<float4>();
+        s.bvh.ids = bvh.ids.as<uint2>();
         s.bvh.ntri = bvh.ntri;
         s.bvh.counters = counters.as<unsigned long long>();
         s.translucent = translucent;

@@ -27,6 +27,7 @@ struct Hit {
 struct Bvh {
     const float4* nodes; // 8 float4 per node; node 0 is the root
     const float4* tris;  // 3 float4 per triangle, in leaf order
+    const uint2* ids;    // per global triangle id: instance, primitive
     uint32_t ntri;
     // Optional traversal counters (null when off): [0] closest-hit rays, [1] their nodes visited,
     // [2] their triangles tested, [3..5] the same for shadow (any-hit) rays, [6] rays that hit.
@@ -384,13 +385,12 @@ constexpr int kSpill = kStack - kShort;              // the rest, in the lane's 
 
 struct WaveTris {
     unsigned long long key[64];  // per ray (its owner lane): (t bits << 32) | global id of the best hit so far, ~0 for none
-    uint2 win[64];               // instance, primitive of that hit
     uint32_t stackN[kShort][64]; // the first kShort stack entries of each lane: node
     uint16_t stackT[kShort][64]; // and (closest hit) the upper half of its entry distance's bits
     uint16_t list[kWaveTests];   // pending tests: listing lane << (2 + kLeafBits) | child slot << kLeafBits | triangle;
                                  // while work is handed out, the offered entries: donor lane << 4 | stack index
 };
-static_assert(sizeof(WaveTris) == 7680, "LDS of one wave's traversal");
+static_assert(sizeof(WaveTris) == 7168, "LDS of one wave's traversal");
 
 __device__ __forceinline__ void waveSync()
 {
@@ -529,24 +529,14 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
             rp.sy = fromLane(owner, s.pre.sy);
             rp.sz = fromLane(owner, s.pre.sz);
             const uint32_t ax = fromLane(owner, s.axes);
-            bool found = false;
-            unsigned long long mine = 0;
-            uint2 w = make_uint2(0u, 0u);
             if (q < total) {
                 rp.kx = (int)(ax & 3u);
                 rp.ky = (int)((ax >> 2) & 3u);
                 rp.kz = (int)((ax >> 4) & 3u);
                 float t, b1, b2;
-                if (intersect<false>(rp, o, xyz(a), xyz(bb), xyz(cc), olim, t, b1, b2) && t < otmax) {
-                    found = true;
-                    mine = ((unsigned long long)asu(t) << 32) | asu(a.w); // t > 0: its bits order like the value
-                    w = make_uint2(asu(bb.w), asu(cc.w));
-                    atomicMin(&sh.key[oray], mine);
-                }
+                if (intersect<false>(rp, o, xyz(a), xyz(bb), xyz(cc), olim, t, b1, b2) && t < otmax)
+                    atomicMin(&sh.key[oray], ((unsigned long long)asu(t) << 32) | asu(a.w)); // t > 0: its bits order like the value
             }
-            waveSync();
-            if (found && sh.key[oray] == mine)
-                sh.win[oray] = w;
             waveSync();
         }
     }
@@ -638,8 +628,9 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
     }
 }
 
-// One ray per lane, traced by the whole wave together; a ray's best (t, id) lives in sh.key / sh.win
-// while the wave's lanes work on it.
+// One ray per lane, traced by the whole wave together; a ray's best (t, id) lives in sh.key while
+// the wave's lanes work on it, and the owner looks up the winner's instance and primitive by its id
+// at the end (one load per ray instead of a second barrier and an LDS write per triangle round).
 template <bool ANY>
 __device__ Hit traceWave(const Bvh& bvh, V3 o, V3 d, float tmax, bool active, WaveTris& sh)
 {
@@ -658,10 +649,10 @@ __device__ Hit traceWave(const Bvh& bvh, V3 o, V3 d, float tmax, bool active, Wa
     Hit& h = s.h;
     const unsigned long long kk = sh.key[lane];
     if (active && kk != ~0ull) {
-        const uint2 w = sh.win[lane];
         h.hit = true;
         h.t = asf((uint32_t)(kk >> 32));
         h.gid = (uint32_t)kk;
+        const uint2 w = bvh.ids[h.gid];
         h.inst = w.x;
         h.prim = w.y;
     }
