@@ -215,8 +215,15 @@ constexpr uint32_t kPersistWavesPerCu = 16; // 4 per SIMD, the kernel's occupanc
 constexpr uint32_t kTraceWavesPerCu = VH_TRACE_WAVES_PER_CU;
 int launch_megakernel_samples(const Scene& S, const Frame& F, const Params& P, uint32_t samples, float* staged, uint32_t* next,
                               float* shRec, uint32_t shK, uint32_t cus, hipStream_t s);
-// Paths in flight that fill the GPU: smaller images take several samples per dispatch in parallel.
-constexpr uint64_t kParallelPaths = 1u << 21;
+// Paths of one persistent-megakernel dispatch: a call's samples run up to this many (pixel, sample)
+// paths per dispatch in parallel (12 bytes of staged radiance each: 384 MB at most), so each
+// dispatch's tail (its last, longest paths) and its film addition are paid once per many samples.
+// The reference's 5M-triangle scene at 1024x768, 1,024 spp: 0.413 s with 2M paths per dispatch,
+// 0.201 s with 8M, 0.174 s with 16M, 0.162 s with 32M.
+#ifndef VH_PARALLEL_PATHS
+#define VH_PARALLEL_PATHS (1u << 25)
+#endif
+constexpr uint64_t kParallelPaths = VH_PARALLEL_PATHS;
 int launch_wf_sample(const Scene& S, const Frame& F, const Params& P, const Wavefront& W, bool restir, bool split, const LaunchCfg& cfg,
                      StageMarks* marks);
 // film[i] += pass[i] and pass[i] = 0 over n doubles: a pass in flight's FP64 film (each pixel's
