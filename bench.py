@@ -277,7 +277,7 @@ def main():
         line["cpu_baseline"] = base
         line["parity_full_size"] = parity
     if args.integrator == "wavefront":
-        line["megakernel_1080p"] = megakernel_same_scene(r, W, H)
+        line["megakernel_1080p"] = megakernel_same_scene(r, W, H, spp)
     if not args.no_cornell:
         line["cornell_256"] = cornell_256(ctx)
     if not args.no_nrc:
@@ -291,23 +291,28 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def megakernel_same_scene(r, W, H, steps=16):
+def megakernel_same_scene(r, W, H, spp, steps=8):
     """The other integrator on the same scene and image: the plain megakernel (persistent waves,
-    every ray traced wave-cooperatively), 1 spp per call like the timed steps, after 2 untimed."""
+    every ray traced wave-cooperatively), `spp` samples per call like the timed steps (a call's
+    samples run in parallel, 16 per dispatch at 1080p), after 2 untimed calls; and 1 spp per call."""
     from verion_amd.backend import MEGAKERNEL, WAVEFRONT
 
     r.set_integrator(MEGAKERNEL)
     r.set_stage_timing(False)
-    acc = r.new_accumulation(W, H)
-    for _ in range(2):
-        r.accumulate(acc, 1)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        r.accumulate(acc, 1)
-    dt = time.perf_counter() - t0
-    acc.close()
+    out = {}
+    for per_call, key in ((spp, "msamples_per_s"), (1, "msamples_per_s_1_spp_per_call")):
+        acc = r.new_accumulation(W, H)
+        for _ in range(2):
+            r.accumulate(acc, per_call)
+        n = steps if per_call > 1 else 16
+        t0 = time.perf_counter()
+        for _ in range(n):
+            r.accumulate(acc, per_call)
+        dt = time.perf_counter() - t0
+        acc.close()
+        out[key] = round(n * per_call * W * H / dt / 1e6, 1)
+    out["spp_per_call"] = spp
     r.set_integrator(WAVEFRONT)
-    out = {"msamples_per_s": round(steps * W * H / dt / 1e6, 1), "steps": steps}
     log(f"megakernel at {W}x{H}: {out}")
     return out
 
