@@ -777,30 +777,35 @@ int vh_accumulate(vh_renderer* r, vh_accumulation* a, uint32_t samples)
         // film j to the accumulation's film once pass k is done, in sample order, and pass k + depth
         // waits for that add before it reuses buffers and film j
         hipEvent_t fork = a->pipeEv[2 * kMaxPipeline];
-        HIPTRY(hipEventRecord(fork, s));
-        for (uint32_t j = 0; j < depth; j++)
-            HIPTRY(hipStreamWaitEvent(r->ctx->pipe[j], fork, 0));
+        rc = check_hip(hipEventRecord(fork, s), "hipEventRecord");
+        for (uint32_t j = 0; j < depth && rc == VH_OK; j++)
+            rc = check_hip(hipStreamWaitEvent(r->ctx->pipe[j], fork, 0), "hipStreamWaitEvent");
         for (uint32_t k = 0; k < samples && rc == VH_OK; k++) {
             const uint32_t j = k % depth;
             hipStream_t ps = r->ctx->pipe[j];
             hipEvent_t done = a->pipeEv[j], added = a->pipeEv[kMaxPipeline + j];
             if (k >= depth)
-                HIPTRY(hipStreamWaitEvent(ps, added, 0));
+                rc = check_hip(hipStreamWaitEvent(ps, added, 0), "hipStreamWaitEvent");
             Frame Fj = F;
             Fj.film = a->pipeFilm[j].as<double>();
             Params P = base;
             P.w[6] = a->samples + k;
             P.w[31] = (uint32_t)n;
             LaunchCfg cfg{ps, r->wfBlocks ? r->wfBlocks : std::max(256u, r->ctx->cus * 8)};
-            rc = launch_wf_sample(S, Fj, P, a->wavefront(false, j), false, false, cfg, nullptr);
-            if (rc != VH_OK)
-                break;
-            HIPTRY(hipEventRecord(done, ps));
-            HIPTRY(hipStreamWaitEvent(s, done, 0));
-            rc = launch_film_add_pass(F.film, Fj.film, 3 * n, s);
             if (rc == VH_OK)
-                HIPTRY(hipEventRecord(added, s));
+                rc = launch_wf_sample(S, Fj, P, a->wavefront(false, j), false, false, cfg, nullptr);
+            if (rc == VH_OK)
+                rc = check_hip(hipEventRecord(done, ps), "hipEventRecord");
+            if (rc == VH_OK)
+                rc = check_hip(hipStreamWaitEvent(s, done, 0), "hipStreamWaitEvent");
+            if (rc == VH_OK)
+                rc = launch_film_add_pass(F.film, Fj.film, 3 * n, s);
+            if (rc == VH_OK)
+                rc = check_hip(hipEventRecord(added, s), "hipEventRecord");
         }
+        if (rc != VH_OK) // nothing of this call may still run once it returns
+            for (uint32_t j = 0; j < depth; j++)
+                (void)hipStreamSynchronize(r->ctx->pipe[j]);
     } else if (wave) {
         Wavefront W = a->wavefront(split);
         LaunchCfg cfg{s, r->wfBlocks ? r->wfBlocks : std::max(256u, r->ctx->cus * 8)};
