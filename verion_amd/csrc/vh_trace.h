@@ -545,17 +545,21 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
         if (ANY && nh) {
             live = false;
         } else {
-            // pop the next node that can still hold a closer hit
+            // pop the next node that can still hold a closer hit; the LDS entry is read whatever
+            // the depth (at a clamped index) and the overflow only past kShort, so the common read
+            // stays an LDS read rather than a generic-address load of either array
             live = false;
             while (s.sp > 0) {
                 s.sp--;
-                uint32_t node;
+                const int li = s.sp < kShort ? s.sp : kShort - 1;
+                uint32_t node = sh.stackN[li][lane];
+                // (an empty asm that takes the value keeps the read here: sunk past the loop it
+                // became one load through a pointer to either array, a generic-address load)
+                asm volatile("" : "+v"(node));
                 float t = 0.0f;
-                if (s.sp < kShort) {
-                    node = sh.stackN[s.sp][lane];
-                    if (!ANY)
-                        t = asf((uint32_t)sh.stackT[s.sp][lane] << 16);
-                } else {
+                if (!ANY)
+                    t = asf((uint32_t)sh.stackT[li][lane] << 16);
+                if (s.sp >= kShort) {
                     node = spillN[s.sp - kShort];
                     if (!ANY)
                         t = spillT[s.sp - kShort];
