@@ -139,11 +139,30 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_nrc_infer(NrcNet net, cons
     // wave-uniform loop over this wave's 16-query tiles
     for (uint32_t tile = (blockIdx.x * kInferWaves + wave) * 16u; tile < n; tile += gridDim.x * kInferWaves * 16u) {
         const uint32_t q = tile + r; // this lane's query in the B operand and the result
-        bool valid = q < n && (MODE == 0 || rec[(size_t)q * stride + 4].w != 0.0f);
+        // the whole record at once (no load waits on another), and in mode 1 the pixel's film
+        // while the network runs (each pixel has at most one query, so nothing else writes it)
+        float4 q0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), q1 = q0, q2 = q0, q3 = q0, q4 = q0;
+        if (q < n) {
+            const float4* qr = rec + (size_t)q * stride;
+            q0 = qr[0];
+            q1 = qr[1];
+            q2 = qr[2];
+            if (MODE == 1) {
+                q3 = qr[3];
+                q4 = qr[4];
+            }
+        }
+        const bool valid = q < n && (MODE == 0 || q4.w != 0.0f);
+        double fp[3] = {0.0, 0.0, 0.0};
+        if (MODE == 1 && g == 0 && valid) {
+            const double* F = film + 3 * (size_t)asu(q1.w);
+            fp[0] = F[0];
+            fp[1] = F[1];
+            fp[2] = F[2];
+        }
         __bf16* a0 = s_act[wave][0] + r * kLdX + g * 24;
         if (valid) {
-            const float4* qr = rec + (size_t)q * stride;
-            encodePart(net, qr[0], qr[1], qr[2], g, a0);
+            encodePart(net, q0, q1, q2, g, a0);
         } else {
             for (int k = 0; k < 24; k++)
                 a0[k] = (__bf16)0.0f;
@@ -171,17 +190,12 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_nrc_infer(NrcNet net, cons
 #pragma unroll
                         for (int ch = 0; ch < 3; ch++)
                             out[(size_t)q * 3 + ch] = c[ch] + s_b[bo + ch];
-                    } else {
-                        const float4* qr = rec + (size_t)q * stride;
-                        float4 q4 = qr[4];
-                        if (q4.w != 0.0f) {
-                            uint32_t pixel = asu(qr[1].w);
-                            float4 thr = qr[3];
-                            const float tv[3] = {thr.x, thr.y, thr.z}, tot[3] = {q4.x, q4.y, q4.z};
+                    } else if (valid) {
+                        double* F = film + 3 * (size_t)asu(q1.w);
+                        const float tv[3] = {q3.x, q3.y, q3.z}, tot[3] = {q4.x, q4.y, q4.z};
 #pragma unroll
-                            for (int ch = 0; ch < 3; ch++)
-                                film[3 * (size_t)pixel + ch] += (double)(tot[ch] + tv[ch] * fmaxf(c[ch] + s_b[bo + ch], 0.0f));
-                        }
+                        for (int ch = 0; ch < 3; ch++)
+                            F[ch] = fp[ch] + (double)(tot[ch] + tv[ch] * fmaxf(c[ch] + s_b[bo + ch], 0.0f));
                     }
                 }
             }
