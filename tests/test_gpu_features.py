@@ -311,6 +311,26 @@ def test_passes_in_flight_do_not_change_the_film(ctx, scene):
         r.set_wavefront_pipeline(5)
 
 
+def test_many_samples_per_call_split_into_megakernel_dispatches_and_wavefront_passes(ctx):
+    # 128 x 128 at 2,100 spp in one call: 34.4M (pixel, sample) paths, more than one persistent
+    # megakernel dispatch holds (32M), and 2,100 wavefront passes 3 at a time; the same film bit
+    # for bit from both integrators, and from calls of 1,000 + 1,100 samples
+    r = Renderer(ctx, scenes.cornell())
+    films = []
+    for integ, calls in ((MEGAKERNEL, (2100,)), (WAVEFRONT, (2100,)), (MEGAKERNEL, (1000, 1100))):
+        r.set_integrator(integ)
+        acc = r.new_accumulation(128, 128)
+        for k in calls:
+            r.accumulate(acc, k)
+        assert acc.samples() == 2100
+        films.append(acc.film())
+        acc.close()
+    r.set_integrator(MEGAKERNEL)
+    assert (films[0] > 0).any()
+    for f in films[1:]:
+        assert np.array_equal(f, films[0])
+
+
 def test_many_delta_lights_shade_fused_and_match_the_oracle(ctx):
     # more than 7 delta lights: the wavefront shades with its shadow rays inline (no deferred records)
     from verion_amd import scene as sc
