@@ -380,7 +380,10 @@ __device__ __forceinline__ bool traceStep(const Bvh& bvh, TraceState& s, uint32_
 // would have skipped, never skips one that could hold the hit.
 
 constexpr uint32_t kWaveTests = 64 * 4 * kLeafGroup; // 64 lanes x 4 leaf children x kLeafMax triangles at most
-constexpr int kShort = 12;                           // stack entries per lane in LDS
+#ifndef VH_SHORT_STACK
+#define VH_SHORT_STACK 12
+#endif
+constexpr int kShort = VH_SHORT_STACK;               // stack entries per lane in LDS
 constexpr int kSpill = kStack - kShort;              // the rest, in the lane's overflow array
 
 struct WaveTris {
@@ -390,7 +393,7 @@ struct WaveTris {
     uint16_t list[kWaveTests];   // pending tests: listing lane << (2 + kLeafBits) | child slot << kLeafBits | triangle;
                                  // while work is handed out, the offered entries: donor lane << 4 | stack index
 };
-static_assert(sizeof(WaveTris) == 7168, "LDS of one wave's traversal");
+static_assert(sizeof(WaveTris) == 512 + 6 * 64 * kShort + 2048, "LDS of one wave's traversal");
 
 __device__ __forceinline__ void waveSync()
 {
