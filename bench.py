@@ -54,6 +54,19 @@ def traffic_from_profiles(kernel: str):
         return None
 
 
+def occupancy_from_profiles(kernel: str):
+    """Resident waves per CU measured by rocprofv3 --pmc OccupancyPercent (per depth and weighted by
+    launch time) and the resource that caps them, when a summary for this kernel is committed."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "occupancy_procedural_1080p.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        k = data.get(kernel)
+        return dict(k, unit=data["unit"], source=data["source"]) if k else None
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def algorithmic_bytes(c: dict, pixels: int, delta_lights: int, env: bool):
     """Algorithmic bytes of one sample pass, split by kernel, from the traversal counters.
 
@@ -250,6 +263,7 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "achieved_is": "algorithmic bytes (every node and triangle fetch counted) over launch time",
                 "hbm_measured": measured,
+                "occupancy": occupancy_from_profiles(dom),
                 "limiter": "instruction issue and dependent-fetch latency of the traversal: node and triangle "
                            "fetches hit L2 and the Infinity Cache, so measured HBM traffic stays far below peak",
                 "launch_ms": round(dom_s * 1e3, 4), "algorithmic_bytes_per_launch": int(dom_b),
