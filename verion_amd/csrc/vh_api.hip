@@ -803,9 +803,14 @@ int vh_accumulate(vh_renderer* r, vh_accumulation* a, uint32_t samples)
             if (rc == VH_OK)
                 rc = check_hip(hipEventRecord(added, s), "hipEventRecord");
         }
-        if (rc != VH_OK) // nothing of this call may still run once it returns
+        if (rc != VH_OK) { // nothing of this call may still run once it returns, and no pass film
+                           // may carry a pass that was never added into the next call
             for (uint32_t j = 0; j < depth; j++)
                 (void)hipStreamSynchronize(r->ctx->pipe[j]);
+            for (uint32_t j = 0; j < depth; j++)
+                (void)hipMemsetAsync(a->pipeFilm[j].p, 0, 24 * n, s);
+            (void)hipStreamSynchronize(s);
+        }
     } else if (wave) {
         Wavefront W = a->wavefront(split);
         LaunchCfg cfg{s, r->wfBlocks ? r->wfBlocks : std::max(256u, r->ctx->cus * 8)};
