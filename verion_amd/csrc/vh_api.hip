@@ -281,6 +281,38 @@ int upload_emitter_guide(vh_renderer* r, const vh_scene_tables* t)
     return reupload(r->ctx, r->emitGuide, r->emitGuideHost.data(), r->emitGuideHost.size());
 }
 
+// Cutpoints for envSample: the marginal CDF's guide (2^bm + 1 entries), then one guide per row of
+// the conditional CDFs (2^bc + 1 entries each). CDFs that are not non-decreasing (never made by the
+// packer) get guides spanning the whole table, i.e. the full binary searches.
+int upload_env_guide(vh_renderer* r, const vh_scene_tables* t)
+{
+    const uint32_t w = t->info.env_width, h = t->info.env_height;
+    std::vector<uint32_t>& g = r->envGuideHost;
+    if (w == 0 || h == 0) {
+        r->envGuideBits = 0;
+        g.assign(2, 0u);
+        return reupload(r->ctx, r->envGuide, g.data(), g.size());
+    }
+    uint32_t bm = cdf_guide_bits(h), bc = cdf_guide_bits(w);
+    auto fill = [&]() {
+        const uint32_t gm = (1u << bm) + 1, gc = (1u << bc) + 1;
+        g.assign(gm + (size_t)h * gc, 0u);
+        bool ok = cdf_guide(t->env_marginal, h, bm, g.data());
+        for (uint32_t j = 0; j < h && ok; j++)
+            ok = cdf_guide(t->env_cond + (size_t)j * w, w, bc, g.data() + gm + (size_t)j * gc);
+        return ok;
+    };
+    if (!fill()) {
+        bm = bc = 0;
+        g.assign(2 + 2 * (size_t)h, 0u);
+        g[1] = h - 1;
+        for (uint32_t j = 0; j < h; j++)
+            g[2 + 2 * (size_t)j + 1] = w - 1;
+    }
+    r->envGuideBits = bm | (bc << 8);
+    return reupload(r->ctx, r->envGuide, g.data(), g.size());
+}
+
 int upload_tables(vh_renderer* r, const vh_scene_tables* t)
 {
     std::vector<uint32_t> instStart;
@@ -307,6 +339,7 @@ int upload_tables(vh_renderer* r, const vh_scene_tables* t)
     TRY(upload(c, r->envPixels, t->env_pixels, t->env_pixels_len));
     TRY(upload(c, r->envMarginal, t->env_marginal, t->env_marginal_len));
     TRY(upload(c, r->envCond, t->env_cond, t->env_cond_len));
+    TRY(upload_env_guide(r, t));
     TRY(upload(c, r->texels, t->texels, t->texels_len));
     TRY(upload(c, r->texInfo, t->tex_info, t->tex_info_len));
     TRY(upload(c, r->instStart, instStart.data(), instStart.size()));
@@ -531,6 +564,7 @@ static int update_tables(vh_renderer* r, const vh_scene_tables* t, const vh_upda
             TRY(reupload(c, r->envPixels, t->env_pixels, t->env_pixels_len));
             TRY(reupload(c, r->envMarginal, t->env_marginal, t->env_marginal_len));
             TRY(reupload(c, r->envCond, t->env_cond, t->env_cond_len));
+            TRY(upload_env_guide(r, t));
             r->info.env_width = t->info.env_width;
             r->info.env_height = t->info.env_height;
             r->info.env_pdf_scale = t->info.env_pdf_scale;

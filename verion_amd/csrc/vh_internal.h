@@ -265,9 +265,10 @@ struct vh_context {
 struct vh_renderer {
     vh_context* ctx = nullptr;
     vh::DevBuf positions, indices, uvs, vnormals, instances, materials, lights, emitters, emitCdf, envPixels, envMarginal,
-        envCond, texels, texInfo, instStart, emitGuide;
-    std::vector<uint32_t> emitGuideHost; // kept until the upload has been read
+        envCond, texels, texInfo, instStart, emitGuide, envGuide;
+    std::vector<uint32_t> emitGuideHost, envGuideHost; // kept until the upload has been read
     uint32_t emitGuideBits = 0;
+    uint32_t envGuideBits = 0; // marginal | row << 8
     vh::BvhBuild bvh;
     vh_scene_info info{};
     vh_view view{};
@@ -307,6 +308,8 @@ struct vh_renderer {
         s.envPixels = envPixels.as<float>();
         s.envMarginal = envMarginal.as<float>();
         s.envCond = envCond.as<float>();
+        s.envGuide = envGuide.as<uint32_t>();
+        s.envGuideBits = envGuideBits;
         s.texels = texels.as<float>();
         s.texInfo = texInfo.as<uint32_t>();
         s.bvh.nodes = bvh.nodes_buf.as<float4>();

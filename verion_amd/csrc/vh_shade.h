@@ -45,11 +45,13 @@ struct Scene {
     const float* envPixels;
     const float* envMarginal;
     const float* envCond;
+    const uint32_t* envGuide; // cutpoints of envMarginal, then of each row of envCond (envSample)
     const float* texels;
     const uint32_t* texInfo;
     Bvh bvh;
     uint32_t translucent; // some material has opacity < 1 or an opacity texture
     uint32_t emitGuideBits;
+    uint32_t envGuideBits; // bits of the marginal's guide | bits of a row's guide << 8
 };
 
 // Running sums of one accumulation.
@@ -280,10 +282,17 @@ __device__ __forceinline__ float envPdf(const Ctx& c, V3 d)
     return lum(envTexel(c, i, j)) * c.P.f(26);
 }
 
+// The environment's texel for draws (u1, u2): the row whose marginal CDF value first reaches u1,
+// then the column whose CDF in that row first reaches u2, or the last one (the binary searches of
+// pt_core.slang's environment sampling). Each search starts from the cutpoints of u's interval in
+// the CDF's guide table (Chen and Asau's method, as emitterIndex), which hold the answer between
+// them: the same texel in ~1-2 dependent loads per search instead of log2(h) and log2(w).
 __device__ V3 envSample(const Ctx& c, float u1, float u2, float u3, float u4, float& pdf, V3& le)
 {
     uint32_t w = c.P.w[24], h = c.P.w[25];
-    uint32_t lo = 0, hi = h - 1;
+    const uint32_t bm = c.S.envGuideBits & 0xffu, bc = c.S.envGuideBits >> 8;
+    const uint32_t km = umin((uint32_t)(u1 * (float)(1u << bm)), (1u << bm) - 1u);
+    uint32_t lo = c.S.envGuide[km], hi = c.S.envGuide[km + 1];
     while (lo < hi) {
         uint32_t mid = (lo + hi) / 2;
         if (c.S.envMarginal[mid] >= u1)
@@ -292,8 +301,10 @@ __device__ V3 envSample(const Ctx& c, float u1, float u2, float u3, float u4, fl
             lo = mid + 1;
     }
     uint32_t j = lo;
-    lo = 0;
-    hi = w - 1;
+    const uint32_t* g = c.S.envGuide + (1u << bm) + 1u + j * ((1u << bc) + 1u);
+    const uint32_t kc = umin((uint32_t)(u2 * (float)(1u << bc)), (1u << bc) - 1u);
+    lo = g[kc];
+    hi = g[kc + 1];
     while (lo < hi) {
         uint32_t mid = (lo + hi) / 2;
         if (c.S.envCond[j * w + mid] >= u2)
