@@ -355,6 +355,32 @@ def test_many_delta_lights_shade_fused_and_match_the_oracle(ctx):
     assert np.array_equal(mega.film(), st["film"])
 
 
+def test_environment_maps_with_dark_rows_and_columns_match_the_oracle(ctx):
+    # an importance-sampled map whose CDFs have plateaus: black rows (the marginal CDF repeats),
+    # black columns and all-black rows (a row's CDF stays at 0), a bright texel; the library
+    # starts each CDF search from guide-table cutpoints and must pick the oracle's texels
+    from verion_amd import scene as sc
+
+    w, h = 64, 24
+    px = np.full((h, w, 3), 0.2, np.float32)
+    px[:5] = 0.0
+    px[15:19] = 0.0
+    px[:, 10:30] = 0.0
+    px[8, 40] = (400.0, 300.0, 200.0)
+    s = scenes.sky_scene(scenes.icosphere(2), sc.BaseBsdf(base_color=(0.7, 0.6, 0.5), roughness=0.4))
+    s.lights.clear()
+    s.add_light(sc.EnvironmentMap(s.add_image(sc.Image(w, h, px.reshape(-1, 3))), 1.0))
+    r = Renderer(ctx, s)
+    o = oracle.OracleScene(r.packed)
+    want = o.render(params(r.packed, 48, 48), 16)["film"]
+    for integrator in (MEGAKERNEL, WAVEFRONT):
+        r.set_integrator(integrator)
+        acc = r.new_accumulation(48, 48)
+        r.accumulate(acc, 16)
+        got = acc.film()
+        assert np.array_equal(got, want), f"{integrator}: exact {np.mean(got == want):.4f}, rel {rel(got, want):.2e}"
+
+
 def test_stage_timing_keeps_the_image_and_books_each_launch(ctx, zoo):
     """Stage timing puts an event after every wavefront launch and changes nothing in the image.
     An opaque scene traces depth d's closest rays with depth d - 1's shadow rays in one launch
