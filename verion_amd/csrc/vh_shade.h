@@ -284,7 +284,7 @@ __device__ __forceinline__ float envPdf(const Ctx& c, V3 d)
 
 // The environment's texel for draws (u1, u2): the row whose marginal CDF value first reaches u1,
 // then the column whose CDF in that row first reaches u2, or the last one (the binary searches of
-// pt_core.slang:276-300). Each search starts from the cutpoints of u's interval in
+// pt_core.slang:274-306). Each search starts from the cutpoints of u's interval in
 // the CDF's guide table (Chen and Asau's method, as emitterIndex), which hold the answer between
 // them: the same texel in ~1-2 dependent loads per search instead of log2(h) and log2(w).
 __device__ V3 envSample(const Ctx& c, float u1, float u2, float u3, float u4, float& pdf, V3& le)
