@@ -408,6 +408,22 @@ __device__ __forceinline__ uint32_t lanesBelow(uint64_t m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Inclusive sum of v over the wave's lanes in lane order (every lane executes it): Hillis-Steele
+// steps inside each row of 16 lanes by DPP row shifts (lanes before the row's start read 0), then
+// row 0's total added to row 1 and row 2's to row 3 (row broadcast of lane 15), then lane 31's
+// total to rows 2 and 3 (row broadcast of lane 31). Six DPP adds where a sum of per-bit ballots
+// needs one ballot, two lane counts and a shift-add per bit.
+__device__ __forceinline__ uint32_t waveInclusiveSum(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true); // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true); // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true); // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true); // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false); // row_bcast:15 -> rows 1, 3
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false); // row_bcast:31 -> rows 2, 3
+    return v;
+}
+
 // v of lane `src` (every lane of the wave executes it: an inactive source lane would read as 0)
 __device__ __forceinline__ uint32_t fromLane(uint32_t src, uint32_t v) { return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v); }
 __device__ __forceinline__ float fromLane(uint32_t src, float v) { return asf(fromLane(src, asu(v))); }
@@ -494,15 +510,11 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
             }
         }
     }
-    // exclusive prefix of the per-lane test counts (at most 4 kLeafMax) over the wave
-    uint32_t cnt = (uint32_t)__builtin_popcount(tris);
-    uint32_t pre = 0, total = 0;
-#pragma unroll
-    for (uint32_t b = 0; b < 3 + kLeafBits; b++) {
-        uint64_t m = __ballot((cnt >> b) & 1u);
-        pre += lanesBelow(m) << b;
-        total += (uint32_t)__builtin_popcountll(m) << b;
-    }
+    // exclusive prefix of the per-lane test counts (at most 4 kLeafMax) over the wave, and the total
+    const uint32_t cnt = (uint32_t)__builtin_popcount(tris);
+    const uint32_t incl = waveInclusiveSum(cnt);
+    const uint32_t pre = incl - cnt;
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
     if (total != 0) {
         for (uint32_t m = tris, j = pre; m != 0; m &= m - 1u, j++)
             sh.list[j] = (uint16_t)((lane << (2 + kLeafBits)) | (uint32_t)__builtin_ctz(m));
