@@ -437,8 +437,7 @@ __device__ __forceinline__ float fromLane(uint32_t src, float v) { return asf(fr
 // or 4) within 1% of the best, 1 idle lane 1.5% and 32 2% slower.
 constexpr uint32_t kStealMinIdle = 16; // idle lanes a wave needs before it hands out work
 constexpr uint32_t kOfferMax = 2;      // stack entries one lane gives away per step
-constexpr uint32_t kOfferBits = 2;     // bits of an offer count (0..kOfferMax)
-static_assert(kOfferMax < (1u << kOfferBits) && kShort <= 16, "offers are counted in kOfferBits bits, entries named in 4");
+static_assert(kShort <= 16, "offered entries are named by 4-bit stack indices");
 
 // best (t, id) of ray `r` so far (sh.key[r], ~0 for none) as (hit, t)
 __device__ __forceinline__ bool bestOf(const WaveTris& sh, uint32_t r, float& t)
@@ -593,13 +592,9 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
     const uint64_t offM = __ballot(offer != 0u);
     const uint32_t nIdle = (uint32_t)__builtin_popcountll(idleM);
     if (offM != 0 && nIdle >= kStealMinIdle) {
-        uint32_t opre = 0, ototal = 0;
-#pragma unroll
-        for (uint32_t b = 0; b < kOfferBits; b++) {
-            uint64_t m = __ballot((offer >> b) & 1u);
-            opre += lanesBelow(m) << b;
-            ototal += (uint32_t)__builtin_popcountll(m) << b;
-        }
+        const uint32_t oincl = waveInclusiveSum(offer); // the branch is wave-uniform: every lane is here
+        const uint32_t opre = oincl - offer;
+        const uint32_t ototal = (uint32_t)__builtin_amdgcn_readlane((int)oincl, 63);
         // the offers in lane order go to the idle lanes in lane order
         const uint32_t take = opre >= nIdle ? 0u : (offer < nIdle - opre ? offer : nIdle - opre);
         for (uint32_t i = 0; i < take; i++)
