@@ -492,20 +492,35 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
         }
         // the inner children hit, nearest first (closest hit), are pushed farthest first; before the
         // triangle tests, which leaves their registers free (the pop after the tests skips what the
-        // tests' hits put out of reach, and an any-hit ray that hits ends with its pushes unread)
+        // tests' hits put out of reach, and an any-hit ray that hits ends with its pushes unread).
+        // Child i goes to slot sp + (children pushed before it: those hit with a larger index).
+        const uint32_t vm = (key[0] < kInf ? 1u : 0u) | (key[1] < kInf ? 2u : 0u) | (key[2] < kInf ? 4u : 0u) | (key[3] < kInf ? 8u : 0u);
+        if (__ballot(s.sp > kShort - 4) == 0) {
+            // every live lane has room for four entries in LDS: four stores without branches, a child
+            // not pushed going to the top slot, which lies above every pushed one (sp + 4 <= kShort)
 #pragma unroll
-        for (int i = 3; i >= 0; i--) {
-            if (key[i] < kInf && s.sp < kStack) { // the build bounds the depth: the stack cannot fill
-                if (s.sp < kShort) {
-                    sh.stackN[s.sp][lane] = val[i];
-                    if (!ANY) // an any-hit ray ends at its first hit: entry distances are never compared
-                        sh.stackT[s.sp][lane] = (uint16_t)(asu(key[i]) >> 16);
-                } else {
-                    spillN[s.sp - kShort] = val[i];
-                    if (!ANY)
-                        spillT[s.sp - kShort] = key[i];
+            for (int i = 0; i < 4; i++) {
+                const int slot = ((vm >> i) & 1u) != 0 ? s.sp + __builtin_popcount(vm >> (i + 1)) : kShort - 1;
+                sh.stackN[slot][lane] = val[i];
+                if (!ANY) // an any-hit ray ends at its first hit: entry distances are never compared
+                    sh.stackT[slot][lane] = (uint16_t)(asu(key[i]) >> 16);
+            }
+            s.sp += __builtin_popcount(vm);
+        } else {
+#pragma unroll
+            for (int i = 3; i >= 0; i--) {
+                if (((vm >> i) & 1u) != 0 && s.sp < kStack) { // the build bounds the depth: the stack cannot fill
+                    if (s.sp < kShort) {
+                        sh.stackN[s.sp][lane] = val[i];
+                        if (!ANY)
+                            sh.stackT[s.sp][lane] = (uint16_t)(asu(key[i]) >> 16);
+                    } else {
+                        spillN[s.sp - kShort] = val[i];
+                        if (!ANY)
+                            spillT[s.sp - kShort] = key[i];
+                    }
+                    s.sp++;
                 }
-                s.sp++;
             }
         }
     }
