@@ -582,12 +582,14 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
                 s.sp--;
                 const int li = s.sp < kShort ? s.sp : kShort - 1;
                 uint32_t node = sh.stackN[li][lane];
-                // (an empty asm that takes the value keeps the read here: sunk past the loop it
-                // became one load through a pointer to either array, a generic-address load)
-                asm volatile("" : "+v"(node));
-                float t = 0.0f;
+                uint32_t tb = 0;
                 if (!ANY)
-                    t = asf((uint32_t)sh.stackT[li][lane] << 16);
+                    tb = (uint32_t)sh.stackT[li][lane];
+                // (an empty asm that takes the values keeps both reads here, issued together: sunk
+                // past the loop the node read became one load through a pointer to either array, a
+                // generic-address load)
+                asm volatile("" : "+v"(node), "+v"(tb));
+                float t = asf(tb << 16);
                 if (s.sp >= kShort) {
                     node = spillN[s.sp - kShort];
                     if (!ANY)
