@@ -577,29 +577,43 @@ __device__ __forceinline__ void waveStep(const Bvh& bvh, TraceState& s, bool& li
             // pop the next node that can still hold a closer hit; the LDS entry is read whatever
             // the depth (at a clamped index) and the overflow only past kShort, so the common read
             // stays an LDS read rather than a generic-address load of either array
-            live = false;
-            while (s.sp > 0) {
+            // (an empty asm that takes the values keeps both reads where they are, issued together:
+            // sunk past the loop the node read became one load through a pointer to either array, a
+            // generic-address load)
+            auto top = [&](uint32_t& node, float& t) {
                 s.sp--;
                 const int li = s.sp < kShort ? s.sp : kShort - 1;
-                uint32_t node = sh.stackN[li][lane];
+                node = sh.stackN[li][lane];
                 uint32_t tb = 0;
                 if (!ANY)
                     tb = (uint32_t)sh.stackT[li][lane];
-                // (an empty asm that takes the values keeps both reads here, issued together: sunk
-                // past the loop the node read became one load through a pointer to either array, a
-                // generic-address load)
                 asm volatile("" : "+v"(node), "+v"(tb));
-                float t = asf(tb << 16);
+                t = asf(tb << 16);
                 if (s.sp >= kShort) {
                     node = spillN[s.sp - kShort];
                     if (!ANY)
                         t = spillT[s.sp - kShort];
                 }
-                if (!ANY && nh && t > bt)
-                    continue;
-                s.node = node;
-                live = true;
-                break;
+            };
+            live = false;
+            if (s.sp > 0) {
+                // the top entry, which most pops take; the loop only when it lies beyond the hit
+                uint32_t node;
+                float t;
+                top(node, t);
+                if (ANY || !(nh && t > bt)) {
+                    s.node = node;
+                    live = true;
+                } else {
+                    while (s.sp > 0) {
+                        top(node, t);
+                        if (nh && t > bt)
+                            continue;
+                        s.node = node;
+                        live = true;
+                        break;
+                    }
+                }
             }
         }
     }
