@@ -80,6 +80,7 @@ struct Surface {
     float metallic, alpha, transmission, ior, eta, fSel, tComp, coat, coatAlpha;
     V3 sheenColor;
     float sheenAlpha, sheenP, coatP;
+    V3 ms; // multiscatterFactor at the vertex's outgoing direction (the same for every lobe evaluation there)
 };
 
 struct Lobes {
@@ -430,7 +431,7 @@ __device__ Lobes evalLobes(const Ctx& c, const Surface& s, V3 woL, V3 wiL)
     float core = Dggx(hh.z, s.alpha) * G1(woL.z, s.alpha) * G1(wiL.z, s.alpha) / (4.0f * woL.z);
     float t = s.transmission;
     Lobes l;
-    l.specular = s.metallic * schlick(s.albedo, woh) * multiscatterFactor(c, s, woL.z) * core +
+    l.specular = s.metallic * schlick(s.albedo, woh) * s.ms * core +
                  (1.0f - s.metallic) * (t * fresnelDielectric(woh, s.eta) * s.tComp + (1.0f - t) * fresnelDielectric(woh, s.ior)) * core;
     l.diffuse = (1.0f - s.metallic) * (1.0f - t) * (1.0f - s.fSel) * s.albedo * (wiL.z / kPi);
     if (s.sheenP > 0.0f) {
@@ -1320,6 +1321,7 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
         float sheenR = fmaxf(M[17], 0.07f);
         surf.sheenAlpha = sheenR * sheenR;
         surf.sheenP = sheenMax > 0.0f ? sheenMax * sheenAlbedo(c, woL.z, sheenR) : 0.0f;
+        surf.ms = multiscatterFactor(c, surf, woL.z);
 
         // delta lights (pt_core.slang:1224-1282); they draw no random numbers
         for (uint32_t li = 0; li < (GIP == 1 ? 0u : lightCount); li++) {
@@ -1548,7 +1550,7 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
             float woh = dot(woL, h);
             wiL = 2.0f * woh * h - woL;
             ok = wiL.z > 0.0f;
-            wgt = schlick(albedo, woh) * multiscatterFactor(c, surf, woL.z) * G1(fmaxf(wiL.z, 0.0f), alpha);
+            wgt = schlick(albedo, woh) * surf.ms * G1(fmaxf(wiL.z, 0.0f), alpha);
         } else if (lobe < metallic + (1.0f - metallic) * transmission) {
             V3 h = sampleVndf(woL, alpha, u1, u2);
             float woh = dot(woL, h);
