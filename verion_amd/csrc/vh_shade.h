@@ -80,7 +80,8 @@ struct Surface {
     float metallic, alpha, transmission, ior, eta, fSel, tComp, coat, coatAlpha;
     V3 sheenColor;
     float sheenAlpha, sheenP, coatP;
-    V3 ms; // multiscatterFactor at the vertex's outgoing direction (the same for every lobe evaluation there)
+    V3 ms;         // multiscatterFactor at the vertex's outgoing direction (the same for every lobe evaluation there)
+    float g1o, g1oC; // G1 of the outgoing direction for the base and the coat roughness (likewise)
 };
 
 struct Lobes {
@@ -428,7 +429,7 @@ __device__ Lobes evalLobes(const Ctx& c, const Surface& s, V3 woL, V3 wiL)
 {
     V3 hh = normalize(woL + wiL);
     float woh = fmaxf(dot(woL, hh), 0.0f);
-    float core = Dggx(hh.z, s.alpha) * G1(woL.z, s.alpha) * G1(wiL.z, s.alpha) / (4.0f * woL.z);
+    float core = Dggx(hh.z, s.alpha) * s.g1o * G1(wiL.z, s.alpha) / (4.0f * woL.z);
     float t = s.transmission;
     Lobes l;
     l.specular = s.metallic * schlick(s.albedo, woh) * s.ms * core +
@@ -439,7 +440,7 @@ __device__ Lobes evalLobes(const Ctx& c, const Surface& s, V3 woL, V3 wiL)
         l.diffuse = l.diffuse * (1.0f - s.sheenP);
     }
     if (s.coat > 0.0f) {
-        float coreC = Dggx(hh.z, s.coatAlpha) * G1(woL.z, s.coatAlpha) * G1(wiL.z, s.coatAlpha) / (4.0f * woL.z);
+        float coreC = Dggx(hh.z, s.coatAlpha) * s.g1oC * G1(wiL.z, s.coatAlpha) / (4.0f * woL.z);
         l.specular = (1.0f - s.coatP) * l.specular + s.coat * fresnelDielectric(woh, 1.5f) * coreC;
         l.diffuse = l.diffuse * (1.0f - s.coatP);
     }
@@ -488,7 +489,7 @@ __device__ float reflectivePdf(const Surface& s, V3 woL, V3 wiL)
 {
     V3 hh = normalize(woL + wiL);
     float woh = fmaxf(dot(woL, hh), 0.0f);
-    float spec = G1(woL.z, s.alpha) * Dggx(hh.z, s.alpha) / (4.0f * woL.z);
+    float spec = s.g1o * Dggx(hh.z, s.alpha) / (4.0f * woL.z);
     float t = s.transmission;
     float pSpec = s.metallic + (1.0f - s.metallic) * (t * fresnelDielectric(woh, s.eta) + (1.0f - t) * s.fSel);
     float pDiff = (1.0f - s.metallic) * (1.0f - t) * (1.0f - s.fSel) * wiL.z / kPi;
@@ -496,7 +497,7 @@ __device__ float reflectivePdf(const Surface& s, V3 woL, V3 wiL)
     if (s.sheenP > 0.0f)
         q = (1.0f - s.sheenP) * q + s.sheenP * wiL.z / kPi;
     if (s.coat > 0.0f) {
-        float specC = G1(woL.z, s.coatAlpha) * Dggx(hh.z, s.coatAlpha) / (4.0f * woL.z);
+        float specC = s.g1oC * Dggx(hh.z, s.coatAlpha) / (4.0f * woL.z);
         q = (1.0f - s.coatP) * q + s.coatP * specC;
     }
     return q;
@@ -1322,6 +1323,8 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
         surf.sheenAlpha = sheenR * sheenR;
         surf.sheenP = sheenMax > 0.0f ? sheenMax * sheenAlbedo(c, woL.z, sheenR) : 0.0f;
         surf.ms = multiscatterFactor(c, surf, woL.z);
+        surf.g1o = G1(woL.z, alpha);
+        surf.g1oC = surf.coat > 0.0f ? G1(woL.z, surf.coatAlpha) : 0.0f;
 
         // delta lights (pt_core.slang:1224-1282); they draw no random numbers
         for (uint32_t li = 0; li < (GIP == 1 ? 0u : lightCount); li++) {
