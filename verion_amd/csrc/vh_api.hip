@@ -333,6 +333,8 @@ int upload_tables(vh_renderer* r, const vh_scene_tables* t)
     TRY(upload(c, r->instances, t->instances, t->instances_len));
     TRY(upload(c, r->materials, t->materials, t->materials_len));
     TRY(upload(c, r->lights, t->lights, t->lights_len));
+    HIPTRY(r->lightDirs.alloc(16 * (size_t)std::max<uint64_t>(t->info.light_count, 1)));
+    TRY(launch_light_dirs(r->lights.as<uint32_t>(), (uint32_t)t->info.light_count, r->lightDirs.as<float>(), c->stream));
     TRY(upload(c, r->emitters, t->emitters, t->emitters_len));
     TRY(upload(c, r->emitCdf, t->emit_cdf, t->emit_cdf_len));
     TRY(upload_emitter_guide(r, t));
@@ -555,6 +557,9 @@ static int update_tables(vh_renderer* r, const vh_scene_tables* t, const vh_upda
     }
     if (plan.lights) {
         TRY(reupload(c, r->lights, t->lights, t->lights_len));
+        if (r->lightDirs.bytes < 16 * (size_t)std::max<uint64_t>(t->info.light_count, 1))
+            HIPTRY(r->lightDirs.alloc(16 * (size_t)std::max<uint64_t>(t->info.light_count, 1)));
+        TRY(launch_light_dirs(r->lights.as<uint32_t>(), (uint32_t)t->info.light_count, r->lightDirs.as<float>(), c->stream));
         r->info.light_count = t->info.light_count;
         std::memcpy(r->info.sky, t->info.sky, sizeof(r->info.sky));
         r->info.virtual_light = t->info.virtual_light;

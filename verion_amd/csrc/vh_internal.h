@@ -230,6 +230,8 @@ int launch_wf_sample(const Scene& S, const Frame& F, const Params& P, const Wave
 // 0 + radiance, exact) added in the order the passes were issued, which gives the film's bits of
 // passes run one at a time.
 int launch_film_add_pass(double* film, double* pass, uint64_t n, hipStream_t s);
+// lightDirs[4 li .. 4 li + 2] = -normalize(direction words of light li), for the count delta lights
+int launch_light_dirs(const uint32_t* lights, uint32_t count, float* out, hipStream_t s);
 int launch_primary_hits(const Scene& S, const Params& P, uint4* out, hipStream_t s);
 int launch_nrc_train_paths(const Scene& S, const Params& P, const Wavefront& W, uint32_t* trainPix, uint32_t nTrain, uint32_t npix,
                            uint32_t frame, float4* vertices, float4* samples, uint32_t* sampleCount, uint32_t sampleCap,
@@ -265,7 +267,7 @@ struct vh_context {
 struct vh_renderer {
     vh_context* ctx = nullptr;
     vh::DevBuf positions, indices, uvs, vnormals, instances, materials, lights, emitters, emitCdf, envPixels, envMarginal,
-        envCond, texels, texInfo, instStart, emitGuide, envGuide;
+        envCond, texels, texInfo, instStart, emitGuide, envGuide, lightDirs;
     std::vector<uint32_t> emitGuideHost, envGuideHost; // kept until the upload has been read
     uint32_t emitGuideBits = 0;
     uint32_t envGuideBits = 0; // marginal | row << 8
@@ -301,6 +303,7 @@ struct vh_renderer {
         s.instances = instances.as<uint32_t>();
         s.materials = materials.as<float>();
         s.lights = lights.as<uint32_t>();
+        s.lightDirs = lightDirs.as<float>();
         s.emitters = emitters.as<float>();
         s.emitCdf = emitCdf.as<float>();
         s.emitGuide = emitGuide.as<uint32_t>();

@@ -1028,6 +1028,29 @@ int launch_wf_sample(const Scene& S, const Frame& F, const Params& P0, const Wav
     return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("wavefront launch: ") + hipGetErrorString(e));
 }
 
+// The unit direction a directional (or virtual) light's shadow rays take, -normalize(its words
+// 5-7), once per light instead of at every shading vertex: the same expression, so the same bits.
+__global__ void k_light_dirs(const uint32_t* __restrict__ lights, uint32_t count, float* __restrict__ out)
+{
+    for (uint32_t li = blockIdx.x * blockDim.x + threadIdx.x; li < count; li += gridDim.x * blockDim.x) {
+        const uint32_t* L = lights + li * 16;
+        const V3 d = -normalize(v3(asf(L[5]), asf(L[6]), asf(L[7])));
+        out[4 * li] = d.x;
+        out[4 * li + 1] = d.y;
+        out[4 * li + 2] = d.z;
+        out[4 * li + 3] = 0.0f;
+    }
+}
+
+int launch_light_dirs(const uint32_t* lights, uint32_t count, float* out, hipStream_t s)
+{
+    if (count == 0)
+        return VH_OK;
+    k_light_dirs<<<(count + 63) / 64, 64, 0, s>>>(lights, count, out);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? VH_OK : fail(VH_ERR_KERNEL, std::string("light directions: ") + hipGetErrorString(e));
+}
+
 int launch_film_add_pass(double* film, double* pass, uint64_t n, hipStream_t s)
 {
     k_film_add_pass<<<blocks_for(n / 2 + 1, 256, 2048), 256, 0, s>>>(film, pass, n);

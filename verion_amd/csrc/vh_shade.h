@@ -39,6 +39,7 @@ struct Scene {
     const uint32_t* instances;
     const float* materials;
     const uint32_t* lights;
+    const float* lightDirs; // per delta light, -normalize(its direction words) (4 floats; directional lights read it)
     const float* emitters;
     const float* emitCdf;
     const uint32_t* emitGuide; // 2^emitGuideBits + 1 cutpoints into emitCdf (emitterIndex)
@@ -1339,7 +1340,7 @@ __device__ bool shadePath(const Ctx& c, PathState& st, const Hit& hq, uint32_t d
             V3 wi, li_;
             float dist;
             if (L[0] == 1 || virt) {
-                wi = -normalize(vec);
+                wi = v3(S.lightDirs[4 * li], S.lightDirs[4 * li + 1], S.lightDirs[4 * li + 2]); // -normalize(vec)
                 dist = 1.0e30f;
                 li_ = col;
             } else {
