@@ -426,6 +426,14 @@ __device__ __forceinline__ float sheenCore(float woz, float wiz, float hz, float
     return d * v * wiz;
 }
 
+// t fresnelDielectric(woh, eta) (times tComp in the lobe values): skipped for an opaque surface
+// (t = 0) when eta is positive and finite, where the product is exactly 0 (the Fresnel term is then
+// finite and not negative); computed otherwise, so a degenerate eta gives the same bits as before
+__device__ __forceinline__ bool transmissionTerm(const Surface& s)
+{
+    return s.transmission != 0.0f || !(s.eta > 0.0f && s.eta <= 3.40282347e38f);
+}
+
 __device__ Lobes evalLobes(const Ctx& c, const Surface& s, V3 woL, V3 wiL)
 {
     V3 hh = normalize(woL + wiL);
@@ -433,8 +441,8 @@ __device__ Lobes evalLobes(const Ctx& c, const Surface& s, V3 woL, V3 wiL)
     float core = Dggx(hh.z, s.alpha) * s.g1o * G1(wiL.z, s.alpha) / (4.0f * woL.z);
     float t = s.transmission;
     Lobes l;
-    l.specular = s.metallic * schlick(s.albedo, woh) * s.ms * core +
-                 (1.0f - s.metallic) * (t * fresnelDielectric(woh, s.eta) * s.tComp + (1.0f - t) * fresnelDielectric(woh, s.ior)) * core;
+    const float ft = transmissionTerm(s) ? t * fresnelDielectric(woh, s.eta) * s.tComp : 0.0f;
+    l.specular = s.metallic * schlick(s.albedo, woh) * s.ms * core + (1.0f - s.metallic) * (ft + (1.0f - t) * fresnelDielectric(woh, s.ior)) * core;
     l.diffuse = (1.0f - s.metallic) * (1.0f - t) * (1.0f - s.fSel) * s.albedo * (wiL.z / kPi);
     if (s.sheenP > 0.0f) {
         l.specular = (1.0f - s.sheenP) * l.specular + s.sheenColor * sheenCore(woL.z, wiL.z, hh.z, s.sheenAlpha);
@@ -492,7 +500,8 @@ __device__ float reflectivePdf(const Surface& s, V3 woL, V3 wiL)
     float woh = fmaxf(dot(woL, hh), 0.0f);
     float spec = s.g1o * Dggx(hh.z, s.alpha) / (4.0f * woL.z);
     float t = s.transmission;
-    float pSpec = s.metallic + (1.0f - s.metallic) * (t * fresnelDielectric(woh, s.eta) + (1.0f - t) * s.fSel);
+    const float ft = transmissionTerm(s) ? t * fresnelDielectric(woh, s.eta) : 0.0f;
+    float pSpec = s.metallic + (1.0f - s.metallic) * (ft + (1.0f - t) * s.fSel);
     float pDiff = (1.0f - s.metallic) * (1.0f - t) * (1.0f - s.fSel) * wiL.z / kPi;
     float q = pSpec * spec + pDiff;
     if (s.sheenP > 0.0f)
